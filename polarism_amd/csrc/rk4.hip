@@ -1,0 +1,302 @@
+// Fused RK4-FDM stage kernels (scalar field, periodic 2-D grid).
+//
+// One launch evaluates one RK4 stage of the stage-coupled condensate +
+// reservoir system for every member of a packed member table
+// (rk4_fdm_solver.py:36-77 with Model.rhs model.py:89-127):
+//
+//   stage input   psi_s = psi0 + a_s dt k_{s-1},  n_s = n0 + a_s dt kn_{s-1}
+//   condensate    k = (-i/hbar)[-hbar^2/2m lap psi_s + (gC|psi_s|^2 + gR nA + gI nI) psi_s]
+//                     + (R nA - gammaC)/2 psi_s + eta nA hbar/2m lap psi_s
+//   reservoir     kn = reservoir RHS (single / double / quadratic-double)
+//   final stage   psi1 = psi0 + dt/6 (k1 + 2k2 + 2k3 + k4), n1 = max(n0 + ..., 0)
+//                 then the Wigner increment of the step (noise/stochastic.py:224-253).
+//
+// Layout: a workgroup of 256 threads (4 wavefronts) owns a 64 x TH tile of
+// one member.  The stage input psi_s of the tile plus a one-point halo is
+// formed once in LDS from coalesced loads of psi0 and k_{s-1}, so HBM sees each
+// array once per stage; the 5-point Laplacian then reads LDS.  Wavefront w
+// handles rows w, w+4, ... so every global access of a wavefront is one
+// contiguous 64-point row segment.  Arithmetic follows the reference's
+// operation order; the library is built with -ffp-contract=off.
+#include "common.hpp"
+
+namespace pgpe {
+
+template <typename T>
+struct Rk4Params {
+    const pgpe_member* table;
+    const int32_t* tile_member;
+    int num_tiles;
+    int noise;
+    T a;          // stage-input scale: 0, dt/2, dt/2, dt
+    T dt6;        // dt / 6
+    const c2<T>* psi0;
+    const c2<T>* kprev;
+    const c2<T>* k1;
+    const c2<T>* k2;
+    c2<T>* kout;
+    c2<T>* psi_out;
+    const T* res0[2];
+    const T* kres_prev[2];
+    const T* kres1[2];
+    const T* kres2[2];
+    T* kres_out[2];
+    T* res_out[2];
+    const T* pump;
+    const T* coef;
+    uint64_t step;
+    const uint64_t* step_base;
+};
+
+// Reservoir right-hand side with the condensate frozen.  Rate sums are
+// pre-combined on the host in double, as the reference adds Python floats
+// before touching the array (double_reservoir.py:268-269).
+template <typename T, int RES>
+__device__ __forceinline__ void reservoir_rhs(const pgpe_member& mb, T rho, T P, const T n[2], T d[2]) {
+    if constexpr (RES == PGPE_RES_SINGLE) {
+        // dnR = P - (gamma_R + R rho) nR
+        d[0] = P - ((T)mb.res_a + (T)mb.R * rho) * n[0];
+    } else if constexpr (RES == PGPE_RES_DOUBLE) {
+        // state (nA, nI); res_a = gI + R_IA, res_b = R_AI, res_c = R_IA, res_d = gA + R_AI
+        const T nA = n[0], nI = n[1];
+        d[1] = (P - (T)mb.res_a * nI) + (T)mb.res_b * nA;
+        d[0] = (T)mb.res_c * nI - ((T)mb.res_d + (T)mb.R * rho) * nA;
+    } else if constexpr (RES == PGPE_RES_QDOUBLE) {
+        // state (nR, nI); res_a = kappa, res_b = gamma_R, res_c = gamma_I
+        const T nR = n[0], nI = n[1];
+        const T transfer = (T)mb.res_a * nI * nI;
+        d[0] = (transfer - (T)mb.res_b * nR) - (T)mb.R * nR * rho;
+        d[1] = (P - transfer) - (T)mb.res_c * nI;
+    }
+}
+
+template <int RES> struct ResFields { static constexpr int n = (RES == 0) ? 0 : (RES == 1 ? 1 : 2); };
+
+// KIND: 0 = first stage (input is psi0), 1 = middle stage, 2 = final stage.
+template <typename T, int RES, int TH, int KIND>
+__global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
+    constexpr int NF = ResFields<RES>::n;
+    constexpr int W = kTileW + 2;
+    __shared__ c2<T> s_psi[(TH + 2) * W];
+
+    const int tile = xcd_tile(blockIdx.x, p.num_tiles);
+    if (tile < 0) return;
+    const int m = p.tile_member[tile];
+    const pgpe_member& mb = p.table[m];
+    const int nx = mb.nx, ny = mb.ny;
+    const int local = tile - mb.tile_begin;
+    const int x0 = (local % mb.tiles_x) * kTileW;
+    const int y0 = (local / mb.tiles_x) * TH;
+    const int64_t off = mb.off;
+    const c2<T>* __restrict__ psi0 = p.psi0 + off;
+
+    // ---- stage input with halo into LDS (periodic wrap) ----
+    for (int i = threadIdx.x; i < (TH + 2) * W; i += 256) {
+        const int ly = i / W;
+        const int lx = i - ly * W;
+        int gy = y0 + ly - 1;
+        int gx = x0 + lx - 1;
+        gy = gy < 0 ? gy + ny : gy;
+        gx = gx < 0 ? gx + nx : gx;
+        if (gy >= ny) gy %= ny;
+        if (gx >= nx) gx %= nx;
+        const int64_t g = (int64_t)gy * nx + gx;
+        c2<T> v = psi0[g];
+        if constexpr (KIND != 0) {
+            const c2<T> k = p.kprev[off + g];
+            v.x = v.x + p.a * k.x;
+            v.y = v.y + p.a * k.y;
+        }
+        s_psi[i] = v;
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int gx = x0 + lane;
+    if (gx >= nx) return;
+    const T inv_dx2 = (T)mb.inv_dx2, inv_dy2 = (T)mb.inv_dy2;
+    const T kin = (T)mb.kin, inv_hbar = (T)mb.inv_hbar;
+    const T gC = (T)mb.g_C, gR = (T)mb.g_R, gI = (T)mb.g_I, R = (T)mb.R, gammaC = (T)mb.gamma_C;
+    const T eta_c = (T)mb.eta;
+    const T hb2m = (T)mb.hbar_over_2m;
+    const T coef = p.coef[m];
+    const T* __restrict__ env = p.pump + mb.pump_off;
+    uint64_t step = p.step;
+    if (KIND == 2 && p.noise && p.step_base) step += *p.step_base;
+
+    for (int r = threadIdx.x >> 6; r < TH; r += 4) {
+        const int gy = y0 + r;
+        if (gy >= ny) break;
+        const int64_t idx = (int64_t)gy * nx + gx;
+        const int c = (r + 1) * W + lane + 1;
+        const c2<T> ps = s_psi[c];
+        const c2<T> pn = s_psi[c + W];
+        const c2<T> pso = s_psi[c - W];
+        const c2<T> pe = s_psi[c + 1];
+        const c2<T> pw = s_psi[c - 1];
+        // (psi[y+1] - 2 psi + psi[y-1]) / dy^2 + (psi[x+1] - 2 psi + psi[x-1]) / dx^2  (model.py:220-224)
+        c2<T> lap;
+        lap.x = ((pn.x - (T)2 * ps.x) + pso.x) * inv_dy2 + ((pe.x - (T)2 * ps.x) + pw.x) * inv_dx2;
+        lap.y = ((pn.y - (T)2 * ps.y) + pso.y) * inv_dy2 + ((pe.y - (T)2 * ps.y) + pw.y) * inv_dx2;
+
+        T n[2] = {(T)0, (T)0};
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            T v = p.res0[f][off + idx];
+            if constexpr (KIND != 0) v = v + p.a * p.kres_prev[f][off + idx];
+            n[f] = v;
+        }
+        const T P = env[idx] * coef;
+        const T rho = ps.x * ps.x + ps.y * ps.y;
+        T dn[2] = {(T)0, (T)0};
+        reservoir_rhs<T, RES>(mb, rho, P, n, dn);
+
+        const T nA = (NF > 0) ? n[0] : (T)0;
+        const T nI = (NF > 1) ? n[1] : (T)0;
+        const T eff = (gC * rho + gR * nA) + gI * nI;
+        const T gain = (R * nA - gammaC) * (T)0.5;
+        const T Xr = kin * lap.x + eff * ps.x;
+        const T Xi = kin * lap.y + eff * ps.y;
+        c2<T> d;
+        d.x = inv_hbar * Xi + gain * ps.x;
+        d.y = -(inv_hbar * Xr) + gain * ps.y;
+        if (eta_c != (T)0) {
+            const T s = (eta_c * nA) * hb2m;
+            d.x = d.x + s * lap.x;
+            d.y = d.y + s * lap.y;
+        }
+
+        if constexpr (KIND != 2) {
+            p.kout[off + idx] = d;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) p.kres_out[f][off + idx] = dn[f];
+        } else {
+            const c2<T> a1 = p.k1[off + idx];
+            const c2<T> a2 = p.k2[off + idx];
+            const c2<T> a3 = p.kprev[off + idx];
+            const c2<T> z = psi0[idx];
+            c2<T> out;
+            out.x = z.x + p.dt6 * (((a1.x + (T)2 * a2.x) + (T)2 * a3.x) + d.x);
+            out.y = z.y + p.dt6 * (((a1.y + (T)2 * a2.y) + (T)2 * a3.y) + d.y);
+            T n_new[2] = {(T)0, (T)0};
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                const T r0 = p.res0[f][off + idx];
+                const T b1 = p.kres1[f][off + idx];
+                const T b2 = p.kres2[f][off + idx];
+                const T b3 = p.kres_prev[f][off + idx];
+                T v = r0 + p.dt6 * (((b1 + (T)2 * b2) + (T)2 * b3) + dn[f]);
+                v = v > (T)0 ? v : (T)0;  // set_state clamp (single_reservoir.py:70-73)
+                n_new[f] = v;
+                p.res_out[f][off + idx] = v;
+            }
+            if (p.noise) {
+                wigner_site<T>(out.x, out.y, mb.seed, (uint32_t)idx, step, 1u, (T)mb.noise_strength,
+                               (T)mb.noise_gamma, (T)mb.noise_R, NF > 0 ? n_new[0] : (T)0,
+                               (T)mb.noise_dt_over_4v);
+            }
+            p.psi_out[off + idx] = out;
+        }
+    }
+}
+
+template <typename T, int RES, int TH>
+static int launch_kind(const pgpe_rk4_args* a, const Rk4Params<T>& p, hipStream_t s) {
+    const dim3 grid(xcd_grid(a->num_tiles)), block(256);
+    if (a->stage == 0)
+        hipLaunchKernelGGL((rk4_stage_kernel<T, RES, TH, 0>), grid, block, 0, s, p);
+    else if (a->stage < 3)
+        hipLaunchKernelGGL((rk4_stage_kernel<T, RES, TH, 1>), grid, block, 0, s, p);
+    else
+        hipLaunchKernelGGL((rk4_stage_kernel<T, RES, TH, 2>), grid, block, 0, s, p);
+    PGPE_LAUNCHED("rk4_stage_kernel");
+    return PGPE_OK;
+}
+
+template <typename T, int RES>
+static int launch_th(const pgpe_rk4_args* a, const Rk4Params<T>& p, hipStream_t s) {
+    switch (a->tile_y) {
+        case 8: return launch_kind<T, RES, 8>(a, p, s);
+        case 16: return launch_kind<T, RES, 16>(a, p, s);
+        case 32: return launch_kind<T, RES, 32>(a, p, s);
+        default: return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: tile_y must be 8, 16 or 32");
+    }
+}
+
+template <typename T>
+static int launch_typed(const pgpe_rk4_args* a, hipStream_t s) {
+    Rk4Params<T> p{};
+    p.table = a->table;
+    p.tile_member = a->tile_member;
+    p.num_tiles = a->num_tiles;
+    p.noise = a->noise;
+    static const double scale[4] = {0.0, 0.5, 0.5, 1.0};
+    p.a = (T)(scale[a->stage] * a->dt);
+    p.dt6 = (T)(a->dt / 6.0);
+    p.psi0 = (const c2<T>*)a->psi0;
+    p.kprev = (const c2<T>*)a->kprev;
+    p.k1 = (const c2<T>*)a->k1;
+    p.k2 = (const c2<T>*)a->k2;
+    p.kout = (c2<T>*)a->kout;
+    p.psi_out = (c2<T>*)a->psi_out;
+    for (int f = 0; f < 2; ++f) {
+        p.res0[f] = (const T*)a->res0[f];
+        p.kres_prev[f] = (const T*)a->kres_prev[f];
+        p.kres1[f] = (const T*)a->kres1[f];
+        p.kres2[f] = (const T*)a->kres2[f];
+        p.kres_out[f] = (T*)a->kres_out[f];
+        p.res_out[f] = (T*)a->res_out[f];
+    }
+    p.pump = (const T*)a->pump;
+    p.coef = (const T*)a->coef;
+    p.step = a->step;
+    p.step_base = a->step_base;
+    switch (a->reservoir) {
+        case PGPE_RES_NONE: return launch_th<T, PGPE_RES_NONE>(a, p, s);
+        case PGPE_RES_SINGLE: return launch_th<T, PGPE_RES_SINGLE>(a, p, s);
+        case PGPE_RES_DOUBLE: return launch_th<T, PGPE_RES_DOUBLE>(a, p, s);
+        case PGPE_RES_QDOUBLE: return launch_th<T, PGPE_RES_QDOUBLE>(a, p, s);
+        default: return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: unknown reservoir kind");
+    }
+}
+
+// Host-side validation of the pointers a stage dereferences.
+static int check_args(const pgpe_rk4_args* a) {
+    if (!a) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null args");
+    if (a->stage < 0 || a->stage > 3) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: stage must be 0..3");
+    if (a->num_tiles <= 0 || a->members <= 0) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: empty member table");
+    if (!a->table || !a->tile_member || !a->psi0 || !a->pump || !a->coef)
+        return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null table, field, pump or coefficient");
+    const int nf = a->reservoir == PGPE_RES_NONE ? 0 : (a->reservoir == PGPE_RES_SINGLE ? 1 : 2);
+    for (int f = 0; f < nf; ++f)
+        if (!a->res0[f]) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null reservoir field");
+    if (a->stage > 0) {
+        if (!a->kprev) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null kprev");
+        for (int f = 0; f < nf; ++f)
+            if (!a->kres_prev[f]) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null kres_prev");
+    }
+    if (a->stage < 3) {
+        if (!a->kout) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null kout");
+        for (int f = 0; f < nf; ++f)
+            if (!a->kres_out[f]) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null kres_out");
+    } else {
+        if (!a->k1 || !a->k2 || !a->psi_out) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null k1/k2/psi_out");
+        if (a->psi_out == a->psi0) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: psi_out must not alias psi0");
+        for (int f = 0; f < nf; ++f)
+            if (!a->kres1[f] || !a->kres2[f] || !a->res_out[f])
+                return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null reservoir stage buffers");
+    }
+    return PGPE_OK;
+}
+
+}  // namespace pgpe
+
+using namespace pgpe;
+
+extern "C" int pgpe_rk4_stage(const pgpe_rk4_args* a, void* stream) {
+    int rc = check_args(a);
+    if (rc) return rc;
+    if (a->precision == 0) return launch_typed<float>(a, (hipStream_t)stream);
+    if (a->precision == 1) return launch_typed<double>(a, (hipStream_t)stream);
+    return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: precision must be 0 or 1");
+}

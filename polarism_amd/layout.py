@@ -1,0 +1,115 @@
+"""Packed member layout and the device member table.
+
+Members of a batch (equal or uneven grids) are packed back to back in every
+per-point device array; member ``m`` starts at ``offsets[m]`` grid points.
+Each member is cut into 64 x ``tile_h`` tiles; ``tile_member[t]`` names the
+member that owns tile ``t``.  One launch of any stencil kernel covers every
+tile of every member, which is how an uneven-grid sweep runs as one launch
+(an addition; the reference runs each grid shape as its own batch,
+sweep_config.py:18 and sweep.py:128-138).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+from .device import DeviceArray
+from .reservoir import reservoir_rates
+
+TILE_W = 64
+
+
+@dataclass
+class MemberSpec:
+    """Everything the device needs to know about one member."""
+    nx: int
+    ny: int
+    dx: float
+    dy: float
+    physics: object            # attribute access: hbar, m_eff, g_C, ...
+    reservoir: str = "single"
+    components: int = 1
+    res_components: int = 1
+    pump_off: int = 0          # points; members sharing one envelope use the same offset
+    seed: int = 0
+    noise_strength: float = 0.0
+    dt: float = 1e-3
+    circular_polarization: float = 0.0
+
+
+class MemberTable:
+    def __init__(self, specs: list[MemberSpec], tile_h: int = 16):
+        if tile_h not in (8, 16, 32):
+            raise ValueError("tile_h must be 8, 16 or 32")
+        self.specs = list(specs)
+        self.tile_h = tile_h
+        self.members = len(self.specs)
+        rows = (N.Member * self.members)()
+        offsets = []
+        tile_member = []
+        off = 0
+        tile = 0
+        for m, s in enumerate(self.specs):
+            p = s.physics
+            get = lambda name, p=p: float(getattr(p, name))
+            row = rows[m]
+            row.off = off
+            row.pump_off = s.pump_off
+            row.nx, row.ny = s.nx, s.ny
+            row.tiles_x = -(-s.nx // TILE_W)
+            row.tiles_y = -(-s.ny // tile_h)
+            row.tile_begin = tile
+            row.components = s.components
+            row.tile_h = tile_h
+            row.res_components = s.res_components
+            row.inv_dx2 = 1.0 / (s.dx ** 2)
+            row.inv_dy2 = 1.0 / (s.dy ** 2)
+            row.hbar_over_2m = get("hbar") / (2.0 * get("m_eff"))
+            row.inv_hbar = 1.0 / get("hbar")
+            row.kin = -(get("hbar") ** 2) / (2 * get("m_eff"))
+            row.g_C, row.g_R, row.g_I = get("g_C"), get("g_R"), get("g_I")
+            row.R, row.gamma_C = get("R"), get("gamma_C")
+            row.eta = get("kinetic_relaxation_eta")
+            row.res_a, row.res_b, row.res_c, row.res_d = reservoir_rates(s.reservoir, get)
+            row.alpha_2, row.zeeman, row.te_tm = get("alpha_2"), get("zeeman"), get("te_tm")
+            row.inv_4dxdy = 1.0 / (4 * s.dx * s.dy)
+            row.noise_gamma = get("gamma_C")
+            row.noise_R = get("R")
+            row.noise_strength = float(s.noise_strength)
+            row.noise_dt_over_4v = s.dt / (4.0 * (s.dy * s.dx))
+            row.seed = int(s.seed) & 0xFFFFFFFFFFFFFFFF
+            row.pump_w_plus = (1 + s.circular_polarization) / 2
+            row.pump_w_minus = (1 - s.circular_polarization) / 2
+            n_tiles = row.tiles_x * row.tiles_y
+            tile_member.extend([m] * n_tiles)
+            offsets.append(off)
+            off += s.nx * s.ny
+            tile += n_tiles
+        self.rows = rows
+        self.offsets = np.asarray(offsets, dtype=np.int64)
+        self.total_points = off
+        self.num_tiles = tile
+        self.tile_member_host = np.asarray(tile_member, dtype=np.int32)
+        raw = np.frombuffer(bytes(rows), dtype=np.uint8)
+        self.table_dev = DeviceArray.from_numpy(raw)
+        self.tile_member_dev = DeviceArray.from_numpy(self.tile_member_host)
+
+    @property
+    def table_ptr(self) -> int:
+        return self.table_dev.ptr
+
+    @property
+    def tile_member_ptr(self) -> int:
+        return self.tile_member_dev.ptr
+
+    def member_points(self, m: int) -> int:
+        s = self.specs[m]
+        return s.nx * s.ny
+
+    def refresh(self) -> None:
+        """Re-upload the table after host-side edits of ``rows``."""
+        raw = np.frombuffer(bytes(self.rows), dtype=np.uint8)
+        self.table_dev.upload(raw)

@@ -1,0 +1,111 @@
+"""rk4-fdm-hip against the oracle restatement of the reference rk4-fdm."""
+import numpy as np
+import pytest
+
+from oracle import gpe_oracle as O
+from tests.helpers import oracle_rk4_run, quickstart_config
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(cfg, members=None):
+    from polarism_amd import SimulationController
+    ctl = SimulationController(cfg, members=members)
+    ctl.run()
+    return ctl
+
+
+def test_quickstart_fp64_matches_oracle():
+    """Config 1: 128^2, 2000 steps, complex128; rel-L2 <= 1e-10 on psi and nR."""
+    cfg = quickstart_config()
+    ctl = _run(cfg)
+    psi_ref, res_ref = oracle_rk4_run(cfg, 2000)
+    psi = ctl.state.field.get()
+    nR = ctl.reservoir.get_state()[0].get()
+    e_psi = O.rel_l2(psi, psi_ref)
+    e_res = O.rel_l2(nR, res_ref[0])
+    print(f"quickstart fp64 rel-L2 psi={e_psi:.3e} nR={e_res:.3e}")
+    assert e_psi <= 1e-10 and e_res <= 1e-10
+
+
+def test_quickstart_fp32_matches_oracle_and_reports_fp64_error():
+    cfg = quickstart_config(solver__precision="single", solver__total_time=0.5)
+    ctl = _run(cfg)
+    psi_ref, res_ref = oracle_rk4_run(cfg, 500)
+    psi = ctl.state.field.get()
+    nR = ctl.reservoir.get_state()[0].get()
+    e_psi = O.rel_l2(psi, psi_ref)
+    dens = O.rel_l2(np.abs(psi) ** 2, np.abs(psi_ref) ** 2)
+    cfg64 = quickstart_config(solver__total_time=0.5)
+    psi64, _ = oracle_rk4_run(cfg64, 500)
+    e64 = O.rel_l2(psi.astype(np.complex128), psi64)
+    print(f"fp32 rel-L2 psi={e_psi:.3e} density={dens:.3e} nR={O.rel_l2(nR, res_ref[0]):.3e} vs-fp64={e64:.3e}")
+    assert e_psi <= 1e-5 and dens <= 1e-5 and O.rel_l2(nR, res_ref[0]) <= 1e-5
+
+
+@pytest.mark.parametrize("kind", ["double", "quadratic-double", "none"])
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("single", 1e-5)])
+def test_reservoir_kinds_match_oracle(kind, precision, tol):
+    cfg = quickstart_config(reservoir__reservoir_type=kind, solver__precision=precision, solver__total_time=0.2,
+                            grid__nx=96, grid__ny=80, laser__P0=0.2)
+    cfg.solver.dt = 2e-3
+    ctl = _run(cfg)
+    psi_ref, res_ref = oracle_rk4_run(cfg, 100)
+    assert O.rel_l2(ctl.state.field.get(), psi_ref) <= tol
+    for dev, ref in zip(ctl.reservoir.get_state(), res_ref):
+        assert O.rel_l2(dev.get(), ref) <= tol
+
+
+def test_pulsed_pump_and_eta_match_oracle():
+    cfg = quickstart_config(laser__laser_type="pulse-gaussian", solver__total_time=0.2, grid__nx=64, grid__ny=48,
+                            physics__kinetic_relaxation_eta=0.04)
+    cfg.laser.sigma_time = 0.03
+    cfg.laser.P0 = 0.4
+    cfg.solver.dt = 2e-3
+    ctl = _run(cfg)
+    psi_ref, res_ref = oracle_rk4_run(cfg, 100)
+    assert O.rel_l2(ctl.state.field.get(), psi_ref) <= 1e-10
+    assert O.rel_l2(ctl.reservoir.get_state()[0].get(), res_ref[0]) <= 1e-10
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("single", 1e-5)])
+def test_wigner_noise_run_matches_oracle(precision, tol):
+    cfg = quickstart_config(noise__stochastic="wigner", solver__precision=precision, solver__total_time=0.05,
+                            grid__nx=64, grid__ny=64, laser__P0=0.05)
+    cfg.noise.seed = 1234
+    cfg.solver.dt = 5e-3
+    ctl = _run(cfg)
+    psi_ref, res_ref = oracle_rk4_run(cfg, 10, noise=True)
+    assert O.rel_l2(ctl.state.field.get(), psi_ref) <= tol
+    assert O.rel_l2(ctl.reservoir.get_state()[0].get(), res_ref[0]) <= tol
+
+
+def test_graph_chunks_equal_single_launches_bitwise():
+    cfg = quickstart_config(solver__total_time=0.05, noise__stochastic="wigner", grid__nx=64, grid__ny=64)
+    a = _run(cfg)
+    cfg2 = quickstart_config(solver__total_time=0.05, noise__stochastic="wigner", grid__nx=64, grid__ny=64,
+                             solver__cuda_graphs=False)
+    b = _run(cfg2)
+    np.testing.assert_array_equal(a.state.field.get(), b.state.field.get())
+    np.testing.assert_array_equal(a.reservoir.get_state()[0].get(), b.reservoir.get_state()[0].get())
+
+
+def test_batch_member_equals_single_run_bitwise():
+    cfg = quickstart_config(solver__total_time=0.05, grid__nx=64, grid__ny=64, noise__stochastic="wigner")
+    members = [{"laser.P0": 0.02, "noise.seed": 5}, {"laser.P0": 0.05, "noise.seed": 6},
+               {"laser.P0": 0.03, "physics.g_C": 0.002, "noise.seed": 7}]
+    batch = _run(cfg, members)
+    for m, over in enumerate(members):
+        single = _run(cfg, [over])
+        np.testing.assert_array_equal(batch.state.field.get()[m], single.state.field.get()[0])
+
+
+@pytest.mark.parametrize("tile_h", [8, 32])
+def test_tile_height_does_not_change_results(tile_h):
+    from polarism_amd import SimulationController
+    cfg = quickstart_config(solver__total_time=0.02, grid__nx=96, grid__ny=72)
+    ref = _run(cfg)
+    ctl = SimulationController(cfg)
+    ctl.solver.tile_h = tile_h
+    ctl.run()
+    np.testing.assert_array_equal(ctl.state.field.get(), ref.state.field.get())
