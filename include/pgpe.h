@@ -75,7 +75,9 @@ int pgpe_graph_launch(void* graph_exec, void* stream);
 int pgpe_graph_destroy(void* graph_exec);
 
 /* ------------------------------------------------------------------ */
-/* Member table: one row per sweep member, device resident.            */
+/* Member table: one row per sweep member, device resident.  Launches  */
+/* cover a tile map of (member, local tile) int32 pairs; a member of   */
+/* nx x ny points has tiles_x * tiles_y tiles of 64 x tile_h points.   */
 /* Per-member physics replaces BatchPhysics columns                    */
 /* (config/batch_physics.py:17-64; rk4_cuda_solver.py:983-1021).       */
 /* ------------------------------------------------------------------ */
@@ -92,6 +94,7 @@ typedef struct pgpe_member {
     double hbar_over_2m;    /* hbar / (2 m_eff) */
     double inv_hbar;
     double kin;             /* -hbar^2 / (2 m_eff) */
+    double hbar, m_eff;
     double g_C, g_R, g_I, R, gamma_C, eta;
     double res_a, res_b, res_c, res_d; /* single: gamma_R; double: gamma_I, gamma_A, R_IA, R_AI;
                                           quadratic-double: kappa, gamma_R, gamma_I */
@@ -116,7 +119,7 @@ int pgpe_normal_pairs(void* out_dev, int precision, uint64_t key, uint64_t step,
  * sigma = sqrt(strength (gamma + R n_active) dt / (4 dx dy)).  active_dev may be
  * NULL (zero density); active_components is 1 (shared) or equal to components. */
 int pgpe_wigner_add(void* psi_dev, const void* active_dev, int active_components,
-                    const pgpe_member* table_dev, const int32_t* tile_member_dev,
+                    const pgpe_member* table_dev, const int32_t* tile_map_dev,
                     int num_tiles, int members, int precision, uint64_t step, void* stream);
 
 /* ------------------------------------------------------------------ */
@@ -134,7 +137,7 @@ typedef struct pgpe_rk4_args {
     int32_t pad0;
     double dt;
     const pgpe_member* table;        /* device */
-    const int32_t* tile_member;      /* device, num_tiles entries */
+    const int32_t* tile_map;         /* device, num_tiles (member, local tile) int32 pairs */
     const void* psi0;                /* state at the step start */
     const void* kprev;               /* stages 1..3: derivative of the previous stage */
     const void* k1; const void* k2;  /* stage 3: derivatives of stages 0..2 */
@@ -157,12 +160,138 @@ int pgpe_rk4_stage(const pgpe_rk4_args* args, void* stream);
 /* Steady-state reduction (replaces simulation_controller.py:710-734)   */
 /* For every member: sums[3m+0] = sum rho, sums[3m+1] = sum (rho-old)^2, */
 /* sums[3m+2] = sum old^2 in double, then old <- rho when update != 0.   */
-/* partials_dev holds 3 * num_tiles doubles.                            */
+/* partials_dev holds 3 doubles per tile of the whole table; the tile  */
+/* map may list a subset of the tiles (members still running).         */
 /* ------------------------------------------------------------------ */
 int pgpe_density_delta(const void* psi_dev, void* old_density_dev, int precision,
                        int components, const pgpe_member* table_dev,
-                       const int32_t* tile_member_dev, int num_tiles, int members,
+                       const int32_t* tile_map_dev, int num_tiles, int members,
                        double* partials_dev, double* sums_dev, int update, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Batched 2-D FFT (LDS Stockham, radix 2/3/4) and the fused split-step */
+/* step (replaces cuFFT plans split_step_fft_cuda_solver.py:729-735,    */
+/* numpy.fft in split_step_fft_solver.py:70-79, and the local kernels   */
+/* split_step_fft_cuda_solver.py:106-184).                              */
+/* ------------------------------------------------------------------ */
+typedef struct pgpe_fft_plan pgpe_fft_plan;  /* opaque; twiddles and wave numbers */
+
+/* Sizes 2^a or 3 * 2^a (2 <= n <= 3072); allocates its tables once. */
+int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pgpe_fft_plan** plan);
+int pgpe_fft_plan_destroy(pgpe_fft_plan* plan);
+/* planes x (ny, nx) complex: forward (unnormalised) or inverse (scaled by
+ * 1/(nx ny)), the numpy.fft.fft2 / ifft2 conventions.  in may equal out. */
+int pgpe_fft2(const pgpe_fft_plan* plan, const void* in_dev, void* out_dev, int planes, int inverse,
+              void* stream);
+
+typedef struct pgpe_split_args {
+    int32_t precision;
+    int32_t reservoir;      /* pgpe_reservoir_kind */
+    int32_t ordering;       /* 0: R K N noise K R (CPU solver order); 1: K [R N R noise] K */
+    int32_t noise;          /* add the Wigner increment */
+    int32_t members;        /* planes of equal grids, one table row each */
+    int32_t pad0;
+    double dt;
+    const pgpe_member* table;
+    const pgpe_fft_plan* plan;
+    void* psi;              /* in/out, members x (ny, nx) complex */
+    void* tmp;              /* scratch of the same size */
+    void* res[2];           /* in/out reservoir fields */
+    const void* pump;       /* envelopes (real) */
+    const void* coef;       /* 2 * members reals: pump coefficient at t+dt/4, then at t+3dt/4 */
+    uint64_t step;          /* noise counter step = (*step_base if set) + step */
+    const uint64_t* step_base;
+} pgpe_split_args;
+
+int pgpe_split_step(const pgpe_split_args* args, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Lawson integrating-factor RK (replaces IFRKFFTCudaSolver._generic_step */
+/* and _adaptive_error, ifrk_fft_cuda_solver.py:148-293, and the fused   */
+/* kernels of ifrk_cuda_fused.py:10-374).                                */
+/* op STAGE: psi_i^k = L(frac0 h) psi0^k + sum_j hc[j] L(fr[j] h) f_j,    */
+/*   inverse transform, N(psi_i, n_i) with n_i = n0 + sum_j rc[j] dres_j,  */
+/*   writes dres_out and f_out = FFT(N).  stage_real = 1 takes psi_real    */
+/*   (real space) as psi_i instead of the combination.                    */
+/* op FINAL: psi_new^k = L(h) psi0^k + sum_j hc[j] L(fr[j] h) f_j, inverse */
+/*   into psi_out, res_out = max(n0 + sum_j rc[j] dres_j, 0), optional     */
+/*   noise; with nerr > 0 also the embedded error fields (he5, he3) and    */
+/*   per-member sums of (|e| / (atol + rtol max(|start|, |new|)))^2 into   */
+/*   error_sums[2 m], [2 m + 1] (double).                                  */
+/* ------------------------------------------------------------------ */
+#define PGPE_MAX_STAGES 12
+enum pgpe_lawson_op { PGPE_LAWSON_STAGE = 0, PGPE_LAWSON_FINAL = 1 };
+
+typedef struct pgpe_lawson_args {
+    int32_t precision, reservoir, op, components, res_components, members;
+    int32_t nterms, lap, nerr, stage_real, noise, pad0;
+    double h, frac0, atol, rtol;
+    double hc[PGPE_MAX_STAGES], fr[PGPE_MAX_STAGES], rc[PGPE_MAX_STAGES];
+    double he5[PGPE_MAX_STAGES], he3[PGPE_MAX_STAGES];
+    const pgpe_member* table;
+    const pgpe_fft_plan* plan;
+    const void* psi0k;                       /* members x comps x (ny, nx) complex */
+    const void* f[PGPE_MAX_STAGES];          /* stage transforms */
+    const void* dres[PGPE_MAX_STAGES][2];    /* stage reservoir derivatives */
+    void* work;                              /* scratch, one field */
+    void* aux;                               /* scratch, two fields (lap or error fields) */
+    void* f_out;
+    const void* psi_real;
+    const void* res0[2];
+    void* dres_out[2];
+    const void* pump;
+    const void* coef;                        /* members reals: pump coefficient at the stage time */
+    const void* potential;                   /* complex (ny, nx) potential or NULL */
+    void* psi_out;
+    void* res_out[2];
+    void* psik_out;                          /* optional: psi_new^k */
+    const void* psi_start;                   /* real-space field at the trial start */
+    double* partials;                        /* 2 * members * ny doubles */
+    double* error_sums;                      /* 2 * members doubles */
+    uint64_t step;
+    const uint64_t* step_base;
+} pgpe_lawson_args;
+
+int pgpe_lawson(const pgpe_lawson_args* args, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Surrogate (Fourier neural operator) kernels: BF16 MFMA GEMM with fp32 */
+/* accumulation and fused epilogue, plus the elementwise pieces of the   */
+/* forward and backward passes.  No counterpart in the reference.        */
+/* C[b](m, n) = alpha sum_k A[b](m, k) B[b](k, n) + beta C + bias + res,  */
+/* then optional GELU; element (m, k) of A is at A + b sab + m sam + k sak */
+/* (A needs sak == 1 or sam == 1; B needs sbk == 1 or sbn == 1).          */
+/* ------------------------------------------------------------------ */
+typedef struct pgpe_gemm_args {
+    const void* A;            /* bf16 */
+    const void* B;            /* bf16 */
+    void* C;                  /* fp32, or bf16 when out_bf16 */
+    const float* bias;        /* M (bias_mode 1) or N (bias_mode 2) floats */
+    const float* residual;    /* fp32 added before GELU, strides of C, batch stride srb */
+    int32_t M, N, K, batch;
+    int64_t sam, sak, sab;
+    int64_t sbk, sbn, sbb;
+    int64_t scm, scn, scb;
+    int64_t srb;
+    float alpha, beta;
+    int32_t bias_mode, gelu, out_bf16, pad0;
+} pgpe_gemm_args;
+
+int pgpe_gemm_bf16(const pgpe_gemm_args* args, void* stream);
+int pgpe_f32_to_bf16(const float* x_dev, void* y_dev, int64_t n, void* stream);
+/* y = GELU(x) (erf form) into fp32 y and / or bf16 yb. */
+int pgpe_gelu_fwd(const float* x_dev, float* y_dev, void* yb_dev, int64_t n, void* stream);
+/* dx = dy GELU'(x) into fp32 dx and / or bf16 dxb. */
+int pgpe_gelu_bwd(const float* x_dev, const float* dy_dev, float* dx_dev, void* dxb_dev, int64_t n, void* stream);
+/* Complex (interleaved) y[i] = d[i mod per_batch] x[i] (conj(d) when conj), i < n; fp32 y and / or bf16 yb. */
+int pgpe_cdiag(const float* x_dev, const float* d_dev, float* y_dev, void* yb_dev, int64_t per_batch, int64_t n,
+               int conj, void* stream);
+/* dd[i] = sum_b conj(x[b, i]) g[b, i], complex, i < per_batch. */
+int pgpe_cdiag_grad(const float* x_dev, const float* g_dev, float* dd_dev, int64_t per_batch, int batch,
+                    void* stream);
+/* out[c] = sum_r x[r row_stride + c col_stride], fp32. */
+int pgpe_colsum(const float* x_dev, float* out_dev, int64_t rows, int64_t cols, int64_t row_stride,
+                int64_t col_stride, void* stream);
 
 #ifdef __cplusplus
 }

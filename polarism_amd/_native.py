@@ -31,6 +31,7 @@ class Member(C.Structure):
         ("tile_h", C.c_int32), ("res_components", C.c_int32),
         ("inv_dx2", C.c_double), ("inv_dy2", C.c_double),
         ("hbar_over_2m", C.c_double), ("inv_hbar", C.c_double), ("kin", C.c_double),
+        ("hbar", C.c_double), ("m_eff", C.c_double),
         ("g_C", C.c_double), ("g_R", C.c_double), ("g_I", C.c_double), ("R", C.c_double),
         ("gamma_C", C.c_double), ("eta", C.c_double),
         ("res_a", C.c_double), ("res_b", C.c_double), ("res_c", C.c_double), ("res_d", C.c_double),
@@ -49,13 +50,57 @@ class Rk4Args(C.Structure):
         ("num_tiles", C.c_int32), ("tile_y", C.c_int32), ("noise", C.c_int32),
         ("members", C.c_int32), ("pad0", C.c_int32),
         ("dt", C.c_double),
-        ("table", vp), ("tile_member", vp),
+        ("table", vp), ("tile_map", vp),
         ("psi0", vp), ("kprev", vp), ("k1", vp), ("k2", vp),
         ("kout", vp), ("psi_out", vp),
         ("res0", vp * 2), ("kres_prev", vp * 2), ("kres1", vp * 2), ("kres2", vp * 2),
         ("kres_out", vp * 2), ("res_out", vp * 2),
         ("pump", vp), ("coef", vp),
         ("step", C.c_uint64), ("step_base", vp),
+    ]
+
+
+class SplitArgs(C.Structure):
+    """Mirror of ``pgpe_split_args``."""
+    _fields_ = [
+        ("precision", C.c_int32), ("reservoir", C.c_int32), ("ordering", C.c_int32), ("noise", C.c_int32),
+        ("members", C.c_int32), ("pad0", C.c_int32),
+        ("dt", C.c_double),
+        ("table", vp), ("plan", vp), ("psi", vp), ("tmp", vp), ("res", vp * 2),
+        ("pump", vp), ("coef", vp), ("step", C.c_uint64), ("step_base", vp),
+    ]
+
+
+MAX_STAGES = 12
+
+
+class LawsonArgs(C.Structure):
+    """Mirror of ``pgpe_lawson_args``."""
+    _fields_ = [
+        ("precision", C.c_int32), ("reservoir", C.c_int32), ("op", C.c_int32), ("components", C.c_int32),
+        ("res_components", C.c_int32), ("members", C.c_int32),
+        ("nterms", C.c_int32), ("lap", C.c_int32), ("nerr", C.c_int32), ("stage_real", C.c_int32),
+        ("noise", C.c_int32), ("pad0", C.c_int32),
+        ("h", C.c_double), ("frac0", C.c_double), ("atol", C.c_double), ("rtol", C.c_double),
+        ("hc", C.c_double * MAX_STAGES), ("fr", C.c_double * MAX_STAGES), ("rc", C.c_double * MAX_STAGES),
+        ("he5", C.c_double * MAX_STAGES), ("he3", C.c_double * MAX_STAGES),
+        ("table", vp), ("plan", vp), ("psi0k", vp), ("f", vp * MAX_STAGES), ("dres", (vp * 2) * MAX_STAGES),
+        ("work", vp), ("aux", vp), ("f_out", vp), ("psi_real", vp), ("res0", vp * 2), ("dres_out", vp * 2),
+        ("pump", vp), ("coef", vp), ("potential", vp), ("psi_out", vp), ("res_out", vp * 2), ("psik_out", vp),
+        ("psi_start", vp), ("partials", vp), ("error_sums", vp), ("step", C.c_uint64), ("step_base", vp),
+    ]
+
+
+class GemmArgs(C.Structure):
+    """Mirror of ``pgpe_gemm_args``."""
+    _fields_ = [
+        ("A", vp), ("B", vp), ("C", vp), ("bias", vp), ("residual", vp),
+        ("M", C.c_int32), ("N", C.c_int32), ("K", C.c_int32), ("batch", C.c_int32),
+        ("sam", C.c_int64), ("sak", C.c_int64), ("sab", C.c_int64),
+        ("sbk", C.c_int64), ("sbn", C.c_int64), ("sbb", C.c_int64),
+        ("scm", C.c_int64), ("scn", C.c_int64), ("scb", C.c_int64), ("srb", C.c_int64),
+        ("alpha", C.c_float), ("beta", C.c_float),
+        ("bias_mode", C.c_int32), ("gelu", C.c_int32), ("out_bf16", C.c_int32), ("pad0", C.c_int32),
     ]
 
 
@@ -89,6 +134,18 @@ _SIGNATURES = {
     "pgpe_normal_pairs": (C.c_int, [vp, C.c_int, C.c_uint64, C.c_uint64, C.c_uint32, C.c_int64, vp]),
     "pgpe_wigner_add": (C.c_int, [vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, C.c_uint64, vp]),
     "pgpe_rk4_stage": (C.c_int, [C.POINTER(Rk4Args), vp]),
+    "pgpe_fft_plan_create": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(vp)]),
+    "pgpe_fft_plan_destroy": (C.c_int, [vp]),
+    "pgpe_fft2": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),
+    "pgpe_split_step": (C.c_int, [C.POINTER(SplitArgs), vp]),
+    "pgpe_lawson": (C.c_int, [C.POINTER(LawsonArgs), vp]),
+    "pgpe_gemm_bf16": (C.c_int, [C.POINTER(GemmArgs), vp]),
+    "pgpe_f32_to_bf16": (C.c_int, [vp, vp, C.c_int64, vp]),
+    "pgpe_gelu_fwd": (C.c_int, [vp, vp, vp, C.c_int64, vp]),
+    "pgpe_gelu_bwd": (C.c_int, [vp, vp, vp, vp, C.c_int64, vp]),
+    "pgpe_cdiag": (C.c_int, [vp, vp, vp, vp, C.c_int64, C.c_int64, C.c_int, vp]),
+    "pgpe_cdiag_grad": (C.c_int, [vp, vp, vp, C.c_int64, C.c_int, vp]),
+    "pgpe_colsum": (C.c_int, [vp, vp, C.c_int64, C.c_int64, C.c_int64, C.c_int64, vp]),
     "pgpe_density_delta": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp]),
 }
 

@@ -153,7 +153,7 @@ class SimulationController:
         old = self._steady_buffers()
         prec = 0 if self.cfg.solver.precision == "single" else 1
         N.call("pgpe_density_delta", self.state.field.ptr, old.ptr, prec, self.state.components,
-               tab.table_ptr, tab.tile_member_ptr, tab.num_tiles, tab.members, self._steady_partials.ptr,
+               tab.table_ptr, tab.tile_map_ptr, tab.num_tiles, tab.members, self._steady_partials.ptr,
                self._steady_sums.ptr, 1 if update else 0, None)
         return self._steady_sums.get().reshape(-1, 3)
 

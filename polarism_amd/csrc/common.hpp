@@ -54,12 +54,15 @@ template <> struct vec2<double> { using type = double2; };
 template <typename T> using c2 = typename vec2<T>::type;
 
 template <typename T>
-__device__ __forceinline__ c2<T> make_c(T re, T im) {
+__host__ __device__ __forceinline__ c2<T> make_c(T re, T im) {
     c2<T> v;
     v.x = re;
     v.y = im;
     return v;
 }
+
+__device__ __forceinline__ void sin_cos(float x, float* s, float* c) { sincosf(x, s, c); }
+__device__ __forceinline__ void sin_cos(double x, double* s, double* c) { sincos(x, s, c); }
 
 // ---------------------------------------------------------------- Philox4x32-10
 // Counter (cell, step_lo, step_hi, stream), key = 64-bit seed split lo/hi

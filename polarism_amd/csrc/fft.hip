@@ -1,0 +1,490 @@
+// Batched 2-D FFT (row + column LDS Stockham kernels) and the fused
+// split-step FFT step of the driven-dissipative GPE.
+//
+// A 2-D transform of a batch of (ny, nx) planes is two launches:
+//   rows    : every row (contiguous along x) transformed in LDS,
+//   columns : a workgroup loads C whole columns (C consecutive x, read as
+//             C-element row segments so HBM access stays coalesced),
+//             transforms them along y in LDS and writes them back.
+// The split-step step fuses the pointwise physics into the transforms so each
+// pass reads and writes the field once:
+//   CPU order  (split_step_fft_solver.py:97-159):
+//     rows  : reservoir half-step (P at t+dt/4), forward FFT along x
+//     cols  : forward FFT along y, x exp(-i hbar k^2 dt/(4m)), inverse along y
+//     rows  : inverse along x, exact nonlinear step, Wigner noise, forward along x
+//     cols  : as above
+//     rows  : inverse along x, reservoir half-step (P at t+3dt/4)
+//   K-outer (split_step_fft_cuda_solver.py:106-184 order):
+//     rows  : forward;  cols : propagator;
+//     rows  : inverse, R(dt/2) N(dt) R(dt/2), noise, forward;
+//     cols  : propagator;  rows : inverse.
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "fft_core.hpp"
+#include "fft_plan.hpp"
+
+namespace pgpe {
+
+enum RowMode { ROW_FWD = 0, ROW_INV = 1, ROW_SS_FIRST = 2, ROW_SS_MID = 3, ROW_SS_LAST = 4, ROW_KO_MID = 5 };
+enum ColMode { COL_FWD = 0, COL_INV = 1, COL_PROP = 2 };
+
+template <typename T>
+struct SplitParams {
+    const pgpe_member* table;
+    FftDims dims;
+    const c2<T>* tw;
+    const double* kx;
+    const double* ky;
+    c2<T>* in;
+    c2<T>* out;
+    int nx, ny, planes;
+    int reservoir;        // pgpe_reservoir_kind
+    int noise;
+    T dt;
+    double prop_h;        // propagator step (dt/2 for a half step)
+    T scale;              // inverse-transform normalisation
+    T* res[2];
+    const T* pump;
+    const T* coef;        // coef[0*members + m]: first half-step, coef[1*members + m]: second
+    int members;
+    uint64_t step;
+    const uint64_t* step_base;
+};
+
+template <typename T, int RES>
+__device__ __forceinline__ void res_rhs(const pgpe_member& mb, T rho, T P, const T n[2], T d[2]) {
+    if constexpr (RES == PGPE_RES_SINGLE) {
+        d[0] = P - ((T)mb.res_a + (T)mb.R * rho) * n[0];
+    } else if constexpr (RES == PGPE_RES_DOUBLE) {
+        d[1] = (P - (T)mb.res_a * n[1]) + (T)mb.res_b * n[0];
+        d[0] = (T)mb.res_c * n[1] - ((T)mb.res_d + (T)mb.R * rho) * n[0];
+    } else if constexpr (RES == PGPE_RES_QDOUBLE) {
+        const T transfer = (T)mb.res_a * n[1] * n[1];
+        d[0] = (transfer - (T)mb.res_b * n[0]) - (T)mb.R * n[0] * rho;
+        d[1] = (P - transfer) - (T)mb.res_c * n[1];
+    }
+}
+
+// Midpoint RK2 with psi frozen, then clamp at zero (single_reservoir.py:100-111).
+template <typename T, int RES>
+__device__ __forceinline__ void res_half_step(const pgpe_member& mb, T rho, T P, T h, T n[2]) {
+    constexpr int NF = RES == 0 ? 0 : (RES == 1 ? 1 : 2);
+    if constexpr (NF == 0) return;
+    T k1[2] = {0, 0}, k2[2] = {0, 0}, mid[2] = {0, 0};
+    res_rhs<T, RES>(mb, rho, P, n, k1);
+    const T hh = (T)0.5 * h;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) mid[f] = n[f] + hh * k1[f];
+    res_rhs<T, RES>(mb, rho, P, mid, k2);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const T v = n[f] + h * k2[f];
+        n[f] = v > (T)0 ? v : (T)0;
+    }
+}
+
+// Exact local gain/phase step (split_step_fft_solver.py:124-140), zero potential.
+template <typename T>
+__device__ __forceinline__ void nonlinear_step(const pgpe_member& mb, T nA, T nI, T dt, c2<T>& v) {
+    const T rho = v.x * v.x + v.y * v.y;
+    const T gain = ((T)mb.R * nA - (T)mb.gamma_C) / (T)2;
+    const T x = (T)2 * gain * dt;
+    const T phase_time = fabs(x) < (T)1e-6 ? dt * (((T)1 + x / (T)2) + x * x / (T)6) : dt * (expm1(x) / x);
+    const T phase = (((T)mb.g_R * nA + (T)mb.g_I * nI) * dt + (T)mb.g_C * rho * phase_time) / (T)mb.hbar;
+    const T amp = exp(gain * dt);
+    T s, c;
+    sin_cos(phase, &s, &c);
+    // exp(gain dt - i phase) = amp (cos phase - i sin phase)
+    const c2<T> f = make_c<T>(amp * c, -(amp * s));
+    v = cmul<T>(v, f);
+}
+
+template <typename T, int RES>
+__device__ __forceinline__ void load_res(const SplitParams<T>& p, int64_t g, T n[2]) {
+    constexpr int NF = RES == 0 ? 0 : (RES == 1 ? 1 : 2);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) n[f] = p.res[f][g];
+}
+
+template <typename T, int RES>
+__device__ __forceinline__ void store_res(const SplitParams<T>& p, int64_t g, const T n[2]) {
+    constexpr int NF = RES == 0 ? 0 : (RES == 1 ? 1 : 2);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) p.res[f][g] = n[f];
+}
+
+// ---------------------------------------------------------------- rows
+template <typename T, int V, int MODE, int RES>
+__global__ void __launch_bounds__(256) fft_rows_kernel(const SplitParams<T> p) {
+    extern __shared__ unsigned char smem_raw[];
+    c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
+    constexpr int E = 256 * V;
+    const int n = p.nx;
+    const int rows_per = E / n;
+    const int total_rows = p.planes * p.ny;
+    const int row0 = blockIdx.x * rows_per;
+    const int count = min(rows_per, total_rows - row0);
+    if (count <= 0) return;
+    const int elems = count * n;
+    const int64_t g0 = (int64_t)row0 * n;
+    const bool res_here = RES != 0;
+    uint64_t step = p.step;
+    if (p.step_base) step += *p.step_base;
+
+    // ---- load (+ prologue) ----
+    for (int i = threadIdx.x; i < elems; i += 256) {
+        c2<T> v = p.in[g0 + i];
+        if constexpr (MODE == ROW_SS_FIRST) {
+            const int row = row0 + i / n;
+            const int m = row / p.ny;
+            const int64_t cell = (int64_t)(row - m * p.ny) * n + (i % n);
+            const pgpe_member& mb = p.table[m];
+            T nn[2] = {0, 0};
+            load_res<T, RES>(p, g0 + i, nn);
+            const T P = p.pump[mb.pump_off + cell] * p.coef[m];
+            res_half_step<T, RES>(mb, v.x * v.x + v.y * v.y, P, p.dt / (T)2, nn);
+            if (res_here) store_res<T, RES>(p, g0 + i, nn);
+        }
+        buf[i] = v;
+    }
+    __syncthreads();
+    const bool first_inverse = (MODE == ROW_INV || MODE == ROW_SS_MID || MODE == ROW_SS_LAST || MODE == ROW_KO_MID);
+    lds_fft<T, V>(buf, p.dims, n, count, p.tw, first_inverse);
+
+    if constexpr (MODE == ROW_SS_MID || MODE == ROW_KO_MID) {
+        // inverse done: scale, local physics, then forward again in place
+        for (int i = threadIdx.x; i < elems; i += 256) {
+            c2<T> v = buf[i];
+            v.x *= p.scale;
+            v.y *= p.scale;
+            const int row = row0 + i / n;
+            const int m = row / p.ny;
+            const int64_t cell = (int64_t)(row - m * p.ny) * n + (i % n);
+            const pgpe_member& mb = p.table[m];
+            T nn[2] = {0, 0};
+            load_res<T, RES>(p, g0 + i, nn);
+            if constexpr (MODE == ROW_KO_MID) {
+                const T P1 = p.pump[mb.pump_off + cell] * p.coef[m];
+                res_half_step<T, RES>(mb, v.x * v.x + v.y * v.y, P1, p.dt / (T)2, nn);
+                nonlinear_step<T>(mb, RES ? nn[0] : (T)0, RES >= 2 ? nn[1] : (T)0, p.dt, v);
+                const T P2 = p.pump[mb.pump_off + cell] * p.coef[p.members + m];
+                res_half_step<T, RES>(mb, v.x * v.x + v.y * v.y, P2, p.dt / (T)2, nn);
+                if (res_here) store_res<T, RES>(p, g0 + i, nn);
+            } else {
+                nonlinear_step<T>(mb, RES ? nn[0] : (T)0, RES >= 2 ? nn[1] : (T)0, p.dt, v);
+            }
+            if (p.noise) {
+                wigner_site<T>(v.x, v.y, mb.seed, (uint32_t)cell, step, 1u, (T)mb.noise_strength, (T)mb.noise_gamma,
+                               (T)mb.noise_R, RES ? nn[0] : (T)0, (T)mb.noise_dt_over_4v);
+            }
+            buf[i] = v;
+        }
+        __syncthreads();
+        lds_fft<T, V>(buf, p.dims, n, count, p.tw, false);
+    }
+
+    // ---- store (+ epilogue) ----
+    for (int i = threadIdx.x; i < elems; i += 256) {
+        c2<T> v = buf[i];
+        if constexpr (MODE == ROW_INV || MODE == ROW_SS_LAST) {
+            v.x *= p.scale;
+            v.y *= p.scale;
+        }
+        if constexpr (MODE == ROW_SS_LAST) {
+            const int row = row0 + i / n;
+            const int m = row / p.ny;
+            const int64_t cell = (int64_t)(row - m * p.ny) * n + (i % n);
+            const pgpe_member& mb = p.table[m];
+            T nn[2] = {0, 0};
+            load_res<T, RES>(p, g0 + i, nn);
+            const T P = p.pump[mb.pump_off + cell] * p.coef[p.members + m];
+            res_half_step<T, RES>(mb, v.x * v.x + v.y * v.y, P, p.dt / (T)2, nn);
+            if (res_here) store_res<T, RES>(p, g0 + i, nn);
+        }
+        p.out[g0 + i] = v;
+    }
+}
+
+// ---------------------------------------------------------------- columns
+template <typename T, int V, int MODE>
+__global__ void __launch_bounds__(256) fft_cols_kernel(const SplitParams<T> p) {
+    extern __shared__ unsigned char smem_raw[];
+    c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
+    constexpr int E = 256 * V;
+    const int n = p.ny;
+    const int cols_per = E / (n + 1);  // padded columns must fit the E-element buffer
+    const int groups_per_plane = (p.nx + cols_per - 1) / cols_per;
+    const int plane = blockIdx.x / groups_per_plane;
+    if (plane >= p.planes) return;
+    const int x0 = (blockIdx.x - plane * groups_per_plane) * cols_per;
+    const int count = min(cols_per, p.nx - x0);
+    const int64_t base = (int64_t)plane * n * p.nx;
+    const int ld = n + 1;  // one-element pad per column: strided LDS stores hit distinct banks
+    // load column segments: element (y, x0 + c) -> buf[c * ld + y]
+    for (int i = threadIdx.x; i < count * n; i += 256) {
+        const int y = i / count;
+        const int c = i - y * count;
+        buf[c * ld + y] = p.in[base + (int64_t)y * p.nx + x0 + c];
+    }
+    __syncthreads();
+    lds_fft<T, V>(buf, p.dims, ld, count, p.tw, MODE == COL_INV);
+    if constexpr (MODE == COL_PROP) {
+        // x exp(-i hbar k^2 h / (2 m)), evaluated in double as the reference does
+        const pgpe_member& mb = p.table[plane];
+        const double hbar = mb.hbar;
+        const double inv2m = 1.0 / (2 * mb.m_eff);
+        for (int i = threadIdx.x; i < count * n; i += 256) {
+            const int c = i / n;
+            const int y = i - c * n;
+            const double kxv = p.kx[x0 + c], kyv = p.ky[y];
+            const double k2 = kxv * kxv + kyv * kyv;
+            const double theta = ((-(hbar * k2)) * p.prop_h) * inv2m;
+            double s, co;
+            sincos(theta, &s, &co);
+            const c2<T> v = buf[c * ld + y];
+            const double re = (double)v.x * co - (double)v.y * s;
+            const double im = (double)v.x * s + (double)v.y * co;
+            buf[c * ld + y] = make_c<T>((T)re, (T)im);
+        }
+        __syncthreads();
+        lds_fft<T, V>(buf, p.dims, ld, count, p.tw, true);
+    }
+    for (int i = threadIdx.x; i < count * n; i += 256) {
+        const int y = i / count;
+        const int c = i - y * count;
+        p.out[base + (int64_t)y * p.nx + x0 + c] = buf[c * ld + y];
+    }
+}
+
+template <typename T>
+static void twiddles(int n, std::vector<c2<T>>& out) {
+    out.resize(n);
+    for (int m = 0; m < n; ++m) {
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+        out[m] = make_c<T>((T)(double)cosl(a), (T)(double)sinl(a));
+    }
+}
+
+// k = 2 pi fftfreq(n, d): integer index times 1/(n d), then times 2 pi (periodic_grid.py:66-67).
+static void wavenumbers(int n, double d, std::vector<double>& k) {
+    k.resize(n);
+    const double val = 1.0 / (n * d);
+    const int N = (n - 1) / 2 + 1;
+    for (int i = 0; i < n; ++i) {
+        const int f = i < N ? i : i - n;
+        k[i] = 2.0 * 3.141592653589793 * ((double)f * val);
+    }
+}
+
+template <typename T, int MODE, int RES>
+static int launch_rows(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipStream_t s) {
+    SplitParams<T> p = p0;
+    p.dims = plan->dx_dims;
+    p.tw = (const c2<T>*)plan->tw_x;
+    const int V = plan->vx;
+    const int E = 256 * V;
+    const int rows_per = E / plan->nx;
+    const int total_rows = p.planes * p.ny;
+    const dim3 grid((total_rows + rows_per - 1) / rows_per);
+    const size_t shm = (size_t)E * sizeof(c2<T>);
+    if (V == 16)
+        hipLaunchKernelGGL((fft_rows_kernel<T, 16, MODE, RES>), grid, dim3(256), shm, s, p);
+    else
+        hipLaunchKernelGGL((fft_rows_kernel<T, 12, MODE, RES>), grid, dim3(256), shm, s, p);
+    PGPE_LAUNCHED("fft_rows_kernel");
+    return PGPE_OK;
+}
+
+template <typename T, int MODE>
+static int launch_cols(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipStream_t s) {
+    SplitParams<T> p = p0;
+    p.dims = plan->dy_dims;
+    p.tw = (const c2<T>*)plan->tw_y;
+    const int V = plan->vy;
+    const int E = 256 * V;
+    const int cols_per = E / (plan->ny + 1);
+    const int groups = (plan->nx + cols_per - 1) / cols_per;
+    const dim3 grid(groups * p.planes);
+    const size_t shm = (size_t)E * sizeof(c2<T>);
+    if (V == 16)
+        hipLaunchKernelGGL((fft_cols_kernel<T, 16, MODE>), grid, dim3(256), shm, s, p);
+    else
+        hipLaunchKernelGGL((fft_cols_kernel<T, 12, MODE>), grid, dim3(256), shm, s, p);
+    PGPE_LAUNCHED("fft_cols_kernel");
+    return PGPE_OK;
+}
+
+template <typename T>
+static int fft2_typed(const pgpe_fft_plan* plan, const void* in, void* out, int planes, int inverse, hipStream_t s) {
+    SplitParams<T> p{};
+    p.nx = plan->nx;
+    p.ny = plan->ny;
+    p.planes = planes;
+    p.kx = plan->kx;
+    p.ky = plan->ky;
+    p.scale = (T)(1.0 / ((double)plan->nx * (double)plan->ny));
+    int rc;
+    if (!inverse) {
+        p.in = (c2<T>*)in;
+        p.out = (c2<T>*)out;
+        if ((rc = launch_rows<T, ROW_FWD, 0>(plan, p, s))) return rc;
+        p.in = (c2<T>*)out;
+        return launch_cols<T, COL_FWD>(plan, p, s);
+    }
+    p.in = (c2<T>*)in;
+    p.out = (c2<T>*)out;
+    if ((rc = launch_cols<T, COL_INV>(plan, p, s))) return rc;
+    p.in = (c2<T>*)out;
+    return launch_rows<T, ROW_INV, 0>(plan, p, s);
+}
+
+template <typename T, int RES>
+static int split_typed(const pgpe_split_args* a, hipStream_t s) {
+    const pgpe_fft_plan* plan = (const pgpe_fft_plan*)a->plan;
+    SplitParams<T> p{};
+    p.table = a->table;
+    p.kx = plan->kx;
+    p.ky = plan->ky;
+    p.nx = plan->nx;
+    p.ny = plan->ny;
+    p.planes = a->members;
+    p.members = a->members;
+    p.reservoir = a->reservoir;
+    p.noise = a->noise;
+    p.dt = (T)a->dt;
+    p.prop_h = a->dt / 2.0;
+    p.scale = (T)(1.0 / ((double)plan->nx * (double)plan->ny));
+    p.res[0] = (T*)a->res[0];
+    p.res[1] = (T*)a->res[1];
+    p.pump = (const T*)a->pump;
+    p.coef = (const T*)a->coef;
+    p.step = a->step;
+    p.step_base = a->step_base;
+    c2<T>* psi = (c2<T>*)a->psi;
+    c2<T>* tmp = (c2<T>*)a->tmp;
+    int rc;
+    // pass 1
+    p.in = psi;
+    p.out = tmp;
+    if (a->ordering == 0) rc = launch_rows<T, ROW_SS_FIRST, RES>(plan, p, s);
+    else rc = launch_rows<T, ROW_FWD, RES>(plan, p, s);
+    if (rc) return rc;
+    // pass 2: propagator along y
+    p.in = tmp;
+    p.out = tmp;
+    if ((rc = launch_cols<T, COL_PROP>(plan, p, s))) return rc;
+    // pass 3: local physics between inverse and forward row transforms
+    if (a->ordering == 0) rc = launch_rows<T, ROW_SS_MID, RES>(plan, p, s);
+    else rc = launch_rows<T, ROW_KO_MID, RES>(plan, p, s);
+    if (rc) return rc;
+    // pass 4
+    if ((rc = launch_cols<T, COL_PROP>(plan, p, s))) return rc;
+    // pass 5: inverse rows back into psi (+ second reservoir half-step for the CPU order)
+    p.out = psi;
+    if (a->ordering == 0) return launch_rows<T, ROW_SS_LAST, RES>(plan, p, s);
+    return launch_rows<T, ROW_INV, RES>(plan, p, s);
+}
+
+}  // namespace pgpe
+
+using namespace pgpe;
+
+extern "C" {
+
+int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pgpe_fft_plan** out) {
+    if (!out) return fail(PGPE_ERR_ARG, "pgpe_fft_plan_create: null output");
+    *out = nullptr;
+    if (precision != 0 && precision != 1) return fail(PGPE_ERR_ARG, "pgpe_fft_plan_create: precision must be 0 or 1");
+    FftDims fx{}, fy{};
+    int vx = 0, vy = 0;
+    if (!factor(nx, fx, vx) || !factor(ny, fy, vy))
+        return fail(PGPE_ERR_UNSUPPORTED, "pgpe_fft_plan_create: sizes must be 2^a or 3*2^a with 2 <= n <= 3072");
+    pgpe_fft_plan* plan = new (std::nothrow) pgpe_fft_plan{};
+    if (!plan) return fail(PGPE_ERR_ARG, "pgpe_fft_plan_create: out of host memory");
+    plan->nx = nx;
+    plan->ny = ny;
+    plan->dx = dx;
+    plan->dy = dy;
+    plan->precision = precision;
+    plan->dx_dims = fx;
+    plan->dy_dims = fy;
+    plan->vx = vx;
+    plan->vy = vy;
+    std::vector<double> kx, ky;
+    wavenumbers(nx, dx, kx);
+    wavenumbers(ny, dy, ky);
+    const size_t csz = precision ? sizeof(double2) : sizeof(float2);
+    hipError_t e = hipMalloc(&plan->tw_x, csz * nx);
+    if (e == hipSuccess) e = hipMalloc(&plan->tw_y, csz * ny);
+    if (e == hipSuccess) e = hipMalloc((void**)&plan->kx, sizeof(double) * nx);
+    if (e == hipSuccess) e = hipMalloc((void**)&plan->ky, sizeof(double) * ny);
+    if (e == hipSuccess) {
+        if (precision) {
+            std::vector<double2> tx, ty;
+            twiddles<double>(nx, tx);
+            twiddles<double>(ny, ty);
+            e = hipMemcpy(plan->tw_x, tx.data(), csz * nx, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(plan->tw_y, ty.data(), csz * ny, hipMemcpyHostToDevice);
+        } else {
+            std::vector<float2> tx, ty;
+            twiddles<float>(nx, tx);
+            twiddles<float>(ny, ty);
+            e = hipMemcpy(plan->tw_x, tx.data(), csz * nx, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(plan->tw_y, ty.data(), csz * ny, hipMemcpyHostToDevice);
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpy(plan->kx, kx.data(), sizeof(double) * nx, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(plan->ky, ky.data(), sizeof(double) * ny, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(plan->tw_x);
+        hipFree(plan->tw_y);
+        hipFree(plan->kx);
+        hipFree(plan->ky);
+        delete plan;
+        return hip_fail(e, "pgpe_fft_plan_create");
+    }
+    *out = plan;
+    return PGPE_OK;
+}
+
+int pgpe_fft_plan_destroy(pgpe_fft_plan* plan) {
+    if (!plan) return PGPE_OK;
+    hipFree(plan->tw_x);
+    hipFree(plan->tw_y);
+    hipFree(plan->kx);
+    hipFree(plan->ky);
+    delete plan;
+    return PGPE_OK;
+}
+
+int pgpe_fft2(const pgpe_fft_plan* plan, const void* in, void* out, int planes, int inverse, void* stream) {
+    if (!plan || !in || !out) return fail(PGPE_ERR_ARG, "pgpe_fft2: null argument");
+    if (planes <= 0) return fail(PGPE_ERR_ARG, "pgpe_fft2: planes must be positive");
+    if (plan->precision == 0) return fft2_typed<float>(plan, in, out, planes, inverse, (hipStream_t)stream);
+    return fft2_typed<double>(plan, in, out, planes, inverse, (hipStream_t)stream);
+}
+
+int pgpe_split_step(const pgpe_split_args* a, void* stream) {
+    if (!a || !a->plan || !a->table || !a->psi || !a->tmp || !a->pump || !a->coef)
+        return fail(PGPE_ERR_ARG, "pgpe_split_step: null argument");
+    if (a->members <= 0) return fail(PGPE_ERR_ARG, "pgpe_split_step: members must be positive");
+    if (a->ordering != 0 && a->ordering != 1) return fail(PGPE_ERR_ARG, "pgpe_split_step: ordering must be 0 or 1");
+    const pgpe_fft_plan* plan = (const pgpe_fft_plan*)a->plan;
+    if (plan->precision != a->precision) return fail(PGPE_ERR_ARG, "pgpe_split_step: plan precision mismatch");
+    const int nf = a->reservoir == PGPE_RES_NONE ? 0 : (a->reservoir == PGPE_RES_SINGLE ? 1 : 2);
+    for (int f = 0; f < nf; ++f)
+        if (!a->res[f]) return fail(PGPE_ERR_ARG, "pgpe_split_step: null reservoir field");
+    hipStream_t s = (hipStream_t)stream;
+    const bool dbl = a->precision == 1;
+    switch (a->reservoir) {
+        case PGPE_RES_NONE: return dbl ? split_typed<double, 0>(a, s) : split_typed<float, 0>(a, s);
+        case PGPE_RES_SINGLE: return dbl ? split_typed<double, 1>(a, s) : split_typed<float, 1>(a, s);
+        case PGPE_RES_DOUBLE: return dbl ? split_typed<double, 2>(a, s) : split_typed<float, 2>(a, s);
+        case PGPE_RES_QDOUBLE: return dbl ? split_typed<double, 3>(a, s) : split_typed<float, 3>(a, s);
+        default: return fail(PGPE_ERR_ARG, "pgpe_split_step: unknown reservoir kind");
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,286 @@
+// BF16 MFMA kernels of the Fourier-neural-operator surrogate.
+//
+// pgpe_gemm: batched strided C = alpha op(A) op(B) + beta C with bf16 A and
+// B, fp32 accumulation on the gfx950 matrix cores
+// (v_mfma_f32_32x32x16_bf16), and a fused epilogue (bias along rows or
+// columns, GELU, residual add) storing fp32 or bf16.  Every FNO layer --
+// the truncated DFTs, the per-mode channel mixing, the pointwise channel
+// linear maps -- and every backward product is one call of it.
+//
+// Tiling: a 256-thread workgroup (4 wavefronts) owns a 128 x 128 tile of C;
+// each wavefront a 64 x 64 quarter made of 2 x 2 MFMA 32 x 32 blocks.  K
+// advances 32 at a time through LDS tiles stored k-contiguous (As[m][k],
+// Bs[n][k], padded) so each lane fetches its 8 consecutive k values as one
+// 16-byte LDS read -- the MFMA operand layout: lane l holds row (l % 32),
+// k = 8 (l / 32) .. +7.  The 16 accumulators of a lane are rows
+// 8 (r / 4) + 4 (l / 32) + (r % 4), column l % 32.
+#include <hip/hip_bf16.h>
+
+#include "common.hpp"
+
+namespace pgpe {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int GT_M = 128, GT_N = 128, GT_K = 32, GT_PAD = 8;
+
+struct GemmParams {
+    const __hip_bfloat16* A;
+    const __hip_bfloat16* B;
+    void* C;
+    const float* bias;
+    const float* residual;   // fp32, same strides as C (may alias C only if beta == 0)
+    int M, N, K;
+    int64_t sam, sak, sab;   // A strides (elements): row, k, batch
+    int64_t sbk, sbn, sbb;
+    int64_t scm, scn, scb;
+    int64_t srb;             // residual batch stride
+    float alpha, beta;
+    int bias_mode;           // 0 none, 1 per row (m), 2 per column (n)
+    int gelu;                // 1: apply tanh-free GELU (erf form)
+    int out_bf16;
+};
+
+__device__ __forceinline__ float gelu_erf(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752440f)); }
+
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmParams p) {
+    __shared__ __hip_bfloat16 As[GT_M][GT_K + GT_PAD];
+    __shared__ __hip_bfloat16 Bs[GT_N][GT_K + GT_PAD];
+    const int bz = blockIdx.z;
+    const int m0 = blockIdx.y * GT_M;
+    const int n0 = blockIdx.x * GT_N;
+    const __hip_bfloat16* A = p.A + bz * p.sab;
+    const __hip_bfloat16* B = p.B + bz * p.sbb;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    const __hip_bfloat16 zero = __float2bfloat16(0.0f);
+    for (int k0 = 0; k0 < p.K; k0 += GT_K) {
+        // stage A[m0 .. +128][k0 .. +32] and B[k0 .. +32][n0 .. +128] into LDS
+        for (int e = tid; e < GT_M * GT_K; e += 256) {
+            int mm, kk;
+            if (p.sak == 1) { mm = e / GT_K; kk = e % GT_K; } else { kk = e / GT_M; mm = e % GT_M; }
+            const int gm = m0 + mm, gk = k0 + kk;
+            As[mm][kk] = (gm < p.M && gk < p.K) ? A[gm * p.sam + gk * p.sak] : zero;
+        }
+        for (int e = tid; e < GT_N * GT_K; e += 256) {
+            int nn, kk;
+            if (p.sbk == 1) { nn = e / GT_K; kk = e % GT_K; } else { kk = e / GT_N; nn = e % GT_N; }
+            const int gn = n0 + nn, gk = k0 + kk;
+            Bs[nn][kk] = (gn < p.N && gk < p.K) ? B[gk * p.sbk + gn * p.sbn] : zero;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < GT_K; ks += 16) {
+            bf16x8 a[2], b[2];
+            const int kk = ks + 8 * (lane >> 5);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&As[wm + 32 * i + (lane & 31)][kk]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn + 32 * j + (lane & 31)][kk]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // epilogue
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int gm = m0 + wm + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int gn = n0 + wn + 32 * j + (lane & 31);
+                if (gm >= p.M || gn >= p.N) continue;
+                const int64_t off = bz * p.scb + gm * p.scm + gn * p.scn;
+                float v = p.alpha * acc[i][j][r];
+                if (p.beta != 0.0f) {
+                    const float old = p.out_bf16 ? __bfloat162float(((const __hip_bfloat16*)p.C)[off])
+                                                 : ((const float*)p.C)[off];
+                    v += p.beta * old;
+                }
+                if (p.bias_mode == 1) v += p.bias[gm];
+                else if (p.bias_mode == 2) v += p.bias[gn];
+                if (p.residual) v += p.residual[bz * p.srb + gm * p.scm + gn * p.scn];
+                if (p.gelu) v = gelu_erf(v);
+                if (p.out_bf16) ((__hip_bfloat16*)p.C)[off] = __float2bfloat16(v);
+                else ((float*)p.C)[off] = v;
+            }
+}
+
+// ---------------------------------------------------------------- elementwise
+__global__ void f32_to_bf16_kernel(const float* __restrict__ x, __hip_bfloat16* __restrict__ y, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = __float2bfloat16(x[i]);
+}
+
+// y = gelu(x) (fp32 in, fp32 and optional bf16 copy out)
+__global__ void gelu_fwd_kernel(const float* __restrict__ x, float* __restrict__ y, __hip_bfloat16* __restrict__ yb,
+                                int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = gelu_erf(x[i]);
+    if (y) y[i] = v;
+    if (yb) yb[i] = __float2bfloat16(v);
+}
+
+// dx = dy * gelu'(x), gelu'(x) = Phi(x) + x phi(x)
+__global__ void gelu_bwd_kernel(const float* __restrict__ x, const float* __restrict__ dy, float* __restrict__ dx,
+                                __hip_bfloat16* __restrict__ dxb, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = x[i];
+    const float cdf = 0.5f * (1.0f + erff(v * 0.70710678118654752440f));
+    const float pdf = 0.39894228040143267794f * expf(-0.5f * v * v);
+    const float g = dy[i] * (cdf + v * pdf);
+    if (dx) dx[i] = g;
+    if (dxb) dxb[i] = __float2bfloat16(g);
+}
+
+// Complex diagonal mode scaling over interleaved (re, im) planes:
+// y[b, c, q] = d[c, q] * x[b, c, q] for complex q (conj = 1 uses conj(d)).
+__global__ void cdiag_kernel(const float2* __restrict__ x, const float2* __restrict__ d, float2* __restrict__ y,
+                             __hip_bfloat16* __restrict__ yb, int64_t per_batch, int64_t n, int conj) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 a = x[i];
+    float2 w = d[i % per_batch];
+    if (conj) w.y = -w.y;
+    const float2 r = make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+    if (y) y[i] = r;
+    if (yb) {
+        yb[2 * i] = __float2bfloat16(r.x);
+        yb[2 * i + 1] = __float2bfloat16(r.y);
+    }
+}
+
+// dd[c, q] = sum_b conj(x[b, c, q]) * g[b, c, q]  (gradient of the diagonal weights)
+__global__ void cdiag_grad_kernel(const float2* __restrict__ x, const float2* __restrict__ g, float2* __restrict__ dd,
+                                  int64_t per_batch, int batch) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= per_batch) return;
+    float sr = 0.0f, si = 0.0f;
+    for (int b = 0; b < batch; ++b) {
+        const float2 a = x[b * per_batch + i], q = g[b * per_batch + i];
+        sr += a.x * q.x + a.y * q.y;
+        si += a.x * q.y - a.y * q.x;
+    }
+    dd[i] = make_float2(sr, si);
+}
+
+// out[n] = sum over rows of x[row, n] (bias gradients), rows x cols fp32 row-major.
+__global__ void colsum_kernel(const float* __restrict__ x, float* __restrict__ out, int64_t rows, int64_t cols,
+                              int64_t row_stride, int64_t col_stride) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.0f;
+    for (int64_t r = 0; r < rows; ++r) s += x[r * row_stride + c * col_stride];
+    out[c] = s;
+}
+
+static dim3 blocks1d(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+}  // namespace pgpe
+
+using namespace pgpe;
+
+extern "C" {
+
+int pgpe_gemm_bf16(const pgpe_gemm_args* a, void* stream) {
+    if (!a || !a->A || !a->B || !a->C) return fail(PGPE_ERR_ARG, "pgpe_gemm_bf16: null pointer");
+    if (a->M <= 0 || a->N <= 0 || a->K <= 0 || a->batch <= 0) return fail(PGPE_ERR_ARG, "pgpe_gemm_bf16: empty shape");
+    if ((a->sak != 1 && a->sam != 1) || (a->sbk != 1 && a->sbn != 1))
+        return fail(PGPE_ERR_ARG, "pgpe_gemm_bf16: A and B need a unit stride along k or along m / n");
+    if (a->bias_mode && !a->bias) return fail(PGPE_ERR_ARG, "pgpe_gemm_bf16: bias_mode without bias");
+    GemmParams p;
+    p.A = (const __hip_bfloat16*)a->A;
+    p.B = (const __hip_bfloat16*)a->B;
+    p.C = a->C;
+    p.bias = a->bias;
+    p.residual = a->residual;
+    p.M = a->M;
+    p.N = a->N;
+    p.K = a->K;
+    p.sam = a->sam; p.sak = a->sak; p.sab = a->sab;
+    p.sbk = a->sbk; p.sbn = a->sbn; p.sbb = a->sbb;
+    p.scm = a->scm; p.scn = a->scn; p.scb = a->scb;
+    p.srb = a->srb;
+    p.alpha = a->alpha;
+    p.beta = a->beta;
+    p.bias_mode = a->bias_mode;
+    p.gelu = a->gelu;
+    p.out_bf16 = a->out_bf16;
+    const dim3 grid((a->N + GT_N - 1) / GT_N, (a->M + GT_M - 1) / GT_M, a->batch);
+    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream, p);
+    PGPE_LAUNCHED("gemm_bf16_kernel");
+    return PGPE_OK;
+}
+
+int pgpe_f32_to_bf16(const float* x, void* y, int64_t n, void* stream) {
+    if (n <= 0) return PGPE_OK;
+    if (!x || !y) return fail(PGPE_ERR_ARG, "pgpe_f32_to_bf16: null pointer");
+    hipLaunchKernelGGL(f32_to_bf16_kernel, blocks1d(n), dim3(256), 0, (hipStream_t)stream, x, (__hip_bfloat16*)y, n);
+    PGPE_LAUNCHED("f32_to_bf16_kernel");
+    return PGPE_OK;
+}
+
+int pgpe_gelu_fwd(const float* x, float* y, void* yb, int64_t n, void* stream) {
+    if (n <= 0) return PGPE_OK;
+    if (!x || (!y && !yb)) return fail(PGPE_ERR_ARG, "pgpe_gelu_fwd: null pointer");
+    hipLaunchKernelGGL(gelu_fwd_kernel, blocks1d(n), dim3(256), 0, (hipStream_t)stream, x, y, (__hip_bfloat16*)yb, n);
+    PGPE_LAUNCHED("gelu_fwd_kernel");
+    return PGPE_OK;
+}
+
+int pgpe_gelu_bwd(const float* x, const float* dy, float* dx, void* dxb, int64_t n, void* stream) {
+    if (n <= 0) return PGPE_OK;
+    if (!x || !dy || (!dx && !dxb)) return fail(PGPE_ERR_ARG, "pgpe_gelu_bwd: null pointer");
+    hipLaunchKernelGGL(gelu_bwd_kernel, blocks1d(n), dim3(256), 0, (hipStream_t)stream, x, dy, dx,
+                       (__hip_bfloat16*)dxb, n);
+    PGPE_LAUNCHED("gelu_bwd_kernel");
+    return PGPE_OK;
+}
+
+int pgpe_cdiag(const float* x, const float* d, float* y, void* yb, int64_t per_batch, int64_t n, int conj,
+               void* stream) {
+    if (n <= 0) return PGPE_OK;
+    if (!x || !d || (!y && !yb) || per_batch <= 0) return fail(PGPE_ERR_ARG, "pgpe_cdiag: bad argument");
+    hipLaunchKernelGGL(cdiag_kernel, blocks1d(n), dim3(256), 0, (hipStream_t)stream, (const float2*)x,
+                       (const float2*)d, (float2*)y, (__hip_bfloat16*)yb, per_batch, n, conj);
+    PGPE_LAUNCHED("cdiag_kernel");
+    return PGPE_OK;
+}
+
+int pgpe_cdiag_grad(const float* x, const float* g, float* dd, int64_t per_batch, int batch, void* stream) {
+    if (per_batch <= 0 || batch <= 0) return fail(PGPE_ERR_ARG, "pgpe_cdiag_grad: empty");
+    if (!x || !g || !dd) return fail(PGPE_ERR_ARG, "pgpe_cdiag_grad: null pointer");
+    hipLaunchKernelGGL(cdiag_grad_kernel, blocks1d(per_batch), dim3(256), 0, (hipStream_t)stream, (const float2*)x,
+                       (const float2*)g, (float2*)dd, per_batch, batch);
+    PGPE_LAUNCHED("cdiag_grad_kernel");
+    return PGPE_OK;
+}
+
+int pgpe_colsum(const float* x, float* out, int64_t rows, int64_t cols, int64_t row_stride, int64_t col_stride,
+                void* stream) {
+    if (rows <= 0 || cols <= 0) return fail(PGPE_ERR_ARG, "pgpe_colsum: empty");
+    if (!x || !out) return fail(PGPE_ERR_ARG, "pgpe_colsum: null pointer");
+    hipLaunchKernelGGL(colsum_kernel, blocks1d(cols), dim3(256), 0, (hipStream_t)stream, x, out, rows, cols,
+                       row_stride, col_stride);
+    PGPE_LAUNCHED("colsum_kernel");
+    return PGPE_OK;
+}
+
+}  // extern "C"

@@ -32,13 +32,13 @@ __global__ void normal_pairs_kernel(c2<T>* __restrict__ out, uint64_t key, uint6
 template <typename T>
 __global__ void __launch_bounds__(256) wigner_add_kernel(c2<T>* __restrict__ psi, const T* __restrict__ active,
                                                          int active_components, const pgpe_member* __restrict__ table,
-                                                         const int32_t* __restrict__ tile_member, int num_tiles,
+                                                         const int32_t* __restrict__ tile_map, int num_tiles,
                                                          uint64_t step) {
     const int tile = xcd_tile(blockIdx.x, num_tiles);
     if (tile < 0) return;
-    const int m = tile_member[tile];
+    const int m = tile_map[2 * tile];
     const pgpe_member& mb = table[m];
-    const int local = tile - mb.tile_begin;
+    const int local = tile_map[2 * tile + 1];
     const int tx = local % mb.tiles_x;
     const int ty = local / mb.tiles_x;
     const int x = tx * kTileW + (threadIdx.x & 63);
@@ -100,19 +100,19 @@ int pgpe_normal_pairs(void* out, int precision, uint64_t key, uint64_t step, uin
 }
 
 int pgpe_wigner_add(void* psi, const void* active, int active_components, const pgpe_member* table,
-                    const int32_t* tile_member, int num_tiles, int members, int precision, uint64_t step,
+                    const int32_t* tile_map, int num_tiles, int members, int precision, uint64_t step,
                     void* stream) {
-    if (!psi || !table || !tile_member) return fail(PGPE_ERR_ARG, "pgpe_wigner_add: null pointer");
+    if (!psi || !table || !tile_map) return fail(PGPE_ERR_ARG, "pgpe_wigner_add: null pointer");
     if (num_tiles <= 0 || members <= 0) return fail(PGPE_ERR_ARG, "pgpe_wigner_add: empty table");
     if (active && active_components != 1 && active_components != 2)
         return fail(PGPE_ERR_ARG, "pgpe_wigner_add: active_components must be 1 or 2");
     const dim3 grid(xcd_grid(num_tiles));
     if (precision == 0) {
         hipLaunchKernelGGL(wigner_add_kernel<float>, grid, dim3(256), 0, (hipStream_t)stream, (float2*)psi,
-                           (const float*)active, active_components, table, tile_member, num_tiles, step);
+                           (const float*)active, active_components, table, tile_map, num_tiles, step);
     } else if (precision == 1) {
         hipLaunchKernelGGL(wigner_add_kernel<double>, grid, dim3(256), 0, (hipStream_t)stream, (double2*)psi,
-                           (const double*)active, active_components, table, tile_member, num_tiles, step);
+                           (const double*)active, active_components, table, tile_map, num_tiles, step);
     } else {
         return fail(PGPE_ERR_ARG, "pgpe_wigner_add: precision must be 0 or 1");
     }

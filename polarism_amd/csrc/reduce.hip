@@ -14,14 +14,14 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <typename T>
 __global__ void __launch_bounds__(256) density_delta_tiles(const c2<T>* __restrict__ psi, T* __restrict__ old,
                                                            int components, const pgpe_member* __restrict__ table,
-                                                           const int32_t* __restrict__ tile_member, int num_tiles,
+                                                           const int32_t* __restrict__ tile_map, int num_tiles,
                                                            double* __restrict__ partials, int update) {
     __shared__ double s_part[3][4];
     const int tile = xcd_tile(blockIdx.x, num_tiles);
     if (tile < 0) return;
-    const int m = tile_member[tile];
+    const int m = tile_map[2 * tile];
     const pgpe_member& mb = table[m];
-    const int local = tile - mb.tile_begin;
+    const int local = tile_map[2 * tile + 1];
     const int x = (local % mb.tiles_x) * kTileW + (threadIdx.x & 63);
     const int ybase = (local / mb.tiles_x) * mb.tile_h;
     const int64_t plane = (int64_t)mb.nx * mb.ny;
@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) density_delta_tiles(const c2<T>* __restri
     __syncthreads();
     if (threadIdx.x < 3) {
         const int q = threadIdx.x;
-        partials[3 * (int64_t)tile + q] = ((s_part[q][0] + s_part[q][1]) + s_part[q][2]) + s_part[q][3];
+        partials[3 * ((int64_t)mb.tile_begin + local) + q] = ((s_part[q][0] + s_part[q][1]) + s_part[q][2]) + s_part[q][3];
     }
 }
 
@@ -84,9 +84,9 @@ __global__ void density_delta_members(const pgpe_member* __restrict__ table, con
 using namespace pgpe;
 
 extern "C" int pgpe_density_delta(const void* psi, void* old, int precision, int components, const pgpe_member* table,
-                                  const int32_t* tile_member, int num_tiles, int members, double* partials,
+                                  const int32_t* tile_map, int num_tiles, int members, double* partials,
                                   double* sums, int update, void* stream) {
-    if (!psi || !old || !table || !tile_member || !partials || !sums)
+    if (!psi || !old || !table || !tile_map || !partials || !sums)
         return fail(PGPE_ERR_ARG, "pgpe_density_delta: null pointer");
     if (num_tiles <= 0 || members <= 0) return fail(PGPE_ERR_ARG, "pgpe_density_delta: empty table");
     if (components != 1 && components != 2) return fail(PGPE_ERR_ARG, "pgpe_density_delta: components must be 1 or 2");
@@ -94,10 +94,10 @@ extern "C" int pgpe_density_delta(const void* psi, void* old, int precision, int
     const dim3 grid(xcd_grid(num_tiles));
     if (precision == 0)
         hipLaunchKernelGGL(density_delta_tiles<float>, grid, dim3(256), 0, s, (const float2*)psi, (float*)old,
-                           components, table, tile_member, num_tiles, partials, update);
+                           components, table, tile_map, num_tiles, partials, update);
     else if (precision == 1)
         hipLaunchKernelGGL(density_delta_tiles<double>, grid, dim3(256), 0, s, (const double2*)psi, (double*)old,
-                           components, table, tile_member, num_tiles, partials, update);
+                           components, table, tile_map, num_tiles, partials, update);
     else
         return fail(PGPE_ERR_ARG, "pgpe_density_delta: precision must be 0 or 1");
     PGPE_LAUNCHED("density_delta_tiles");

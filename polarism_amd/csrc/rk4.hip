@@ -25,7 +25,7 @@ namespace pgpe {
 template <typename T>
 struct Rk4Params {
     const pgpe_member* table;
-    const int32_t* tile_member;
+    const int32_t* tile_map;
     int num_tiles;
     int noise;
     T a;          // stage-input scale: 0, dt/2, dt/2, dt
@@ -81,10 +81,10 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
 
     const int tile = xcd_tile(blockIdx.x, p.num_tiles);
     if (tile < 0) return;
-    const int m = p.tile_member[tile];
+    const int m = p.tile_map[2 * tile];
     const pgpe_member& mb = p.table[m];
     const int nx = mb.nx, ny = mb.ny;
-    const int local = tile - mb.tile_begin;
+    const int local = p.tile_map[2 * tile + 1];
     const int x0 = (local % mb.tiles_x) * kTileW;
     const int y0 = (local / mb.tiles_x) * TH;
     const int64_t off = mb.off;
@@ -227,7 +227,7 @@ template <typename T>
 static int launch_typed(const pgpe_rk4_args* a, hipStream_t s) {
     Rk4Params<T> p{};
     p.table = a->table;
-    p.tile_member = a->tile_member;
+    p.tile_map = a->tile_map;
     p.num_tiles = a->num_tiles;
     p.noise = a->noise;
     static const double scale[4] = {0.0, 0.5, 0.5, 1.0};
@@ -265,7 +265,7 @@ static int check_args(const pgpe_rk4_args* a) {
     if (!a) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null args");
     if (a->stage < 0 || a->stage > 3) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: stage must be 0..3");
     if (a->num_tiles <= 0 || a->members <= 0) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: empty member table");
-    if (!a->table || !a->tile_member || !a->psi0 || !a->pump || !a->coef)
+    if (!a->table || !a->tile_map || !a->psi0 || !a->pump || !a->coef)
         return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null table, field, pump or coefficient");
     const int nf = a->reservoir == PGPE_RES_NONE ? 0 : (a->reservoir == PGPE_RES_SINGLE ? 1 : 2);
     for (int f = 0; f < nf; ++f)

@@ -2,8 +2,9 @@
 
 Members of a batch (equal or uneven grids) are packed back to back in every
 per-point device array; member ``m`` starts at ``offsets[m]`` grid points.
-Each member is cut into 64 x ``tile_h`` tiles; ``tile_member[t]`` names the
-member that owns tile ``t``.  One launch of any stencil kernel covers every
+Each member is cut into 64 x ``tile_h`` tiles; ``tile_map[t] = (member,
+local tile)`` lists the tiles one launch covers, so a subset (members that
+have converged removed) is just a shorter list.  One launch of any stencil kernel covers every
 tile of every member, which is how an uneven-grid sweep runs as one launch
 (an addition; the reference runs each grid shape as its own batch,
 sweep_config.py:18 and sweep.py:128-138).
@@ -49,7 +50,7 @@ class MemberTable:
         self.members = len(self.specs)
         rows = (N.Member * self.members)()
         offsets = []
-        tile_member = []
+        tile_map = []
         off = 0
         tile = 0
         for m, s in enumerate(self.specs):
@@ -70,6 +71,7 @@ class MemberTable:
             row.hbar_over_2m = get("hbar") / (2.0 * get("m_eff"))
             row.inv_hbar = 1.0 / get("hbar")
             row.kin = -(get("hbar") ** 2) / (2 * get("m_eff"))
+            row.hbar, row.m_eff = get("hbar"), get("m_eff")
             row.g_C, row.g_R, row.g_I = get("g_C"), get("g_R"), get("g_I")
             row.R, row.gamma_C = get("R"), get("gamma_C")
             row.eta = get("kinetic_relaxation_eta")
@@ -84,7 +86,7 @@ class MemberTable:
             row.pump_w_plus = (1 + s.circular_polarization) / 2
             row.pump_w_minus = (1 - s.circular_polarization) / 2
             n_tiles = row.tiles_x * row.tiles_y
-            tile_member.extend([m] * n_tiles)
+            tile_map.extend((m, j) for j in range(n_tiles))
             offsets.append(off)
             off += s.nx * s.ny
             tile += n_tiles
@@ -92,18 +94,23 @@ class MemberTable:
         self.offsets = np.asarray(offsets, dtype=np.int64)
         self.total_points = off
         self.num_tiles = tile
-        self.tile_member_host = np.asarray(tile_member, dtype=np.int32)
+        self.tile_map_host = np.asarray(tile_map, dtype=np.int32).reshape(-1, 2)
         raw = np.frombuffer(bytes(rows), dtype=np.uint8)
         self.table_dev = DeviceArray.from_numpy(raw)
-        self.tile_member_dev = DeviceArray.from_numpy(self.tile_member_host)
+        self.tile_map_dev = DeviceArray.from_numpy(self.tile_map_host)
 
     @property
     def table_ptr(self) -> int:
         return self.table_dev.ptr
 
     @property
-    def tile_member_ptr(self) -> int:
-        return self.tile_member_dev.ptr
+    def tile_map_ptr(self) -> int:
+        return self.tile_map_dev.ptr
+
+    def tiles_of(self, members) -> np.ndarray:
+        """(member, local) pairs of the given members, in table order."""
+        keep = np.isin(self.tile_map_host[:, 0], np.asarray(list(members), dtype=np.int32))
+        return self.tile_map_host[keep]
 
     def member_points(self, m: int) -> int:
         s = self.specs[m]

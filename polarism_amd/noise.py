@@ -84,7 +84,7 @@ class NoiseApplier:
         if isinstance(active_density, DeviceArray):
             active_ptr, active_c = active_density.ptr, active_density.shape[1]
         from .device import _sp
-        N.call("pgpe_wigner_add", field.ptr, active_ptr, active_c, tab.table_ptr, tab.tile_member_ptr,
+        N.call("pgpe_wigner_add", field.ptr, active_ptr, active_c, tab.table_ptr, tab.tile_map_ptr,
                tab.num_tiles, tab.members, prec, int(step), _sp(stream))
 
     def apply_to_state(self, state, reservoir, step: int) -> None:

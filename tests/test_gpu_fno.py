@@ -1,0 +1,80 @@
+"""FNO surrogate: HIP forward and backward against a float64 CPU autograd reference."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def ref_forward(x, p, cfg):
+    """float64 restatement of the network with explicit DFT matrices (no bf16 rounding)."""
+    H, W, mx, my, L = cfg.H, cfg.W, cfg.mx, cfg.my, cfg.layers
+    ky = np.concatenate([np.arange(my // 2), np.arange(-(my // 2), 0)])
+    kx = np.arange(mx)
+    ey = torch.tensor(np.exp(-2j * np.pi * np.outer(ky, np.arange(H)) / H))      # (my, H)
+    ex = torch.tensor(np.exp(-2j * np.pi * np.outer(np.arange(W), kx) / W))      # (W, mx)
+    w = torch.tensor(np.where(kx == 0, 1.0, 2.0) / (H * W))
+    v = torch.einsum("oi,ibhw->obhw", p["lift_w"], x) + p["lift_b"][:, None, None, None]
+    for l in range(L):
+        vc = v.to(torch.complex128)
+        vh = torch.einsum("yh,cbhw,wx->cbyx", ey, vc, ex)
+        d = torch.complex(p[f"d{l}_re"], p[f"d{l}_im"])            # (C, my, mx)
+        u = d[:, None] * vh
+        M = torch.complex(p[f"m{l}_re"], p[f"m{l}_im"])            # (my, C, C)
+        wk = torch.einsum("yoi,ibyx->obyx", M, u)
+        s = torch.einsum("yh,obyx,wx->obhw", ey.conj(), wk * w, ex.conj()).real
+        h = torch.einsum("oi,ibhw->obhw", p[f"w{l}"], v) + p[f"b{l}"][:, None, None, None] + s
+        v = torch.nn.functional.gelu(h) if l < L - 1 else h
+    q = torch.nn.functional.gelu(torch.einsum("oi,ibhw->obhw", p["q1_w"], v) + p["q1_b"][:, None, None, None])
+    return torch.einsum("oi,ibhw->obhw", p["q2_w"], q) + p["q2_b"][:, None, None, None]
+
+
+def _setup(H=32, W=32, width=16, mx=8, my=8, B=2):
+    from polarism_amd.device import init
+    from polarism_amd.surrogate.fno import FNOConfig, FNOModule
+    init()
+    cfg = FNOConfig(H=H, W=W, width=width, layers=4, modes_x=mx, modes_y=my, hidden=32)
+    mod = FNOModule(cfg, seed=1)
+    torch.manual_seed(0)
+    x = torch.randn(cfg.fin, B, H, W, device="cuda")
+    return cfg, mod, x
+
+
+def test_fno_forward_matches_float64_reference():
+    cfg, mod, x = _setup()
+    out = mod(x)
+    torch.cuda.synchronize()
+    p64 = {k: v.detach().double().cpu() for k, v in mod.params.items()}
+    ref = ref_forward(x.double().cpu(), p64, cfg)
+    err = ((out.double().cpu() - ref).norm() / ref.norm()).item()
+    print("forward rel-L2 vs float64:", err)
+    assert err < 3e-2
+
+
+def test_fno_gradients_match_float64_autograd():
+    cfg, mod, x = _setup()
+    target = torch.randn(cfg.fout, x.shape[1], cfg.H, cfg.W, device="cuda")
+    out = mod(x)
+    loss = ((out - target) ** 2).mean()
+    loss.backward()
+    torch.cuda.synchronize()
+    p64 = {k: v.detach().double().cpu().requires_grad_(True) for k, v in mod.params.items()}
+    ref = ref_forward(x.double().cpu(), p64, cfg)
+    ref_loss = ((ref - target.double().cpu()) ** 2).mean()
+    ref_loss.backward()
+    worst = 0.0
+    for k, prm in mod.params.items():
+        g = prm.grad.double().cpu()
+        r = p64[k].grad
+        rel = ((g - r).norm() / max(r.norm().item(), 1e-30)).item()
+        cos = torch.nn.functional.cosine_similarity(g.flatten(), r.flatten(), dim=0).item()
+        print(f"{k}: rel-L2 {rel:.3e} cos {cos:.6f}")
+        worst = max(worst, rel)
+        assert cos > 0.995, k
+    assert worst < 0.1
+
+
+def test_fno_parameter_count_near_1p5M():
+    from polarism_amd.surrogate.fno import FNOConfig, param_count
+    n = param_count(FNOConfig())
+    assert 1.4e6 < n < 1.7e6

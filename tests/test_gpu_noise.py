@@ -77,7 +77,7 @@ def test_wigner_add_matches_oracle(precision, tol):
     tab = MemberTable(specs, 16)
     d_psi = device.DeviceArray.from_numpy(psi)
     d_act = device.DeviceArray.from_numpy(active)
-    N.call("pgpe_wigner_add", d_psi.ptr, d_act.ptr, 1, tab.table_ptr, tab.tile_member_ptr, tab.num_tiles,
+    N.call("pgpe_wigner_add", d_psi.ptr, d_act.ptr, 1, tab.table_ptr, tab.tile_map_ptr, tab.num_tiles,
            B, 0 if precision == "single" else 1, 77, None)
     ref = psi.copy()
     O.wigner_add(ref, active, seeds, [p.gamma_C] * B, [p.R] * B, strength, dt, dx * dy, 77, precision)

@@ -1,0 +1,86 @@
+"""split-step-fft-hip (CPU order) and split-step-fft-kouter-hip against the oracle."""
+import numpy as np
+import pytest
+
+from oracle import gpe_oracle as O
+from tests.helpers import oracle_physics, quickstart_config
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_split(cfg, n_steps, kouter=False, noise=False):
+    g = O.Grid(cfg.grid.nx, cfg.grid.ny, cfg.grid.lx, cfg.grid.ly)
+    prec = cfg.solver.precision
+    real = np.float32 if prec == "single" else np.float64
+    kind = cfg.reservoir.reservoir_type
+    p = oracle_physics(cfg)
+    psi = O.initial_psi(g.shape, cfg.noise.initial_amplitude, cfg.noise.seed, prec).reshape(1, 1, *g.shape)
+    nf = {"none": 0, "single": 1, "double": 2, "quadratic-double": 2}[kind]
+    res = tuple(np.zeros((1, 1, *g.shape), dtype=real) for _ in range(nf))
+    env = O.gaussian_profile(g, cfg.laser.sigma_space).astype(real)
+    pump_at = lambda t: env * cfg.laser.P0
+    nz = None
+    if noise:
+        nz = dict(seeds=[cfg.noise.seed], gamma=[p.gamma_C], scattering=[p.R],
+                  strength=[cfg.noise.stochastic_strength], volume=g.dx * g.dy, precision=prec)
+    dt = cfg.solver.dt
+    fn = O.split_step_kouter if kouter else O.split_step_cpu
+    for s in range(n_steps):
+        psi, res = fn(kind, p, g, psi, res, pump_at, s * dt, dt, s, nz)
+    return psi, res
+
+
+def _cfg(**kw):
+    base = dict(solver__method="split-step-fft-hip", grid__nx=128, grid__ny=96, grid__lx=64.0, grid__ly=48.0,
+                laser__P0=0.2, laser__sigma_space=10.0, solver__dt=5e-3, solver__total_time=0.25)
+    base.update(kw)
+    return quickstart_config(**base)
+
+
+def _run(cfg):
+    from polarism_amd import SimulationController
+    ctl = SimulationController(cfg)
+    ctl.run()
+    return ctl
+
+
+@pytest.mark.parametrize("method,kouter", [("split-step-fft-hip", False), ("split-step-fft-kouter-hip", True)])
+@pytest.mark.parametrize("kind", ["single", "double", "quadratic-double", "none"])
+def test_split_step_fp64_matches_oracle(method, kouter, kind):
+    cfg = _cfg(solver__method=method, reservoir__reservoir_type=kind)
+    ctl = _run(cfg)
+    psi_ref, res_ref = _oracle_split(cfg, 50, kouter)
+    e = O.rel_l2(ctl.state.field.get(), psi_ref)
+    assert e <= 1e-10, e
+    for d, r in zip(ctl.reservoir.get_state(), res_ref):
+        assert O.rel_l2(d.get(), r) <= 1e-10
+
+
+@pytest.mark.parametrize("method,kouter", [("split-step-fft-hip", False), ("split-step-fft-kouter-hip", True)])
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("single", 1e-5)])
+def test_split_step_with_wigner_noise_matches_oracle(method, kouter, precision, tol):
+    cfg = _cfg(solver__method=method, solver__precision=precision, noise__stochastic="wigner")
+    cfg.noise.seed = 1234
+    ctl = _run(cfg)
+    psi_ref, res_ref = _oracle_split(cfg, 50, kouter, noise=True)
+    psi = ctl.state.field.get()
+    assert O.rel_l2(psi, psi_ref) <= tol
+    assert O.rel_l2(np.abs(psi) ** 2, np.abs(psi_ref) ** 2) <= tol
+    assert O.rel_l2(ctl.reservoir.get_state()[0].get(), res_ref[0]) <= tol
+
+
+def test_orderings_agree_to_second_order():
+    """The two orderings differ at O(dt^2) (reference test_split_step_fft_cuda.py:190-224)."""
+    errs = []
+    for dt in (0.01, 0.005):
+        a = _run(_cfg(solver__dt=dt, solver__total_time=0.5, laser__P0=0.3))
+        b = _run(_cfg(solver__method="split-step-fft-kouter-hip", solver__dt=dt, solver__total_time=0.5,
+                      laser__P0=0.3))
+        errs.append(O.rel_l2(a.state.field.get(), b.state.field.get()))
+    assert errs[1] < errs[0] / 2.5
+
+
+def test_split_step_graph_equals_direct_launches_bitwise():
+    a = _run(_cfg(noise__stochastic="wigner"))
+    b = _run(_cfg(noise__stochastic="wigner", solver__cuda_graphs=False))
+    np.testing.assert_array_equal(a.state.field.get(), b.state.field.get())

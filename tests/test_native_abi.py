@@ -44,7 +44,9 @@ def _c_layout(struct, fields):
     return [int(v) for v in out]
 
 
-@pytest.mark.parametrize("py,cname", [(N.Member, "pgpe_member"), (N.Rk4Args, "pgpe_rk4_args")])
+@pytest.mark.parametrize("py,cname", [(N.Member, "pgpe_member"), (N.Rk4Args, "pgpe_rk4_args"),
+                                     (N.SplitArgs, "pgpe_split_args"), (N.LawsonArgs, "pgpe_lawson_args"),
+                                     (N.GemmArgs, "pgpe_gemm_args")])
 def test_ctypes_struct_layout_matches_c(py, cname):
     names = [f[0] for f in py._fields_]
     got = _c_layout(cname, names)
