@@ -1,25 +1,30 @@
 #!/usr/bin/env python
-"""Benchmark: steady-state sweep members/s on one MI355X, plus the stencil roofline.
+"""Benchmark: steady-state sweep members/s on one MI355X, warm start vs plain.
 
-Workload (one "step" = one pass over the whole batch):
+Workload (one "step" = one pass over the whole held-out batch):
   256 held-out sweep members, 256 x 256 periodic grids (lx = ly = 128 um),
-  single reservoir, complex64, rk4-fdm-hip with dt = 0.05 ps, random pumps
-  (1-4 Gaussians / ring / square lattice, seeded) with peak P0 = U[0.8, 4] x the
-  uniform-pump threshold.  Each member runs from a noise-seeded cold start until
-  the reference's steady-state criterion holds (relative L2 density change and
-  relative norm change < 1e-6 between checks 1 ps apart) or 4000 ps elapse.
-  Converged members are retired from the launches.  value = converged members
-  per second of wall time, launch to last member done.
+  single reservoir, complex64, rk4-fdm-hip with dt = 0.05 ps, seeded random
+  pumps (1-4 Gaussians / ring / square lattice) with peak P0 = U[0.8, 4] x the
+  uniform-pump threshold.  Each member runs until the reference's steady-state
+  criterion holds (relative L2 density change and relative norm change both
+  < 1e-6 between checks 1 ps apart) or 4000 ps elapse; converged members are
+  retired from the launches.
+  plain: noise-seeded cold start.
+  warm : FNO surrogate prediction (psi, n_R) plus the same noise field.
+  members/s = converged members / wall time from launch to the last
+  convergence (warm start includes the surrogate inference).  The warm and
+  plain final densities of every member converged in both are compared
+  (relative L2, target <= 1e-3).
 
 Roofline leg: the middle RK4 stage kernel (FP32, single reservoir) on one
-8192 x 8192 member (537 MB per complex field, far beyond the 256 MB Infinity
+8192 x 8192 member (537 MB per complex field, beyond the 256 MB Infinity
 Cache), timed with HIP events on the solver stream.  Algorithmic bytes per
-point for that stage: psi0, k_prev (2 x 8 B), n0, kn_prev, P (3 x 4 B) in;
-k (8 B), kn (4 B) out = 40 B.
+point: psi0, k_prev (2 x 8 B), n0, kn_prev, P (3 x 4 B) in; k (8 B), kn (4 B)
+out = 40 B.
 
 cpu_baseline: the NumPy oracle's rk4-fdm step (complex64, one 256 x 256
-member) timed on one host core for a bounded number of steps; its point-step
-rate converts the GPU run's point-steps into the CPU time for the same sweep.
+member) timed on one host core for a bounded number of steps; its
+point-step rate converts the plain run's point-steps into CPU time.
 """
 from __future__ import annotations
 
@@ -36,31 +41,26 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 SWEEP = dict(members=256, n=256, L=128.0, dt=0.05, check_every=1.0, rtol=1e-6, max_time=4000.0, seed=777)
-
-
-def held_out_members(count, seed):
-    from polarism_amd.config import PhysicsConstants
-    from polarism_amd.pumps import random_pump, threshold
-    from polarism_amd.sweep import SweepMember
-    p = PhysicsConstants()
-    rng = np.random.default_rng(seed)
-    pth = threshold(p, "single")
-    n, L = SWEEP["n"], SWEEP["L"]
-    out = []
-    for i in range(count):
-        env, desc = random_pump(rng, n, n, L, L)
-        ratio = float(rng.uniform(0.8, 4.0))
-        out.append(SweepMember(n, n, L, L, p, env, ratio * pth, seed=seed * 100003 + i,
-                               label={"ratio": ratio, "kind": desc["kind"]}))
-    return out
+WEIGHTS = os.path.join(ROOT, "polarism_amd", "surrogate", "weights", "fno_steady_256.npz")
 
 
 def run_sweep(members, tile_h=8):
     from polarism_amd.sweep import PackedSweep
     sw = PackedSweep(members, reservoir="single", precision="single", dt=SWEEP["dt"],
                      check_every=SWEEP["check_every"], rtol=SWEEP["rtol"], max_time=SWEEP["max_time"],
-                     tile_h=tile_h)
+                     tile_h=tile_h, progress=lambda m: print(m, file=sys.stderr, flush=True))
     return sw.run()
+
+
+def summarise(res, extra_s=0.0):
+    conv = np.isfinite(res.stop_times)
+    t_last = (res.wall_to_last_s if conv.any() else res.wall_s) + extra_s
+    return {"members_per_s": round(int(conv.sum()) / t_last, 4) if conv.any() else 0.0,
+            "converged": int(conv.sum()), "members": int(conv.size),
+            "wall_to_last_s": round(t_last, 3), "wall_total_s": round(res.wall_s + extra_s, 3),
+            "median_stop_ps": float(np.nanmedian(res.stop_times)) if conv.any() else None,
+            "steps_run": res.steps_run, "point_steps": res.point_steps,
+            "effective_GBps": round(res.point_steps * 172.0 / res.wall_s / 1e9, 1)}
 
 
 def roofline_leg(n=8192, tile_h=16, reps=20):
@@ -109,14 +109,28 @@ def cpu_baseline(point_steps, converged, steps=300):
     res = (np.zeros((1, 1, *g.shape), np.float32),)
     psi, res = O.rk4_fdm_step("single", p, g, psi, res, pump_at, 0.0, SWEEP["dt"])
     t0 = time.perf_counter()
-    for s in range(steps):
+    for _ in range(steps):
         psi, res = O.rk4_fdm_step("single", p, g, psi, res, pump_at, 0.0, SWEEP["dt"])
     dt = time.perf_counter() - t0
     rate = steps * g.nx * g.ny / dt
     cpu_time = point_steps / rate
     return {"value": round(converged / cpu_time, 6), "unit": "members/s", "cores": 1, "kind": "port",
             "sample": f"{steps} oracle rk4-fdm steps, complex64 256x256, {dt:.1f} s on one core "
-                      f"({rate / 1e6:.2f} Mpt-step/s), scaled to the sweep's {point_steps:.3e} point-steps"}
+                      f"({rate / 1e6:.2f} Mpt-step/s), scaled to the plain sweep's {point_steps:.3e} point-steps"}
+
+
+def density_agreement(cold, warm):
+    both = np.isfinite(cold.stop_times) & np.isfinite(warm.stop_times)
+    errs = []
+    for i in np.flatnonzero(both):
+        dc = np.abs(cold.final_psi[i]) ** 2
+        dw = np.abs(warm.final_psi[i]) ** 2
+        den = np.linalg.norm(dc)
+        errs.append(0.0 if den < 1e-30 and np.linalg.norm(dw) < 1e-30 else np.linalg.norm(dw - dc) / max(den, 1e-30))
+    errs = np.asarray(errs)
+    return {"compared": int(both.sum()), "pass_1e-3": int((errs <= 1e-3).sum()),
+            "median_rel_l2": float(np.median(errs)) if errs.size else None,
+            "max_rel_l2": float(np.max(errs)) if errs.size else None}
 
 
 def main():
@@ -126,6 +140,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--members", type=int, default=SWEEP["members"])
     ap.add_argument("--legs", default="sweep,roofline,cpu")
+    ap.add_argument("--weights", default=WEIGHTS)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (the sweep would shard "
@@ -133,38 +148,50 @@ def main():
         sys.exit(2)
     legs = set(args.legs.split(","))
     from polarism_amd.device import init
+    from polarism_amd.surrogate import warmstart as ws
     init()
     result = {"metric": "steady-state sweep members/s (warm vs plain) at <=1e-3 rel L2; % HBM roofline",
               "unit": "members/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-              "data": "synthetic (seeded random pumps, noise-seeded initial fields)"}
-    members = held_out_members(args.members, SWEEP["seed"])
+              "data": "synthetic (seeded random pumps, noise-seeded initial fields; surrogate trained on "
+                      "solver-generated steady states)"}
+    members = ws.make_members(args.members, SWEEP["seed"])
+    have_weights = os.path.exists(args.weights)
     if "sweep" in legs:
-        warm_members = held_out_members(min(16, args.members), SWEEP["seed"] + 1)
         for _ in range(args.warmup):
-            run_sweep(warm_members)
-        t0 = time.perf_counter()
-        runs = [run_sweep(members) for _ in range(args.steps)]
-        wall = time.perf_counter() - t0
-        res = runs[-1]
-        converged = int(np.isfinite(res.stop_times).sum())
-        value = converged * args.steps / wall
-        result.update(value=round(value, 4), ms_per_step=round(wall / args.steps * 1e3, 1))
-        eff_bytes = res.point_steps * 172.0
-        result["sweep"] = {
-            "plain": {"members_per_s": round(value, 4), "converged": converged, "members": len(members),
-                      "wall_s": round(wall / args.steps, 3), "steps_run": res.steps_run,
-                      "median_stop_ps": float(np.nanmedian(res.stop_times)) if converged else None,
-                      "point_steps": res.point_steps,
-                      "effective_GBps": round(eff_bytes / (wall / args.steps) / 1e9, 1)},
-            "warm": None}
+            run_sweep(ws.make_members(8, SWEEP["seed"] + 1))
+        plain_runs, warm_runs, infer_s = [], [], []
+        t_all = time.perf_counter()
+        for _ in range(args.steps):
+            plain_runs.append(run_sweep(members))
+            if have_weights:
+                params, cfg, norm = ws.load_weights(args.weights)
+                t0 = time.perf_counter()
+                y = ws.predict(params, ws.features(members), cfg)
+                wm = ws.warm_start(members, y, norm)
+                infer_s.append(time.perf_counter() - t0)
+                warm_runs.append(run_sweep(wm))
+        wall = time.perf_counter() - t_all
+        plain = summarise(plain_runs[-1])
+        sweep = {"plain": plain, "warm": None}
+        value = plain["members_per_s"]
+        if have_weights:
+            warm = summarise(warm_runs[-1], extra_s=infer_s[-1])
+            warm["inference_s"] = round(infer_s[-1], 3)
+            sweep["warm"] = warm
+            sweep["speedup_warm_over_plain"] = round(warm["members_per_s"] / max(plain["members_per_s"], 1e-12), 3)
+            sweep["density_agreement"] = density_agreement(plain_runs[-1], warm_runs[-1])
+            value = warm["members_per_s"]
+        result.update(value=value, ms_per_step=round(wall / args.steps * 1e3, 1))
+        result["sweep"] = sweep
     if "roofline" in legs:
         result["roofline"] = roofline_leg()
     result["config"] = {"workload": "steady-state pump sweep, 256 members of 256x256, rk4-fdm-hip complex64, "
-                                    "single reservoir, dt 0.05 ps, cold start",
+                                    "single reservoir, dt 0.05 ps; warm start (FNO) vs plain cold start",
                         "members": len(members), "grid": "256x256", "lx_um": SWEEP["L"], "dt_ps": SWEEP["dt"],
                         "steady_rtol": SWEEP["rtol"], "check_every_ps": SWEEP["check_every"],
-                        "max_time_ps": SWEEP["max_time"], "parallelism": "1 GPU, members packed in one launch"}
+                        "max_time_ps": SWEEP["max_time"], "surrogate": os.path.basename(args.weights)
+                        if have_weights else None, "parallelism": "1 GPU, members packed in one launch"}
     if "cpu" in legs and "sweep" in result:
         s = result["sweep"]["plain"]
         result["cpu_baseline"] = cpu_baseline(s["point_steps"], max(s["converged"], 1))

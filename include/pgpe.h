@@ -124,7 +124,10 @@ int pgpe_wigner_add(void* psi_dev, const void* active_dev, int active_components
 
 /* ------------------------------------------------------------------ */
 /* Fused RK4-FDM stage (replaces RK4FDMSolver.step rk4_fdm_solver.py:36-77,  */
-/* Model.rhs model.py:89-127 and rk4_cuda_solver.py:43-419)             */
+/* Model.rhs / _spinor_rhs / _te_tm_energy model.py:89-208 and the      */
+/* rk4_cuda_solver.py kernels :43-419, :1556-2194).  Spinor fields are  */
+/* (members, 2, ny, nx); spinor reservoirs (members, res_components,    */
+/* ny, nx).                                                             */
 /* ------------------------------------------------------------------ */
 typedef struct pgpe_rk4_args {
     int32_t precision;      /* 0 single, 1 double */
@@ -134,6 +137,8 @@ typedef struct pgpe_rk4_args {
     int32_t tile_y;         /* rows per tile, equal to the table's tile_h: 8, 16 or 32 */
     int32_t noise;          /* stage 3 only: add the Wigner increment of this step */
     int32_t members;
+    int32_t components;     /* 1 scalar; 2 spinor (psi+, psi-) with TE-TM, Zeeman and alpha_2 */
+    int32_t res_components; /* spinor reservoir: 1 shared (rho = |psi+|^2 + |psi-|^2), 2 per spin */
     int32_t pad0;
     double dt;
     const pgpe_member* table;        /* device */
@@ -283,12 +288,14 @@ int pgpe_f32_to_bf16(const float* x_dev, void* y_dev, int64_t n, void* stream);
 int pgpe_gelu_fwd(const float* x_dev, float* y_dev, void* yb_dev, int64_t n, void* stream);
 /* dx = dy GELU'(x) into fp32 dx and / or bf16 dxb. */
 int pgpe_gelu_bwd(const float* x_dev, const float* dy_dev, float* dx_dev, void* dxb_dev, int64_t n, void* stream);
-/* Complex (interleaved) y[i] = d[i mod per_batch] x[i] (conj(d) when conj), i < n; fp32 y and / or bf16 yb. */
-int pgpe_cdiag(const float* x_dev, const float* d_dev, float* y_dev, void* yb_dev, int64_t per_batch, int64_t n,
-               int conj, void* stream);
-/* dd[i] = sum_b conj(x[b, i]) g[b, i], complex, i < per_batch. */
-int pgpe_cdiag_grad(const float* x_dev, const float* g_dev, float* dd_dev, int64_t per_batch, int batch,
-                    void* stream);
+/* Planar spectra laid out [ky][c][b][kx] (my x C x B x mx): u = d[c, ky, kx] v, with
+ * conj(d) when conj; fp32 (ur, ui) and / or bf16 (urb, uib) outputs. */
+int pgpe_mode_scale(const float* vr_dev, const float* vi_dev, const float* dr_dev, const float* di_dev,
+                    float* ur_dev, float* ui_dev, void* urb_dev, void* uib_dev, int my, int C, int B, int mx,
+                    int conj, void* stream);
+/* gd[c, ky, kx] = sum_b conj(v) du over the same layout (gradient of the per-mode scale). */
+int pgpe_mode_scale_grad(const float* vr_dev, const float* vi_dev, const float* dur_dev, const float* dui_dev,
+                         float* gdr_dev, float* gdi_dev, int my, int C, int B, int mx, void* stream);
 /* out[c] = sum_r x[r row_stride + c col_stride], fp32. */
 int pgpe_colsum(const float* x_dev, float* out_dev, int64_t rows, int64_t cols, int64_t row_stride,
                 int64_t col_stride, void* stream);

@@ -23,19 +23,38 @@ It restates, in plain NumPy, the reference's CPU algorithms for the hot path
 * spinor dispersion and 2x2 exponential -- dispersion.py:34-89
 * steady-state test -- simulation_controller.py:710-734
 
-Pinning (what checks each function):
-* philox4x32: Random123 known-answer vectors (tests/unit/test_noise.py:56-64).
-* normal_pair: bit-exact uniforms from a closed form; normals vs math.log/cos.
-* lawson_step (rk4 tableau, single and quadratic-double reservoirs, eta 0 and
-  0.04, complex potential, pulsed pump, f32/f64): the reference's golden
-  trajectories tests/data/ifrk4_reference/reference.npz.
-* lawson_step adaptive + StepController: closed-form uniform decay
-  (tests/unit/test_adaptive_dt.py:34-38) and tableau order conditions.
-* laplacian / rk4_fdm_step: discrete cosine eigenfunction, 4th-order
-  convergence in dt, uniform-decay closed form, reservoir 0-D steady state.
-* split_step_*: uniform closed form, plane-wave phase, agreement of the two
-  orderings to O(dt^2), and agreement with rk4_fdm_step as dx -> 0.
-* spinor exponential: brute-force scipy.linalg.expm on random 2x2 Hermitians.
+Pinning (what checks each function; tests/ file in brackets):
+* philox4x32: Random123 known-answer vectors held by the reference's
+  tests/unit/test_noise.py:56-64 [test_oracle_noise.py].
+* uniforms / normal_pair: bit-exact uniforms from the closed form; normals vs
+  math.log/cos/sin; moments; one-step Wigner variance [test_oracle_noise.py].
+* LawsonProblem.step (rk4 tableau; single and quadratic-double reservoirs;
+  eta 0 and 0.04; complex potential; pulsed pump; f32/f64): the reference's
+  golden trajectories tests/data/ifrk4_reference/reference.npz
+  [test_oracle_golden.py]; 4th order on the uniform closed form [test_oracle_pins.py].
+* LawsonProblem adaptive + error + StepController + lawson_advance_adaptive:
+  closed-form uniform decay within 50 rtol (tests/unit/test_adaptive_dt.py:34-38),
+  rejection / tick landing, factor bounds, underflow guard [test_oracle_pins.py].
+* oracle_tableau: order conditions, c = A 1, and an independent rational
+  transcription [test_tableaus.py].
+* laplacian: discrete cosine eigenfunction, second order [test_oracle_pins.py].
+* te_tm_energy: plane-wave stencil symbols and the continuum limit equal to
+  the dispersion matrix off-diagonal [test_oracle_pins.py].
+* fdm_rhs: uniform closed-form steady states of all three reservoirs (a fixed
+  point with mu/hbar phase rotation), Zeeman sign vs a scalar potential
+  [test_oracle_pins.py].
+* rk4_fdm_step: 4th-order convergence on the uniform closed form [test_oracle_pins.py].
+* reservoir_derivatives / reservoir_rk2 / thresholds: the threshold makes the
+  gain vanish at the psi = 0 fixed point; exact RK2 decay factor; clamp.
+* split_step_*: exact for uniform fields, plane-wave phase, orderings
+  differing at O(dt^2) [test_oracle_pins.py].
+* spinor_propagator: scipy.linalg.expm per k, unitarity [test_oracle_pins.py].
+* steady_metrics, initial_psi, pulse_gaussian_coeff: invariants and closed
+  forms [test_oracle_pins.py].
+Parity unpinned (no reference execution possible; restatement-pinned only):
+the full multi-step trajectories of rk4_fdm_step and split_step_* at the
+quickstart / split-step configurations, and the exact DP5 / DOP853 trial
+sequences of the reference.
 """
 from __future__ import annotations
 
@@ -436,7 +455,7 @@ def oracle_tableau(name: str) -> OracleTableau:
     if name == "dp5":
         s = _rk.RK45
         A = np.zeros((7, 7))
-        A[:6, :6] = s.A
+        A[:6, :5] = s.A
         A[6, :6] = s.B
         b = np.append(s.B, 0.0)
         c = np.append(s.C, 1.0)

@@ -48,7 +48,7 @@ class Rk4Args(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("reservoir", C.c_int32), ("stage", C.c_int32),
         ("num_tiles", C.c_int32), ("tile_y", C.c_int32), ("noise", C.c_int32),
-        ("members", C.c_int32), ("pad0", C.c_int32),
+        ("members", C.c_int32), ("components", C.c_int32), ("res_components", C.c_int32), ("pad0", C.c_int32),
         ("dt", C.c_double),
         ("table", vp), ("tile_map", vp),
         ("psi0", vp), ("kprev", vp), ("k1", vp), ("k2", vp),
@@ -143,8 +143,8 @@ _SIGNATURES = {
     "pgpe_f32_to_bf16": (C.c_int, [vp, vp, C.c_int64, vp]),
     "pgpe_gelu_fwd": (C.c_int, [vp, vp, vp, C.c_int64, vp]),
     "pgpe_gelu_bwd": (C.c_int, [vp, vp, vp, vp, C.c_int64, vp]),
-    "pgpe_cdiag": (C.c_int, [vp, vp, vp, vp, C.c_int64, C.c_int64, C.c_int, vp]),
-    "pgpe_cdiag_grad": (C.c_int, [vp, vp, vp, C.c_int64, C.c_int, vp]),
+    "pgpe_mode_scale": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
+    "pgpe_mode_scale_grad": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
     "pgpe_colsum": (C.c_int, [vp, vp, C.c_int64, C.c_int64, C.c_int64, C.c_int64, vp]),
     "pgpe_density_delta": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp]),
 }

@@ -150,35 +150,45 @@ __global__ void gelu_bwd_kernel(const float* __restrict__ x, const float* __rest
     if (dxb) dxb[i] = __float2bfloat16(g);
 }
 
-// Complex diagonal mode scaling over interleaved (re, im) planes:
-// y[b, c, q] = d[c, q] * x[b, c, q] for complex q (conj = 1 uses conj(d)).
-__global__ void cdiag_kernel(const float2* __restrict__ x, const float2* __restrict__ d, float2* __restrict__ y,
-                             __hip_bfloat16* __restrict__ yb, int64_t per_batch, int64_t n, int conj) {
+// Per-mode complex channel scale on planar (re, im) spectra laid out
+// [ky][c][b][kx]: u = d[c, ky, kx] v (conj(d) when conj), fp32 and bf16 copies.
+__global__ void mode_scale_kernel(const float* __restrict__ vr, const float* __restrict__ vi,
+                                  const float* __restrict__ dr, const float* __restrict__ di, float* __restrict__ ur,
+                                  float* __restrict__ ui, __hip_bfloat16* __restrict__ urb,
+                                  __hip_bfloat16* __restrict__ uib, int my, int C, int B, int mx, int conj) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)my * C * B * mx;
     if (i >= n) return;
-    const float2 a = x[i];
-    float2 w = d[i % per_batch];
-    if (conj) w.y = -w.y;
-    const float2 r = make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
-    if (y) y[i] = r;
-    if (yb) {
-        yb[2 * i] = __float2bfloat16(r.x);
-        yb[2 * i + 1] = __float2bfloat16(r.y);
-    }
+    const int kx = (int)(i % mx);
+    const int c = (int)((i / ((int64_t)B * mx)) % C);
+    const int ky = (int)(i / ((int64_t)C * B * mx));
+    const int64_t q = ((int64_t)c * my + ky) * mx + kx;
+    const float wr = dr[q], wi = conj ? -di[q] : di[q];
+    const float a = vr[i], b = vi[i];
+    const float re = wr * a - wi * b, im = wr * b + wi * a;
+    if (ur) ur[i] = re;
+    if (ui) ui[i] = im;
+    if (urb) urb[i] = __float2bfloat16(re);
+    if (uib) uib[i] = __float2bfloat16(im);
 }
 
-// dd[c, q] = sum_b conj(x[b, c, q]) * g[b, c, q]  (gradient of the diagonal weights)
-__global__ void cdiag_grad_kernel(const float2* __restrict__ x, const float2* __restrict__ g, float2* __restrict__ dd,
-                                  int64_t per_batch, int batch) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= per_batch) return;
+// gd[c, ky, kx] = sum_b conj(v) du  (complex), v and du planar [ky][c][b][kx].
+__global__ void mode_scale_grad_kernel(const float* __restrict__ vr, const float* __restrict__ vi,
+                                       const float* __restrict__ dur, const float* __restrict__ dui,
+                                       float* __restrict__ gdr, float* __restrict__ gdi, int my, int C, int B, int mx) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)C * my * mx) return;
+    const int kx = (int)(q % mx);
+    const int ky = (int)((q / mx) % my);
+    const int c = (int)(q / ((int64_t)my * mx));
     float sr = 0.0f, si = 0.0f;
-    for (int b = 0; b < batch; ++b) {
-        const float2 a = x[b * per_batch + i], q = g[b * per_batch + i];
-        sr += a.x * q.x + a.y * q.y;
-        si += a.x * q.y - a.y * q.x;
+    for (int b = 0; b < B; ++b) {
+        const int64_t i = (((int64_t)ky * C + c) * B + b) * mx + kx;
+        sr += vr[i] * dur[i] + vi[i] * dui[i];
+        si += vr[i] * dui[i] - vi[i] * dur[i];
     }
-    dd[i] = make_float2(sr, si);
+    gdr[q] = sr;
+    gdi[q] = si;
 }
 
 // out[n] = sum over rows of x[row, n] (bias gradients), rows x cols fp32 row-major.
@@ -254,22 +264,24 @@ int pgpe_gelu_bwd(const float* x, const float* dy, float* dx, void* dxb, int64_t
     return PGPE_OK;
 }
 
-int pgpe_cdiag(const float* x, const float* d, float* y, void* yb, int64_t per_batch, int64_t n, int conj,
-               void* stream) {
-    if (n <= 0) return PGPE_OK;
-    if (!x || !d || (!y && !yb) || per_batch <= 0) return fail(PGPE_ERR_ARG, "pgpe_cdiag: bad argument");
-    hipLaunchKernelGGL(cdiag_kernel, blocks1d(n), dim3(256), 0, (hipStream_t)stream, (const float2*)x,
-                       (const float2*)d, (float2*)y, (__hip_bfloat16*)yb, per_batch, n, conj);
-    PGPE_LAUNCHED("cdiag_kernel");
+int pgpe_mode_scale(const float* vr, const float* vi, const float* dr, const float* di, float* ur, float* ui,
+                    void* urb, void* uib, int my, int C, int B, int mx, int conj, void* stream) {
+    if (!vr || !vi || !dr || !di || (!ur && !urb) || my <= 0 || C <= 0 || B <= 0 || mx <= 0)
+        return fail(PGPE_ERR_ARG, "pgpe_mode_scale: bad argument");
+    const int64_t n = (int64_t)my * C * B * mx;
+    hipLaunchKernelGGL(mode_scale_kernel, blocks1d(n), dim3(256), 0, (hipStream_t)stream, vr, vi, dr, di, ur, ui,
+                       (__hip_bfloat16*)urb, (__hip_bfloat16*)uib, my, C, B, mx, conj);
+    PGPE_LAUNCHED("mode_scale_kernel");
     return PGPE_OK;
 }
 
-int pgpe_cdiag_grad(const float* x, const float* g, float* dd, int64_t per_batch, int batch, void* stream) {
-    if (per_batch <= 0 || batch <= 0) return fail(PGPE_ERR_ARG, "pgpe_cdiag_grad: empty");
-    if (!x || !g || !dd) return fail(PGPE_ERR_ARG, "pgpe_cdiag_grad: null pointer");
-    hipLaunchKernelGGL(cdiag_grad_kernel, blocks1d(per_batch), dim3(256), 0, (hipStream_t)stream, (const float2*)x,
-                       (const float2*)g, (float2*)dd, per_batch, batch);
-    PGPE_LAUNCHED("cdiag_grad_kernel");
+int pgpe_mode_scale_grad(const float* vr, const float* vi, const float* dur, const float* dui, float* gdr,
+                         float* gdi, int my, int C, int B, int mx, void* stream) {
+    if (!vr || !vi || !dur || !dui || !gdr || !gdi || my <= 0 || C <= 0 || B <= 0 || mx <= 0)
+        return fail(PGPE_ERR_ARG, "pgpe_mode_scale_grad: bad argument");
+    hipLaunchKernelGGL(mode_scale_grad_kernel, blocks1d((int64_t)C * my * mx), dim3(256), 0, (hipStream_t)stream,
+                       vr, vi, dur, dui, gdr, gdi, my, C, B, mx);
+    PGPE_LAUNCHED("mode_scale_grad_kernel");
     return PGPE_OK;
 }
 

@@ -200,9 +200,183 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
     }
 }
 
+// ------------------------------------------------------------------ spinor
+// Circular-basis spinor stage (model.py:129-208): per component c, with o the
+// other component,
+//   X_c = -hbar^2/2m lap psi_c + (gC rho_c + a2 rho_o +- Z/2 + gR nA + gI nI) psi_c
+//         - beta [(dxx - dyy) o -/+ 2i dxy o]
+//   k_c = (-i/hbar) X_c + (R nA - gammaC)/2 psi_c + eta nA hbar/2m lap psi_c
+// dxy is the 4-corner central difference, so the LDS tile keeps its corners.
+template <typename T, int RES, int TH, int KIND>
+__global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T> p, int res_comps) {
+    constexpr int NF = ResFields<RES>::n;
+    constexpr int W = kTileW + 2;
+    constexpr int PL = (TH + 2) * W;
+    __shared__ c2<T> s_psi[2 * PL];
+
+    const int tile = xcd_tile(blockIdx.x, p.num_tiles);
+    if (tile < 0) return;
+    const int m = p.tile_map[2 * tile];
+    const pgpe_member& mb = p.table[m];
+    const int nx = mb.nx, ny = mb.ny;
+    const int local = p.tile_map[2 * tile + 1];
+    const int x0 = (local % mb.tiles_x) * kTileW;
+    const int y0 = (local / mb.tiles_x) * TH;
+    const int64_t plane = (int64_t)nx * ny;
+    const int64_t poff = mb.off * 2;              // first psi plane of this member
+    const int64_t roff = mb.off * res_comps;      // first reservoir plane
+
+    for (int i = threadIdx.x; i < 2 * PL; i += 256) {
+        const int c = i / PL;
+        const int j = i - c * PL;
+        const int ly = j / W;
+        const int lx = j - ly * W;
+        int gy = y0 + ly - 1;
+        int gx = x0 + lx - 1;
+        gy = gy < 0 ? gy + ny : gy;
+        gx = gx < 0 ? gx + nx : gx;
+        if (gy >= ny) gy %= ny;
+        if (gx >= nx) gx %= nx;
+        const int64_t g = poff + c * plane + (int64_t)gy * nx + gx;
+        c2<T> v = p.psi0[g];
+        if constexpr (KIND != 0) {
+            const c2<T> k = p.kprev[g];
+            v.x = v.x + p.a * k.x;
+            v.y = v.y + p.a * k.y;
+        }
+        s_psi[i] = v;
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int gx = x0 + lane;
+    if (gx >= nx) return;
+    const T inv_dx2 = (T)mb.inv_dx2, inv_dy2 = (T)mb.inv_dy2, inv4 = (T)mb.inv_4dxdy;
+    const T kin = (T)mb.kin, inv_hbar = (T)mb.inv_hbar;
+    const T gC = (T)mb.g_C, gR = (T)mb.g_R, gI = (T)mb.g_I, R = (T)mb.R, gammaC = (T)mb.gamma_C;
+    const T a2 = (T)mb.alpha_2, zh = (T)mb.zeeman, beta = (T)mb.te_tm;
+    const T eta_c = (T)mb.eta, hb2m = (T)mb.hbar_over_2m;
+    const T coef = p.coef[m];
+    const T* __restrict__ env = p.pump + mb.pump_off;
+    uint64_t step = p.step;
+    if (KIND == 2 && p.noise && p.step_base) step += *p.step_base;
+
+    for (int r = threadIdx.x >> 6; r < TH; r += 4) {
+        const int gy = y0 + r;
+        if (gy >= ny) break;
+        const int64_t cell = (int64_t)gy * nx + gx;
+        const int ci = (r + 1) * W + lane + 1;
+        c2<T> ps[2], lap[2], te[2];
+        T rho[2];
+        for (int c = 0; c < 2; ++c) {
+            const c2<T>* t = s_psi + c * PL;
+            const c2<T> z = t[ci], n = t[ci + W], so = t[ci - W], e = t[ci + 1], w = t[ci - 1];
+            ps[c] = z;
+            rho[c] = z.x * z.x + z.y * z.y;
+            lap[c].x = ((n.x - (T)2 * z.x) + so.x) * inv_dy2 + ((e.x - (T)2 * z.x) + w.x) * inv_dx2;
+            lap[c].y = ((n.y - (T)2 * z.y) + so.y) * inv_dy2 + ((e.y - (T)2 * z.y) + w.y) * inv_dx2;
+        }
+        for (int c = 0; c < 2; ++c) {
+            te[c] = make_c<T>(0, 0);
+            if (beta == (T)0) continue;
+            const c2<T>* t = s_psi + (1 - c) * PL;   // the other component
+            const c2<T> z = t[ci], n = t[ci + W], so = t[ci - W], e = t[ci + 1], w = t[ci - 1];
+            const c2<T> ne = t[ci + W + 1], nw = t[ci + W - 1], se = t[ci - W + 1], sw = t[ci - W - 1];
+            c2<T> dxx, dyy, dxy;
+            dxx.x = ((e.x - (T)2 * z.x) + w.x) * inv_dx2;
+            dxx.y = ((e.y - (T)2 * z.y) + w.y) * inv_dx2;
+            dyy.x = ((n.x - (T)2 * z.x) + so.x) * inv_dy2;
+            dyy.y = ((n.y - (T)2 * z.y) + so.y) * inv_dy2;
+            dxy.x = (((ne.x - nw.x) - se.x) + sw.x) * inv4;
+            dxy.y = (((ne.y - nw.y) - se.y) + sw.y) * inv4;
+            // cross = -2i dxy (c = 0) or +2i dxy (c = 1)
+            const c2<T> cross = c == 0 ? make_c<T>((T)2 * dxy.y, -(T)2 * dxy.x) : make_c<T>(-(T)2 * dxy.y, (T)2 * dxy.x);
+            te[c].x = -beta * ((dxx.x - dyy.x) + cross.x);
+            te[c].y = -beta * ((dxx.y - dyy.y) + cross.y);
+        }
+        // reservoir stage values and derivatives
+        T n[2][2] = {{0, 0}, {0, 0}}, dn[2][2] = {{0, 0}, {0, 0}};
+        const T P = env[cell] * coef;
+        for (int rc = 0; rc < res_comps; ++rc) {
+            const int64_t g = roff + rc * plane + cell;
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                T v = p.res0[f][g];
+                if constexpr (KIND != 0) v = v + p.a * p.kres_prev[f][g];
+                n[rc][f] = v;
+            }
+            const T rr = res_comps == 2 ? rho[rc] : rho[0] + rho[1];
+            const T Pc = res_comps == 2 ? P * (T)(rc == 0 ? mb.pump_w_plus : mb.pump_w_minus) : P;
+            reservoir_rhs<T, RES>(mb, rr, Pc, n[rc], dn[rc]);
+        }
+        c2<T> d[2];
+        for (int c = 0; c < 2; ++c) {
+            const int rc = res_comps == 2 ? c : 0;
+            const T nA = NF > 0 ? n[rc][0] : (T)0;
+            const T nI = NF > 1 ? n[rc][1] : (T)0;
+            const T sgn = c == 0 ? (T)1 : (T)-1;
+            const T eff = ((((gC * rho[c]) + a2 * rho[1 - c]) + zh * sgn / (T)2) + gR * nA) + gI * nI;
+            const T gain = (R * nA - gammaC) * (T)0.5;
+            const T Xr = (kin * lap[c].x + eff * ps[c].x) + te[c].x;
+            const T Xi = (kin * lap[c].y + eff * ps[c].y) + te[c].y;
+            d[c].x = inv_hbar * Xi + gain * ps[c].x;
+            d[c].y = -(inv_hbar * Xr) + gain * ps[c].y;
+            if (eta_c != (T)0) {
+                const T s = (eta_c * nA) * hb2m;
+                d[c].x = d[c].x + s * lap[c].x;
+                d[c].y = d[c].y + s * lap[c].y;
+            }
+        }
+        if constexpr (KIND != 2) {
+            for (int c = 0; c < 2; ++c) p.kout[poff + c * plane + cell] = d[c];
+            for (int rc = 0; rc < res_comps; ++rc)
+#pragma unroll
+                for (int f = 0; f < NF; ++f) p.kres_out[f][roff + rc * plane + cell] = dn[rc][f];
+        } else {
+            T n_new[2][2] = {{0, 0}, {0, 0}};
+            for (int rc = 0; rc < res_comps; ++rc) {
+                const int64_t g = roff + rc * plane + cell;
+#pragma unroll
+                for (int f = 0; f < NF; ++f) {
+                    const T r0 = p.res0[f][g];
+                    T v = r0 + p.dt6 * (((p.kres1[f][g] + (T)2 * p.kres2[f][g]) + (T)2 * p.kres_prev[f][g]) + dn[rc][f]);
+                    v = v > (T)0 ? v : (T)0;
+                    n_new[rc][f] = v;
+                    p.res_out[f][g] = v;
+                }
+            }
+            for (int c = 0; c < 2; ++c) {
+                const int64_t g = poff + c * plane + cell;
+                const c2<T> a1 = p.k1[g], a2v = p.k2[g], a3 = p.kprev[g], z = p.psi0[g];
+                c2<T> out;
+                out.x = z.x + p.dt6 * (((a1.x + (T)2 * a2v.x) + (T)2 * a3.x) + d[c].x);
+                out.y = z.y + p.dt6 * (((a1.y + (T)2 * a2v.y) + (T)2 * a3.y) + d[c].y);
+                if (p.noise) {
+                    const T act = NF > 0 ? n_new[res_comps == 2 ? c : 0][0] : (T)0;
+                    wigner_site<T>(out.x, out.y, mb.seed, (uint32_t)cell, step, 1u + (uint32_t)c,
+                                   (T)mb.noise_strength, (T)mb.noise_gamma, (T)mb.noise_R, act,
+                                   (T)mb.noise_dt_over_4v);
+                }
+                p.psi_out[g] = out;
+            }
+        }
+    }
+}
+
 template <typename T, int RES, int TH>
 static int launch_kind(const pgpe_rk4_args* a, const Rk4Params<T>& p, hipStream_t s) {
     const dim3 grid(xcd_grid(a->num_tiles)), block(256);
+    if (a->components == 2) {
+        const int rc = a->res_components;
+        if (a->stage == 0)
+            hipLaunchKernelGGL((rk4_spinor_stage_kernel<T, RES, TH, 0>), grid, block, 0, s, p, rc);
+        else if (a->stage < 3)
+            hipLaunchKernelGGL((rk4_spinor_stage_kernel<T, RES, TH, 1>), grid, block, 0, s, p, rc);
+        else
+            hipLaunchKernelGGL((rk4_spinor_stage_kernel<T, RES, TH, 2>), grid, block, 0, s, p, rc);
+        PGPE_LAUNCHED("rk4_spinor_stage_kernel");
+        return PGPE_OK;
+    }
     if (a->stage == 0)
         hipLaunchKernelGGL((rk4_stage_kernel<T, RES, TH, 0>), grid, block, 0, s, p);
     else if (a->stage < 3)
@@ -265,6 +439,9 @@ static int check_args(const pgpe_rk4_args* a) {
     if (!a) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null args");
     if (a->stage < 0 || a->stage > 3) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: stage must be 0..3");
     if (a->num_tiles <= 0 || a->members <= 0) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: empty member table");
+    if (a->components != 1 && a->components != 2) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: components must be 1 or 2");
+    if (a->components == 2 && a->res_components != 1 && a->res_components != 2)
+        return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: res_components must be 1 or 2");
     if (!a->table || !a->tile_map || !a->psi0 || !a->pump || !a->coef)
         return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null table, field, pump or coefficient");
     const int nf = a->reservoir == PGPE_RES_NONE ? 0 : (a->reservoir == PGPE_RES_SINGLE ? 1 : 2);

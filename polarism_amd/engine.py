@@ -35,12 +35,15 @@ class Rk4Engine:
         self.reservoir = reservoir
         self.nres = N.RESERVOIR_FIELDS[reservoir]
         self.stream = stream or Stream()
-        n = table.total_points
+        self.components = table.specs[0].components
+        self.res_components = table.specs[0].res_components if self.components == 2 else 1
+        n = table.total_points * self.components
+        nr = table.total_points * self.res_components
         c, r = self.cdtype, self.rdtype
         self.psi = [DeviceArray.empty((n,), c), DeviceArray.empty((n,), c)]
         self.k = [DeviceArray.empty((n,), c) for _ in range(3)]
-        self.kres = [[DeviceArray.empty((n,), r) for _ in range(self.nres)] for _ in range(3)]
-        self.res = [DeviceArray.zeros((n,), r) for _ in range(self.nres)]
+        self.kres = [[DeviceArray.empty((nr,), r) for _ in range(self.nres)] for _ in range(3)]
+        self.res = [DeviceArray.zeros((nr,), r) for _ in range(self.nres)]
         self.step_base = DeviceArray.zeros((1,), np.uint64)
         # active tile list: starts as every tile; the sweep driver may retire members
         self.tile_map = table.tile_map_dev
@@ -69,6 +72,8 @@ class Rk4Engine:
         a.num_tiles = self.num_tiles
         a.tile_y = tab.tile_h
         a.members = tab.members
+        a.components = self.components
+        a.res_components = self.res_components
         a.dt = float(self._dt)
         a.table = tab.table_ptr
         a.tile_map = self.tile_map.ptr

@@ -23,20 +23,18 @@ _ALL_RES = frozenset({"single", "double", "quadratic-double", "none"})
 @register_solver("rk4-fdm-hip")
 class RK4FDMHipSolver(HipSolverBase):
     capabilities = SolverCapabilities(
-        components=frozenset({1}), ndim=frozenset({2}), grid_types=frozenset({"periodic"}),
+        components=frozenset({1, 2}), ndim=frozenset({2}), grid_types=frozenset({"periodic"}),
         reservoirs=_ALL_RES, custom_reservoirs="unsupported", potential=False,
         kinetic_relaxation=True, dispersion=frozenset({"parabolic"}), terms=False,
         noise=frozenset({"none", "wigner"}), quantitative_reservoirs=_ALL_RES,
         description="Stage-coupled FDM RK4 on gfx950: one fused HIP launch per RK stage over all "
-                    "members, LDS halo tiles, in-kernel Wigner noise, HIP-graph chunks. Periodic 2-D, "
-                    "zero potential.")
+                    "members, LDS halo tiles, in-kernel Wigner noise, HIP-graph chunks; scalar or spinor "
+                    "(TE-TM 9-point, Zeeman, alpha_2). Periodic 2-D, zero potential.")
     injects_noise = True   # advance() adds the Wigner increment inside the final stage kernel
     max_graph_chunk = 1000
 
     def __init__(self, config, grid, model):
         super().__init__(config, grid, model)
-        if model.components != 1:
-            raise ValueError("rk4-fdm-hip is scalar; spinor fields are not supported by this method")
         self.engine = None
         self._const_coef = None
         self._coef_chunk = None

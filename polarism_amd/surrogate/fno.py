@@ -29,7 +29,7 @@ import math
 import numpy as np
 import torch
 
-from .kernels import colsum, gelu_bwd, gelu_fwd, gemm, to_bf16
+from .kernels import colsum, gelu_bwd, gelu_fwd, gemm, mode_scale, mode_scale_grad, to_bf16
 
 BF = torch.bfloat16
 
@@ -135,11 +135,9 @@ class FNO:
         gemm(bs.cy, v1ib, vi, my, mx, H, batch=CB, sa=sa_y, sb=sb_y, sc=sc_y)
         gemm(bs.sy, v1rb, vi, my, mx, H, batch=CB, sa=sa_y, sb=sb_y, sc=sc_y, alpha=-1.0, beta=1.0)
         # per-mode channel scale: U = d * Vhat  (d laid out (C, my, mx) -> broadcast over b)
-        dr = p[f"d{l}_re"].permute(1, 0, 2).unsqueeze(2)       # (my, C, 1, mx)
-        di = p[f"d{l}_im"].permute(1, 0, 2).unsqueeze(2)
-        ur = dr * vr - di * vi
-        ui = dr * vi + di * vr
-        urb, uib = to_bf16(ur.contiguous()), to_bf16(ui.contiguous())
+        urb = torch.empty(my, C, B, mx, device=dev, dtype=BF)
+        uib = torch.empty(my, C, B, mx, device=dev, dtype=BF)
+        mode_scale(vr, vi, p[f"d{l}_re"].contiguous(), p[f"d{l}_im"].contiguous(), urb=urb, uib=uib)
         # channel mixing per ky: Wk = M[ky] U[ky]   (C x (B mx))
         mr, mi = to_bf16(p[f"m{l}_re"].contiguous()), to_bf16(p[f"m{l}_im"].contiguous())
         Nn = B * mx
@@ -220,15 +218,14 @@ class FNO:
         gemm(mi, dwrb, dui, C, Nn, C, batch=my, sa=sa_h, sb=sb_w, sc=sc_u, alpha=-1.0, beta=1.0)
         # U = d Vhat: dd = sum_b conj(Vhat) dU ; dVhat = conj(d) dU
         vr, vi = cache[f"vr{l}"], cache[f"vi{l}"]
-        gdr = (vr * dur + vi * dui).sum(dim=2)          # (my, C, mx)
-        gdi = (vr * dui - vi * dur).sum(dim=2)
-        grads[f"d{l}_re"] = gdr.permute(1, 0, 2).contiguous()
-        grads[f"d{l}_im"] = gdi.permute(1, 0, 2).contiguous()
-        dr = p[f"d{l}_re"].permute(1, 0, 2).unsqueeze(2)
-        di = p[f"d{l}_im"].permute(1, 0, 2).unsqueeze(2)
-        dvr = dr * dur + di * dui
-        dvi = dr * dui - di * dur
-        dvrb, dvib = to_bf16(dvr.contiguous()), to_bf16(dvi.contiguous())
+        gdr = torch.empty(C, my, mx, device=dev)
+        gdi = torch.empty(C, my, mx, device=dev)
+        mode_scale_grad(vr, vi, dur, dui, gdr, gdi)
+        grads[f"d{l}_re"] = gdr
+        grads[f"d{l}_im"] = gdi
+        dvrb = torch.empty(my, C, B, mx, device=dev, dtype=BF)
+        dvib = torch.empty(my, C, B, mx, device=dev, dtype=BF)
+        mode_scale(dur, dui, p[f"d{l}_re"].contiguous(), p[f"d{l}_im"].contiguous(), urb=dvrb, uib=dvib, conj=True)
         # Vhat = (Cy - i Sy) V1 -> dV1r = Cy^T dVr - Sy^T dVi ; dV1i = Cy^T dVi + Sy^T dVr   ((c, b) x H x mx)
         dv1r = torch.empty(C, B, H, mx, device=dev)
         dv1i = torch.empty(C, B, H, mx, device=dev)

@@ -61,3 +61,16 @@ def gelu_bwd(x, dy, dx=None, dxb=None):
 
 def colsum(x, out, rows, cols, row_stride, col_stride):
     N.call("pgpe_colsum", x.data_ptr(), out.data_ptr(), rows, cols, row_stride, col_stride, _stream())
+
+
+def mode_scale(vr, vi, dr, di, ur=None, ui=None, urb=None, uib=None, conj=False):
+    """u = d v per mode and channel; v, u planar (my, C, B, mx); d (C, my, mx)."""
+    my, C_, B, mx = vr.shape
+    N.call("pgpe_mode_scale", vr.data_ptr(), vi.data_ptr(), dr.data_ptr(), di.data_ptr(), _ptr(ur), _ptr(ui),
+           _ptr(urb), _ptr(uib), my, C_, B, mx, 1 if conj else 0, _stream())
+
+
+def mode_scale_grad(vr, vi, dur, dui, gdr, gdi):
+    my, C_, B, mx = vr.shape
+    N.call("pgpe_mode_scale_grad", vr.data_ptr(), vi.data_ptr(), dur.data_ptr(), dui.data_ptr(), gdr.data_ptr(),
+           gdi.data_ptr(), my, C_, B, mx, _stream())

@@ -1,0 +1,201 @@
+"""Training data, training loop and warm starts of the FNO surrogate.
+
+* ``make_members``  -- seeded sweep members (256 x 256, lx = ly = 128 um,
+  single reservoir): random pumps from ``polarism_amd.pumps`` with peak
+  P0 = U[0.8, 4] x the uniform-pump threshold.
+* ``generate``      -- runs the members from noise-seeded cold starts with
+  ``PackedSweep`` (rk4-fdm-hip, complex64) to a steady state and returns the
+  surrogate's inputs and targets.  Targets are gauge fixed: the global phase
+  is chosen so that sum psi |psi|^2 is real and positive (the steady state
+  of the U(1)-symmetric GPE is defined only up to a global phase).
+* ``train``         -- AdamW on the mean squared error of normalised targets;
+  the network's forward and backward run on the HIP kernels.
+* ``warm_start``    -- predicted (psi, n_R) plus the same noise field a cold
+  start would get, as initial state of the same steady-state run.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from ..config import PhysicsConstants
+from ..noise import make_initial_psi
+from ..pumps import coords, random_pump, threshold
+from ..sweep import PackedSweep, SweepMember
+
+GRID = dict(n=256, L=128.0)
+IN_FEATURES = 6
+
+
+def make_members(count, seed, n=GRID["n"], L=GRID["L"], ratio=(0.8, 4.0)):
+    p = PhysicsConstants()
+    rng = np.random.default_rng(seed)
+    pth = threshold(p, "single")
+    out = []
+    for i in range(count):
+        env, desc = random_pump(rng, n, n, L, L)
+        r = float(rng.uniform(*ratio))
+        out.append(SweepMember(n, n, L, L, p, env, r * pth, seed=seed * 100003 + i,
+                               label={"ratio": r, "kind": desc["kind"]}))
+    return out
+
+
+def features(members) -> np.ndarray:
+    """(6, B, H, W) float32: local pump / P_th, P0 / P_th, gamma_C / 0.083, g_C / 0.001, x / (lx/2), y / (ly/2)."""
+    m0 = members[0]
+    H, W = m0.ny, m0.nx
+    X, Y = coords(m0.nx, m0.ny, m0.lx, m0.ly)
+    out = np.empty((IN_FEATURES, len(members), H, W), dtype=np.float32)
+    for b, m in enumerate(members):
+        pth = threshold(m.physics, "single")
+        ratio = m.P0 / pth
+        out[0, b] = m.envelope * ratio
+        out[1, b] = ratio
+        out[2, b] = m.physics.gamma_C / 0.083
+        out[3, b] = m.physics.g_C / 0.001
+        out[4, b] = X / (m.lx / 2)
+        out[5, b] = Y / (m.ly / 2)
+    return out
+
+
+def gauge_fix(psi: np.ndarray) -> np.ndarray:
+    w = np.sum(psi * np.abs(psi) ** 2)
+    if abs(w) == 0:
+        return psi
+    return psi * np.exp(-1j * np.angle(w))
+
+
+@dataclass
+class Normaliser:
+    psi_scale: float
+    n_scale: float
+
+    def targets(self, psi_list, n_list) -> np.ndarray:
+        B = len(psi_list)
+        H, W = psi_list[0].shape
+        out = np.empty((3, B, H, W), dtype=np.float32)
+        for b in range(B):
+            z = gauge_fix(psi_list[b].astype(np.complex128))
+            out[0, b] = z.real / self.psi_scale
+            out[1, b] = z.imag / self.psi_scale
+            out[2, b] = n_list[b] / self.n_scale
+        return out
+
+    def fields(self, y: np.ndarray, b: int):
+        psi = (y[0, b] + 1j * y[1, b]) * self.psi_scale
+        n = np.maximum(y[2, b] * self.n_scale, 0.0)
+        return psi.astype(np.complex64), n.astype(np.float32)
+
+
+def default_normaliser() -> Normaliser:
+    p = PhysicsConstants()
+    return Normaliser(psi_scale=math.sqrt(p.gamma_C / p.g_C) * 0.1, n_scale=p.gamma_C / p.R)
+
+
+def generate(members, dt=0.05, rtol=1e-4, max_time=3000.0, log=print):
+    """Cold-start steady states; returns (features, targets, converged mask, result)."""
+    t0 = time.perf_counter()
+    sw = PackedSweep(members, reservoir="single", precision="single", dt=dt, check_every=1.0, rtol=rtol,
+                     max_time=max_time, tile_h=8, progress=log)
+    res = sw.run()
+    conv = np.isfinite(res.stop_times)
+    norm = default_normaliser()
+    y = norm.targets(res.final_psi, [r[0] for r in res.final_res])
+    log(f"generated {len(members)} members ({conv.sum()} converged) in {time.perf_counter() - t0:.1f} s")
+    return features(members), y, conv, res
+
+
+def train(x, y, epochs=40, batch=16, lr=2e-3, seed=0, log=print, cfg=None, time_limit=None):
+    """Train on (6, N, H, W) / (3, N, H, W) arrays; returns (params as numpy dict, history)."""
+    import torch
+    from .fno import FNOConfig, FNOModule
+    cfg = cfg or FNOConfig(H=x.shape[2], W=x.shape[3])
+    torch.manual_seed(seed)
+    dev = "cuda"
+    X = torch.tensor(x, device=dev)
+    Y = torch.tensor(y, device=dev)
+    n = X.shape[1]
+    mod = FNOModule(cfg, seed=seed, device=dev)
+    opt = torch.optim.AdamW(mod.parameters(), lr=lr, weight_decay=1e-5)
+    steps_per_epoch = max(1, n // batch)
+    total = epochs * steps_per_epoch
+    sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=lr, total_steps=total, pct_start=0.1)
+    hist = []
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    t0 = time.perf_counter()
+    step = 0
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g)
+        acc = 0.0
+        for s in range(steps_per_epoch):
+            idx = perm[s * batch:(s + 1) * batch].to(dev)
+            xb = X[:, idx].contiguous()
+            yb = Y[:, idx].contiguous()
+            out = mod(xb)
+            loss = ((out - yb) ** 2).mean()
+            opt.zero_grad(set_to_none=True)
+            loss.backward()
+            torch.nn.utils.clip_grad_norm_(mod.parameters(), 1.0)
+            opt.step()
+            sched.step()
+            acc += float(loss)
+            step += 1
+        hist.append(acc / steps_per_epoch)
+        log(f"epoch {ep + 1}/{epochs} loss {hist[-1]:.4e} ({time.perf_counter() - t0:.0f} s)")
+        if time_limit and time.perf_counter() - t0 > time_limit:
+            log("training stopped at the time limit")
+            break
+    params = {k: v.detach().cpu().numpy() for k, v in mod.params.items()}
+    return params, hist, cfg
+
+
+def predict(params, x, cfg=None, batch=16):
+    import torch
+    from .fno import FNOConfig, FNOModule
+    cfg = cfg or FNOConfig(H=x.shape[2], W=x.shape[3])
+    mod = FNOModule(cfg, device="cuda")
+    with torch.no_grad():
+        for k, v in params.items():
+            mod.params[k].copy_(torch.tensor(v))
+        X = torch.tensor(x, device="cuda")
+        outs = []
+        for s in range(0, X.shape[1], batch):
+            outs.append(mod.net.forward(X[:, s:s + batch].contiguous(), {k: mod.params[k] for k in mod.names}))
+        torch.cuda.synchronize()
+        return torch.cat(outs, dim=1).cpu().numpy()
+
+
+def warm_start(members, y_pred, norm: Normaliser | None = None, noise_amplitude=1e-3):
+    """Members whose initial state is the prediction plus the cold start's noise field."""
+    norm = norm or default_normaliser()
+    out = []
+    for b, m in enumerate(members):
+        psi, n = norm.fields(y_pred, b)
+        noise = make_initial_psi((m.ny, m.nx), noise_amplitude, seed=m.seed, precision="single")
+        out.append(SweepMember(m.nx, m.ny, m.lx, m.ly, m.physics, m.envelope, m.P0, seed=m.seed,
+                               psi0=(psi + noise).astype(np.complex64), res0=(n,), label=dict(m.label)))
+    return out
+
+
+def save_weights(path, params, cfg, norm: Normaliser, meta=None):
+    arrays = {f"param/{k}": v.astype(np.float32) for k, v in params.items()}
+    arrays["cfg"] = np.asarray([cfg.H, cfg.W, cfg.width, cfg.layers, cfg.mx, cfg.my, cfg.fin, cfg.hidden, cfg.fout])
+    arrays["norm"] = np.asarray([norm.psi_scale, norm.n_scale])
+    if meta:
+        for k, v in meta.items():
+            arrays[f"meta/{k}"] = np.asarray(v)
+    np.savez_compressed(path, **arrays)
+
+
+def load_weights(path):
+    from .fno import FNOConfig
+    with np.load(path, allow_pickle=False) as z:
+        c = [int(v) for v in z["cfg"]]
+        cfg = FNOConfig(H=c[0], W=c[1], width=c[2], layers=c[3], modes_x=c[4], modes_y=c[5], in_features=c[6],
+                        hidden=c[7], out_features=c[8])
+        params = {k[len("param/"):]: z[k] for k in z.files if k.startswith("param/")}
+        norm = Normaliser(float(z["norm"][0]), float(z["norm"][1]))
+    return params, cfg, norm

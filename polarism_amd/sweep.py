@@ -52,12 +52,13 @@ class SweepResult:
     final_psi: list
     final_res: list
     relative_density: np.ndarray    # last relative density change per member
+    wall_to_last_s: float = float("nan")  # wall time from launch to the last convergence
 
 
 class PackedSweep:
     def __init__(self, members: list[SweepMember], *, reservoir="single", precision="single", dt=0.05,
                  check_every=1.0, rtol=1e-6, min_time=0.0, max_time=1000.0, tile_h=16,
-                 initial_amplitude=1e-3, retire=True, graphs=True):
+                 initial_amplitude=1e-3, retire=True, graphs=True, progress=None, progress_every_s=30.0):
         init()
         self.members = members
         self.reservoir = reservoir
@@ -73,6 +74,8 @@ class PackedSweep:
         self.retire = retire
         self.graphs = graphs
         self.amplitude = initial_amplitude
+        self.progress = progress
+        self.progress_every_s = progress_every_s
         self.stream = Stream()
         specs = []
         off = 0
@@ -133,7 +136,8 @@ class PackedSweep:
         t0 = time.perf_counter()
         step = 0
         checks = 0
-        self._density_check()                # prime the previous density (not a criterion check)
+        last_conv = float("nan")
+        next_report = t0 + self.progress_every_s
         for c in range(self.max_checks):
             self.engine.run_steps(self.steps_per_check, self.coef, True, False, step, self.graphs)
             point_steps += float(points[active].sum()) * self.steps_per_check
@@ -150,12 +154,18 @@ class PackedSweep:
                 if t >= self.min_time:
                     newly = reached & active
                     stop[newly] = t
+                    if newly.any():
+                        last_conv = time.perf_counter() - t0
                     if newly.any() and self.retire:
                         active &= ~newly
                         self.engine.set_active_tiles(self.table.tiles_of(np.flatnonzero(active)))
                     elif newly.any():
                         active &= ~newly
             prev_norm = norms
+            if self.progress is not None and time.perf_counter() > next_report:
+                next_report = time.perf_counter() + self.progress_every_s
+                self.progress(f"sweep t={t:.0f} ps: {int(np.isfinite(stop).sum())}/{M} converged, "
+                              f"{int(active.sum())} running, {time.perf_counter() - t0:.0f} s")
             if not active.any():
                 break
         self.stream.synchronize()
@@ -168,4 +178,4 @@ class PackedSweep:
             n = m.nx * m.ny
             final_psi.append(psi_all[o:o + n].reshape(m.ny, m.nx))
             final_res.append(tuple(r[o:o + n].reshape(m.ny, m.nx) for r in res_all))
-        return SweepResult(stop, wall, step, checks, point_steps, final_psi, final_res, rel_d_last)
+        return SweepResult(stop, wall, step, checks, point_steps, final_psi, final_res, rel_d_last, last_conv)

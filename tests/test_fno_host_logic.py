@@ -66,6 +66,22 @@ def _colsum(x, out, rows, cols, row_stride, col_stride):
     out.copy_(v.float())
 
 
+def _mode_scale(vr, vi, dr, di, ur=None, ui=None, urb=None, uib=None, conj=False):
+    d_r = dr.permute(1, 0, 2).unsqueeze(2).double()
+    d_i = di.permute(1, 0, 2).unsqueeze(2).double() * (-1.0 if conj else 1.0)
+    re = d_r * vr.double() - d_i * vi.double()
+    im = d_r * vi.double() + d_i * vr.double()
+    for dst, val in ((ur, re), (ui, im), (urb, re), (uib, im)):
+        if dst is not None:
+            dst.copy_(val.to(dst.dtype))
+
+
+def _mode_scale_grad(vr, vi, dur, dui, gdr, gdi):
+    a, b, c, d = (t.double() for t in (vr, vi, dur, dui))
+    gdr.copy_((a * c + b * d).sum(dim=2).permute(1, 0, 2).float())
+    gdi.copy_((a * d - b * c).sum(dim=2).permute(1, 0, 2).float())
+
+
 @pytest.fixture()
 def cpu_fno(monkeypatch):
     monkeypatch.setattr(F, "gemm", _gemm)
@@ -73,6 +89,8 @@ def cpu_fno(monkeypatch):
     monkeypatch.setattr(F, "gelu_fwd", _gelu_fwd)
     monkeypatch.setattr(F, "gelu_bwd", _gelu_bwd)
     monkeypatch.setattr(F, "colsum", _colsum)
+    monkeypatch.setattr(F, "mode_scale", _mode_scale)
+    monkeypatch.setattr(F, "mode_scale_grad", _mode_scale_grad)
     cfg = F.FNOConfig(H=16, W=24, width=8, layers=4, modes_x=6, modes_y=8, hidden=12)
     mod = F.FNOModule(cfg, seed=3, device="cpu")
     return cfg, mod

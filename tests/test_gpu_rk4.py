@@ -109,3 +109,57 @@ def test_tile_height_does_not_change_results(tile_h):
     ctl.solver.tile_h = tile_h
     ctl.run()
     np.testing.assert_array_equal(ctl.state.field.get(), ref.state.field.get())
+
+
+def _oracle_spinor_rk4(cfg, n_steps, res_components=2):
+    g = O.Grid(cfg.grid.nx, cfg.grid.ny, cfg.grid.lx, cfg.grid.ly)
+    prec = cfg.solver.precision
+    real = np.float32 if prec == "single" else np.float64
+    kind = cfg.reservoir.reservoir_type
+    from tests.helpers import oracle_physics
+    p = oracle_physics(cfg)
+    a = O.initial_psi(g.shape, cfg.noise.initial_amplitude, cfg.noise.seed, prec)
+    b = O.initial_psi(g.shape, cfg.noise.initial_amplitude, cfg.noise.seed + 1, prec)
+    psi = np.stack((a, b))[None]
+    nf = {"none": 0, "single": 1, "double": 2, "quadratic-double": 2}[kind]
+    res = tuple(np.zeros((1, res_components, *g.shape), dtype=real) for _ in range(nf))
+    env = O.gaussian_profile(g, cfg.laser.sigma_space).astype(real)
+    cp = cfg.laser.circular_polarization
+    weights = np.asarray([(1 + cp) / 2, (1 - cp) / 2], dtype=real).reshape(1, 2, 1, 1)
+    dt = cfg.solver.dt
+    for s in range(n_steps):
+        psi, res = O.rk4_fdm_step(kind, p, g, psi, res, lambda t: env * cfg.laser.P0, s * dt, dt,
+                                  pump_weights=weights, reservoir_components=res_components)
+    return psi, res
+
+
+@pytest.mark.parametrize("kind", ["single", "double"])
+@pytest.mark.parametrize("res_components", [2, 1])
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("single", 1e-5)])
+def test_spinor_rk4_matches_oracle(kind, res_components, precision, tol):
+    cfg = quickstart_config(grid__nx=96, grid__ny=64, grid__lx=48.0, grid__ly=32.0, solver__dt=2e-3,
+                            solver__total_time=0.2, physics__spin="spinor", physics__te_tm=0.05,
+                            physics__zeeman=0.02, physics__alpha_2=-1e-4, laser__circular_polarization=0.3,
+                            laser__P0=0.3, laser__sigma_space=8.0, reservoir__reservoir_type=kind,
+                            solver__precision=precision, noise__initial_amplitude=0.05)
+    cfg.reservoir.components = res_components
+    ctl = _run(cfg)
+    psi_ref, res_ref = _oracle_spinor_rk4(cfg, 100, res_components)
+    assert O.rel_l2(ctl.state.field.get(), psi_ref) <= tol
+    for d, r in zip(ctl.reservoir.get_state(), res_ref):
+        assert O.rel_l2(d.get(), r) <= tol
+
+
+def test_spinor_without_coupling_equals_two_scalar_runs():
+    """Zeeman, TE-TM and alpha_2 off, shared nothing: each spin evolves as the scalar field
+    (reference tests/unit/test_spin_8a.py:78-103)."""
+    kw = dict(grid__nx=64, grid__ny=64, solver__total_time=0.05, laser__P0=0.03, noise__initial_amplitude=0.05)
+    spin = quickstart_config(physics__spin="spinor", **kw)
+    s = _run(spin)
+    # circular_polarization 0 gives each spin its own reservoir and half the pump
+    kw["laser__P0"] = 0.015
+    half = quickstart_config(**kw)
+    for c, seed in enumerate((0, 1)):
+        half.noise.seed = seed
+        ref = _run(half)
+        np.testing.assert_allclose(s.state.field.get()[0, c], ref.state.field.get()[0, 0], rtol=1e-12, atol=1e-18)

@@ -1,0 +1,72 @@
+"""Packed sweeps: uneven grids in one launch, retirement, steady-state semantics."""
+import numpy as np
+import pytest
+
+from oracle import gpe_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(sizes, reservoir="double", seed=5):
+    from polarism_amd.config import PhysicsConstants
+    from polarism_amd.pumps import random_pump, threshold
+    from polarism_amd.sweep import SweepMember
+    p = PhysicsConstants()
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, (nx, ny) in enumerate(sizes):
+        lx, ly = 0.5 * nx, 0.5 * ny
+        env, _ = random_pump(rng, nx, ny, lx, ly, "gaussians")
+        out.append(SweepMember(nx, ny, lx, ly, p, env, rng.uniform(0.5, 3.0) * threshold(p, reservoir),
+                               seed=100 + i))
+    return out
+
+
+def _run(members, reservoir="double", precision="single", max_time=2.0, rtol=0.0, tile_h=8):
+    from polarism_amd.sweep import PackedSweep
+    sw = PackedSweep(members, reservoir=reservoir, precision=precision, dt=0.05, check_every=0.5, rtol=rtol,
+                     max_time=max_time, tile_h=tile_h)
+    return sw.run()
+
+
+SIZES = [(64, 64), (96, 96), (128, 96), (192, 64), (64, 128), (256, 256)]
+
+
+def test_uneven_grid_members_equal_single_member_runs_bitwise():
+    members = _members(SIZES)
+    packed = _run(members)
+    for i, m in enumerate(members):
+        single = _run([m])
+        np.testing.assert_array_equal(packed.final_psi[i], single.final_psi[0])
+        for a, b in zip(packed.final_res[i], single.final_res[0]):
+            np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("single", 1e-5)])
+def test_uneven_grid_member_matches_oracle(precision, tol):
+    members = _members(SIZES[:3])
+    res = _run(members, precision=precision, max_time=1.0)
+    from polarism_amd.noise import make_initial_psi
+    for i, m in enumerate(members):
+        g = O.Grid(m.nx, m.ny, m.lx, m.ly)
+        real = np.float32 if precision == "single" else np.float64
+        psi = make_initial_psi((m.ny, m.nx), 1e-3, seed=m.seed, precision=precision).reshape(1, 1, m.ny, m.nx)
+        res0 = (np.zeros((1, 1, m.ny, m.nx), real), np.zeros((1, 1, m.ny, m.nx), real))
+        env = m.envelope.astype(real)
+        p = O.Physics()
+        for s in range(20):
+            psi, res0 = O.rk4_fdm_step("double", p, g, psi, res0, lambda t: env * m.P0, s * 0.05, 0.05)
+        assert O.rel_l2(res.final_psi[i], psi[0, 0]) <= tol
+        assert O.rel_l2(res.final_res[i][0], res0[0][0, 0]) <= tol
+
+
+def test_retirement_does_not_change_running_members():
+    members = _members(SIZES[:4], reservoir="single")
+    # member 0 gets no pump and a zero initial field: it converges (norm < 1e-30) at the second check
+    members[0].P0 = 0.0
+    members[0].psi0 = np.zeros((64, 64), np.complex64)
+    a = _run(members, reservoir="single", max_time=3.0, rtol=1e-12)
+    assert np.isfinite(a.stop_times[0]) and not np.isfinite(a.stop_times[1:]).any()
+    b = _run(members[1:], reservoir="single", max_time=3.0, rtol=1e-12)
+    for i in range(1, 4):
+        np.testing.assert_array_equal(a.final_psi[i], b.final_psi[i - 1])

@@ -1,0 +1,337 @@
+"""Pins of the oracle by closed forms, invariants and brute force (CPU only).
+
+Each test checks an oracle function against something the mathematics fixes,
+chosen so that a dropped term, a wrong sign or a transposed index fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+from oracle import gpe_oracle as O
+
+
+def _zero_pump(grid, dtype=np.float64):
+    z = np.zeros((1, 1) + grid.shape, dtype=dtype)
+    return lambda t: z
+
+
+# ---------------------------------------------------------------- Laplacian
+def test_laplacian_cosine_eigenfunction_and_second_order():
+    """The periodic 5-point stencil maps cos(kx x) cos(ky y) to the discrete symbol times itself."""
+    errs = []
+    for n in (32, 64):
+        g = O.Grid(n, n // 2 * 3 // 2, 40.0, 30.0)
+        kx, ky = 2 * np.pi * 3 / g.lx, 2 * np.pi * 2 / g.ly
+        f = np.cos(kx * g.X) * np.cos(ky * g.Y)
+        lap = O.laplacian(f, g.dx, g.dy)
+        sym = -(2 - 2 * np.cos(kx * g.dx)) / g.dx ** 2 - (2 - 2 * np.cos(ky * g.dy)) / g.dy ** 2
+        np.testing.assert_allclose(lap, sym * f, atol=1e-12 * abs(sym))
+        errs.append(abs(sym + kx ** 2 + ky ** 2))
+    assert 3.8 < errs[0] / errs[1] < 4.2
+
+
+def test_te_tm_stencil_reduces_to_dispersion_coupling():
+    """-beta[(dxx - dyy) - 2i dxy] on a plane-wave psi_- equals beta(kx - i ky)^2 psi_- up to the stencil symbols."""
+    g = O.Grid(64, 48, 64.0, 48.0)
+    p = O.Physics(te_tm=0.05)
+    kx, ky = 2 * np.pi * 2 / g.lx, 2 * np.pi * 3 / g.ly
+    wave = np.exp(1j * (kx * g.X + ky * g.Y))
+    sx = -(2 - 2 * np.cos(kx * g.dx)) / g.dx ** 2
+    sy = -(2 - 2 * np.cos(ky * g.dy)) / g.dy ** 2
+    sxy = -np.sin(kx * g.dx) * np.sin(ky * g.dy) / (g.dx * g.dy)
+    for src in (0, 1):
+        field = np.zeros((1, 2) + g.shape, dtype=complex)
+        field[0, src] = wave
+        out = O.te_tm_energy(p, field, g.dx, g.dy)
+        dst = 1 - src
+        sign = -1 if dst == 0 else 1          # psi_+ row gets -2i dxy, psi_- row +2i dxy
+        expect = -p.te_tm * ((sx - sy) + sign * 2j * sxy) * wave
+        np.testing.assert_allclose(out[0, dst], expect, atol=1e-12)
+        np.testing.assert_allclose(out[0, src], 0, atol=1e-15)
+    # continuum limit of the psi_+ row: beta (kx - i ky)^2, the off-diagonal of the dispersion matrix
+    H = O.spinor_hamiltonian(g, p)
+    i, j = 3, 2                                 # grid index of (ky, kx) = (3, 2) modes
+    assert np.isclose(g.KX[i, j], kx) and np.isclose(g.KY[i, j], ky)
+    cont = -p.te_tm * ((sx - sy) - 2j * sxy)
+    # stencil symbols differ from kx, ky at O((k dx)^2); a wrong sign of the 2i dxy term is O(1) off
+    assert abs(cont - H[0, 1][i, j]) < 5e-2 * abs(H[0, 1][i, j])
+
+
+# ---------------------------------------------------------------- spinor dispersion
+@pytest.mark.parametrize("zeeman,te_tm", [(0.02, 0.05), (0.0, 0.05), (0.02, 0.0), (0.0, 0.0)])
+def test_spinor_propagator_equals_matrix_exponential(zeeman, te_tm):
+    g = O.Grid(8, 6, 10.0, 7.0)
+    p = O.Physics(zeeman=zeeman, te_tm=te_tm)
+    tau = 0.37
+    M = O.spinor_propagator(g, p, tau)
+    H = O.spinor_hamiltonian(g, p)
+    for i in range(g.ny):
+        for j in range(g.nx):
+            ref = expm(-1j * H[:, :, i, j] * tau / p.hbar)
+            np.testing.assert_allclose(M[:, :, i, j], ref, atol=1e-13)
+    # Hermitian H: the propagator is unitary
+    MM = np.einsum("abij,cbij->acij", M, M.conj())
+    np.testing.assert_allclose(MM, np.eye(2)[:, :, None, None] * np.ones(g.shape), atol=1e-13)
+
+
+def test_zeeman_only_spinor_rhs_equals_scalar_with_offset_potential():
+    """+Z/2 on psi_+, -Z/2 on psi_-: the same sign as the dispersion matrix diagonal."""
+    g = O.Grid(16, 16, 20.0, 20.0)
+    p = O.Physics(zeeman=0.03)
+    rng = np.random.default_rng(3)
+    psi = (rng.standard_normal((1, 2) + g.shape) + 1j * rng.standard_normal((1, 2) + g.shape)) * 0.3
+    res = (np.abs(rng.standard_normal((1, 2) + g.shape)),)
+    pump = np.full((1, 1) + g.shape, 0.02)
+    d, _ = O.fdm_rhs("single", p, g, psi, res, pump, pump_weights=np.ones((1, 2, 1, 1)), reservoir_components=2)
+    for c, sgn in ((0, 1), (1, -1)):
+        ds, _ = O.fdm_rhs("single", p.with_(zeeman=0.0), g, psi[:, c:c + 1], (res[0][:, c:c + 1],), pump,
+                          potential=sgn * p.zeeman / 2)
+        np.testing.assert_allclose(d[:, c], ds[:, 0], rtol=1e-13, atol=1e-16)
+
+
+# ---------------------------------------------------------------- uniform closed forms
+def _uniform_exact(p, t, amp=1.0):
+    gamma = p.gamma_C
+    phase = -p.g_C * amp ** 2 / (p.hbar * gamma) * (1 - np.exp(-gamma * t))
+    return amp * np.exp(-gamma * t / 2 + 1j * phase)
+
+
+def test_rk4_fdm_uniform_decay_is_fourth_order():
+    g = O.Grid(4, 4, 4.0, 4.0)
+    p = O.Physics(g_C=0.5, gamma_C=0.4)
+    T = 2.0
+    errs = []
+    for dt in (0.2, 0.1):
+        psi = np.ones((1, 1) + g.shape, dtype=complex)
+        for s in range(int(round(T / dt))):
+            psi, _ = O.rk4_fdm_step("none", p, g, psi, (), _zero_pump(g), s * dt, dt)
+        errs.append(np.max(np.abs(psi - _uniform_exact(p, T))))
+    assert errs[1] < 1e-5
+    assert 13 < errs[0] / errs[1] < 19
+
+
+@pytest.mark.parametrize("stepper", [O.split_step_cpu, O.split_step_kouter])
+def test_split_step_exact_for_uniform_field(stepper):
+    """The exact local step integrates uniform gain/loss and the self phase exactly."""
+    g = O.Grid(8, 8, 8.0, 8.0)
+    p = O.Physics(g_C=0.5, gamma_C=0.4)
+    dt, n = 0.05, 40
+    psi = np.full((1, 1) + g.shape, 0.7 + 0j)
+    for s in range(n):
+        psi, _ = stepper("none", p, g, psi, (), _zero_pump(g), s * dt, dt, s)
+    np.testing.assert_allclose(psi, _uniform_exact(p, n * dt, 0.7), rtol=1e-12)
+
+
+@pytest.mark.parametrize("stepper", [O.split_step_cpu, O.split_step_kouter])
+def test_split_step_plane_wave_phase(stepper):
+    g = O.Grid(16, 8, 20.0, 10.0)
+    p = O.Physics(g_C=0.0, gamma_C=0.0)
+    kx, ky = 2 * np.pi * 3 / g.lx, -2 * np.pi * 1 / g.ly
+    psi = np.exp(1j * (kx * g.X + ky * g.Y))[None, None]
+    psi0 = psi.copy()
+    dt, n = 0.1, 25
+    for s in range(n):
+        psi, _ = stepper("none", p, g, psi, (), _zero_pump(g), s * dt, dt, s)
+    omega = p.hbar * (kx ** 2 + ky ** 2) / (2 * p.m_eff)
+    np.testing.assert_allclose(psi, psi0 * np.exp(-1j * omega * n * dt), atol=1e-12)
+
+
+def test_split_step_orderings_differ_at_second_order():
+    g = O.Grid(32, 32, 30.0, 30.0)
+    p = O.Physics(g_C=0.05)
+    env = O.gaussian_profile(g, 6.0)[None, None]
+    pump = lambda t: 0.5 * env
+    psi0 = np.full((1, 1) + g.shape, 0.5 + 0j) * (1 + 0.2 * np.cos(2 * np.pi * g.X / g.lx))
+    diffs = []
+    T = 1.0
+    for dt in (0.02, 0.01):
+        a = b = psi0
+        ra = rb = (np.zeros((1, 1) + g.shape),)
+        for s in range(int(round(T / dt))):
+            a, ra = O.split_step_cpu("single", p, g, a, ra, pump, s * dt, dt, s)
+            b, rb = O.split_step_kouter("single", p, g, b, rb, pump, s * dt, dt, s)
+        diffs.append(O.rel_l2(ra[0], rb[0]))
+    assert diffs[1] > 1e-11
+    assert 3.5 < diffs[0] / diffs[1] < 4.5
+
+
+# ---------------------------------------------------------------- reservoirs and thresholds
+def _fixed_point_at_zero_density(kind, p, P):
+    """Reservoir fixed point with psi = 0 by solving the linear system dn/dt = 0."""
+    if kind == "single":
+        return (P / p.gamma_R,)
+    # double: [[-(gA + R_AI), R_IA], [R_AI, -(gI + R_IA)]] (nA, nI) = (0, -P)
+    A = np.array([[-(p.gamma_A + p.R_AI), p.R_IA], [p.R_AI, -(p.gamma_I + p.R_IA)]])
+    nA, nI = np.linalg.solve(A, [0.0, -P])
+    return (nA, nI)
+
+
+@pytest.mark.parametrize("kind,thr", [("single", O.threshold_single), ("double", O.threshold_double)])
+def test_threshold_makes_the_gain_vanish(kind, thr):
+    p = O.Physics()
+    state = _fixed_point_at_zero_density(kind, p, thr(p))
+    d = O.reservoir_derivatives(kind, p, 0.0, thr(p), tuple(np.float64(s) for s in state))
+    assert max(abs(x) for x in d) < 1e-15
+    assert abs(state[0] - p.gamma_C / p.R) < 1e-12 * p.gamma_C / p.R
+    assert abs(O.threshold_single(p) - 0.02075) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["single", "double", "quadratic-double"])
+def test_condensed_uniform_steady_state_is_a_fixed_point(kind):
+    """Closed-form uniform steady state above threshold: the reservoir stops and |psi| is constant."""
+    p = O.Physics(g_I=0.001)
+    g = O.Grid(4, 4, 4.0, 4.0)
+    nA = p.gamma_C / p.R
+    if kind == "single":
+        P = 3 * O.threshold_single(p)
+        rho = P / p.gamma_C - p.gamma_R / p.R
+        state = (nA,)
+    elif kind == "double":
+        P = 3 * O.threshold_double(p)
+        nI = (P + p.R_AI * nA) / (p.gamma_I + p.R_IA)
+        rho = (p.R_IA * nI / nA - p.gamma_A - p.R_AI) / p.R
+        state = (nA, nI)
+    else:
+        nI = 1.5 * math.sqrt(p.gamma_R * nA / p.kappa)
+        P = p.kappa * nI ** 2 + p.gamma_I * nI
+        rho = (p.kappa * nI ** 2 - p.gamma_R * nA) / (p.R * nA)
+        state = (nA, nI)
+    assert rho > 0
+    psi = np.full((1, 1) + g.shape, math.sqrt(rho) * np.exp(0.3j))
+    res = tuple(np.full((1, 1) + g.shape, s) for s in state)
+    pump = np.full((1, 1) + g.shape, P)
+    dpsi, dres = O.fdm_rhs(kind, p, g, psi, res, pump)
+    for d in dres:
+        np.testing.assert_allclose(d, 0, atol=1e-15)
+    nI_eff = state[1] if kind != "single" else 0.0
+    mu = p.g_C * rho + p.g_R * nA + p.g_I * nI_eff
+    np.testing.assert_allclose(dpsi, (-1j * mu / p.hbar) * psi, atol=1e-15)
+
+
+def test_reservoir_rk2_zero_pump_decay_factor():
+    """psi = 0, P = 0: one midpoint step multiplies nR by 1 - gR dt + (gR dt)^2 / 2 exactly."""
+    p = O.Physics()
+    dt = 0.7
+    n0 = np.full((1, 1, 3, 3), 2.5)
+    psi = np.zeros((1, 1, 3, 3), dtype=complex)
+    (n1,) = O.reservoir_rk2("single", p, dt, psi, 0.0, (n0,))
+    x = p.gamma_R * dt
+    np.testing.assert_allclose(n1, n0 * (1 - x + x * x / 2), rtol=1e-15)
+    # second order against exp(-gR t)
+    errs = []
+    for h in (0.5, 0.25):
+        n = n0.copy()
+        for _ in range(int(round(20 / h))):
+            (n,) = O.reservoir_rk2("single", p, h, psi, 0.0, (n,))
+        errs.append(abs(n[0, 0, 0, 0] - 2.5 * math.exp(-p.gamma_R * 20)))
+    assert 3.5 < errs[0] / errs[1] < 4.5
+
+
+def test_reservoir_clamp_keeps_densities_non_negative():
+    p = O.Physics()
+    psi = np.zeros((1, 1, 2, 2), dtype=complex)
+    n0 = np.array([-1.0, 0.5, -1e-3, 2.0]).reshape(1, 1, 2, 2)
+    (n1,) = O.reservoir_rk2("single", p, 0.5, psi, 0.0, (n0,))
+    x = p.gamma_R * 0.5
+    np.testing.assert_array_equal(n1 == 0, n0 < 0)
+    np.testing.assert_allclose(n1[n0 > 0], n0[n0 > 0] * (1 - x + x * x / 2), rtol=1e-15)
+
+
+# ---------------------------------------------------------------- adaptive Lawson + controller
+@pytest.mark.parametrize("tableau", ["dp5", "dop853"])
+def test_lawson_adaptive_uniform_decay_closed_form(tableau):
+    """Error <= 50 rtol against the closed form; a tighter rtol takes more accepted steps."""
+    g = O.Grid(4, 4, 4.0, 4.0)
+    p = O.Physics(g_C=0.5, gamma_C=0.4)
+    accepted = []
+    for rtol in (1e-6, 1e-9):
+        prob = O.LawsonProblem("none", p, g, tableau=tableau, adaptive=True, rtol=rtol, atol=rtol * 1e-3)
+        psi = np.ones((1, 1) + g.shape, dtype=complex)
+        psi, _, ctl = O.lawson_advance_adaptive(prob, psi, (), _zero_pump(g), 0.0, 1.0, 3)
+        assert np.max(np.abs(psi - _uniform_exact(p, 3.0))) <= 50 * rtol
+        accepted.append(ctl.accepted)
+    assert accepted[0] < accepted[1]
+
+
+def test_lawson_fixed_rk4_is_fourth_order_on_uniform_decay():
+    g = O.Grid(4, 4, 4.0, 4.0)
+    p = O.Physics(g_C=0.5, gamma_C=0.4)
+    errs = []
+    for h in (0.2, 0.1):
+        prob = O.LawsonProblem("none", p, g, tableau="rk4")
+        psi = np.ones((1, 1) + g.shape, dtype=complex)
+        for s in range(int(round(2.0 / h))):
+            _, _, psi, _ = prob.step(psi, (), _zero_pump(g), s * h, h)
+        errs.append(np.max(np.abs(psi - _uniform_exact(p, 2.0))))
+    assert 13 < errs[0] / errs[1] < 19
+
+
+def test_controller_rejects_then_lands_on_the_tick():
+    g = O.Grid(4, 4, 4.0, 4.0)
+    p = O.Physics(g_C=0.5, gamma_C=0.4)
+    prob = O.LawsonProblem("none", p, g, tableau="dp5", adaptive=True, rtol=1e-6, atol=1e-9)
+    psi = np.ones((1, 1) + g.shape, dtype=complex)
+    psi, _, ctl = O.lawson_advance_adaptive(prob, psi, (), _zero_pump(g), 0.0, 5.0, 1)
+    assert ctl.rejected > 0 and ctl.accepted > 1
+    assert np.max(np.abs(psi - _uniform_exact(p, 5.0))) <= 50e-6
+
+
+def test_controller_underflow_guard():
+    ctl = O.StepController(1.0, 5, 4)
+    ctl.h_next = 1e-300
+    with pytest.raises(RuntimeError, match="underflow"):
+        ctl.propose(1.0, 2.0)
+
+
+def test_controller_factor_bounds():
+    ctl = O.StepController(1.0, 5, 4)
+    h, clamped, prop = ctl.propose(0.0, 10.0)
+    ctl.update(True, 0.0, h, clamped, prop)
+    assert ctl.h_next == 5.0                     # error 0 -> factor 5
+    h, clamped, prop = ctl.propose(0.0, 100.0)
+    ctl.update(False, 1e12, h, clamped, prop)
+    assert ctl.h_next == pytest.approx(0.2 * h)  # huge error -> factor 0.2
+    h2, clamped, prop = ctl.propose(0.0, 100.0)
+    ctl.update(True, 1e-12, h2, clamped, prop)
+    assert ctl.h_next <= h2                      # no growth right after a reject
+
+
+# ---------------------------------------------------------------- steady state, initial field, pulses
+def test_steady_metrics_of_scaled_field():
+    rng = np.random.default_rng(1)
+    psi = rng.standard_normal((2, 1, 8, 8)) + 1j * rng.standard_normal((2, 1, 8, 8))
+    dens, norms, _, _ = O.steady_metrics(psi, None, 0.5)
+    np.testing.assert_allclose(norms, 0.5 * (np.abs(psi) ** 2).sum(axis=(1, 2, 3)))
+    _, _, rd, rn = O.steady_metrics(psi, dens, 0.5)
+    np.testing.assert_array_equal(rd, 0)
+    np.testing.assert_array_equal(rn, 0)
+    s = np.array([1.1, 0.9]).reshape(2, 1, 1, 1)
+    _, _, rd, rn = O.steady_metrics(psi * s, dens, 0.5)
+    np.testing.assert_allclose(rd, np.abs(s.ravel() ** 2 - 1), rtol=1e-12)
+    np.testing.assert_allclose(rn, np.abs(s.ravel() ** 2 - 1), rtol=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_initial_psi_zero_mean_and_rms(precision):
+    psi = O.initial_psi((32, 24), 1e-3, 7, precision)
+    tol = 1e-6 if precision == "single" else 1e-13
+    assert abs(np.mean(psi)) < tol * 1e-3
+    assert abs(math.sqrt(np.mean(np.abs(psi) ** 2)) - 1e-3) < tol * 1e-3
+    np.testing.assert_array_equal(psi, O.initial_psi((32, 24), 1e-3, 7, precision))
+    assert not np.array_equal(psi, O.initial_psi((32, 24), 1e-3, 8, precision))
+
+
+def test_pulse_train_peaks_and_cutoffs():
+    kw = dict(P0=0.4, Pmax=1.2, sigma_time=0.1, cutoff_sigma=3.0, pulse_separation=0.6, n_pulses=0, delay=0.05)
+    first = kw["delay"] + 0.3
+    assert O.pulse_gaussian_coeff(first, **kw) == pytest.approx(0.4)
+    assert O.pulse_gaussian_coeff(first + 0.6, **kw) == pytest.approx(1.2)
+    assert O.pulse_gaussian_coeff(first + 1.2, **kw) == pytest.approx(1.2)
+    assert O.pulse_gaussian_coeff(first + 0.1, **kw) == pytest.approx(0.4 * math.exp(-0.5))
+    assert O.pulse_gaussian_coeff(0.0, **kw) == 0.0
+    sparse = {**kw, "pulse_separation": 1.0}
+    assert O.pulse_gaussian_coeff(first + 0.29, **sparse) > 0.0
+    assert O.pulse_gaussian_coeff(first + 0.31, **sparse) == 0.0          # beyond 3 sigma, before the next pulse
+    assert O.pulse_gaussian_coeff(first + 0.6, **{**kw, "n_pulses": 1}) == 0.0
