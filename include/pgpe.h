@@ -161,6 +161,17 @@ typedef struct pgpe_rk4_args {
 
 int pgpe_rk4_stage(const pgpe_rk4_args* args, void* stream);
 
+/* One whole RK4 step (all four stages, the combination, the clamp and the optional
+ * Wigner increment) in a single launch over the same member table and tile map; the
+ * same result as the four pgpe_rk4_stage launches, bit for bit.  Replaces the
+ * reference's one-launch tiled step (rk4_cuda_solver.py:1835-2194).  Scalar fields
+ * (components 1); tile_y is the table's tile_h: 16, 32, or 64 (single precision).
+ * Uses psi0, res0, pump, psi_out, res_out, noise, step, step_base and coef, which here
+ * points at 4 x members coefficients (stage s of member m at coef[s * members + m]).
+ * psi_out must not alias psi0 and res_out must not alias res0 (tiles read their
+ * neighbours' step-start state). */
+int pgpe_rk4_step(const pgpe_rk4_args* args, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* Steady-state reduction (replaces simulation_controller.py:710-734)   */
 /* For every member: sums[3m+0] = sum rho, sums[3m+1] = sum (rho-old)^2, */

@@ -664,14 +664,21 @@ class StepController:
         self.h_next = min(h_next, self.dt_max or math.inf)
 
 
-def lawson_advance_adaptive(prob: LawsonProblem, psi, res, pump_at, t0, dt, n_ticks, ctl=None):
-    """Adaptive trials landing exactly on ticks t0 + k*dt (abstract_solver.py:86-96)."""
+def lawson_advance_adaptive(prob: LawsonProblem, psi, res, pump_at, t0, dt, n_ticks, ctl=None, chunk=1):
+    """Adaptive trials in advances of ``chunk`` ticks (abstract_solver.py:86-96).
+
+    Each advance runs trials until ``t`` reaches its target ``t0 + (step + chunk) dt``;
+    only that target is landed on exactly (the controller clamps the last trial).
+    The reference's run loop advances one tick at a time when it records every step.
+    """
     tab = prob._tab
     if ctl is None:
         ctl = StepController(dt, tab.order, tab.embedded_order)
     t = t0
-    for tick in range(1, n_ticks + 1):
-        target = t0 + tick * dt
+    step = 0
+    while step < n_ticks:
+        n = min(chunk, n_ticks - step)
+        target = t0 + (step + n) * dt
         while t < target:
             h, clamped, prop = ctl.propose(t, target)
             ok, err, p_new, r_new = prob.step(psi, res, pump_at, t, h)
@@ -679,7 +686,8 @@ def lawson_advance_adaptive(prob: LawsonProblem, psi, res, pump_at, t0, dt, n_ti
                 psi, res = p_new, r_new
                 t = target if clamped else t + h
             ctl.update(ok, err, h, clamped, prop)
-        t = t0 + tick * dt
+        step += n
+        t = t0 + step * dt
     return psi, res, ctl
 
 

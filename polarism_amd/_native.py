@@ -134,6 +134,7 @@ _SIGNATURES = {
     "pgpe_normal_pairs": (C.c_int, [vp, C.c_int, C.c_uint64, C.c_uint64, C.c_uint32, C.c_int64, vp]),
     "pgpe_wigner_add": (C.c_int, [vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, C.c_uint64, vp]),
     "pgpe_rk4_stage": (C.c_int, [C.POINTER(Rk4Args), vp]),
+    "pgpe_rk4_step": (C.c_int, [C.POINTER(Rk4Args), vp]),
     "pgpe_fft_plan_create": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(vp)]),
     "pgpe_fft_plan_destroy": (C.c_int, [vp]),
     "pgpe_fft2": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),

@@ -40,13 +40,57 @@ struct GemmParams {
     int bias_mode;           // 0 none, 1 per row (m), 2 per column (n)
     int gelu;                // 1: apply tanh-free GELU (erf form)
     int out_bf16;
+    int vec_a, vec_b;        // 16-byte vector loads are aligned for A / B
 };
 
 __device__ __forceinline__ float gelu_erf(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752440f)); }
 
+// Stage a 128 (rows r) x 32 (k) bf16 tile of a strided operand into LDS as
+// S[r][k].  A k-contiguous operand (sk == 1) is read as 16-byte vectors of 8
+// consecutive k; an r-contiguous one (sr == 1) as vectors of 8 consecutive r,
+// scattered into the k-contiguous LDS rows.  ``vec`` says the base pointer and
+// strides keep every vector 16-byte aligned; edge vectors fall back to
+// element loads with bounds checks.
+__device__ __forceinline__ void load_tile(__hip_bfloat16 (*S)[GT_K + GT_PAD], const __hip_bfloat16* __restrict__ G,
+                                          int r0, int R, int64_t sr, int k0, int K, int64_t sk, int vec, int tid) {
+    const __hip_bfloat16 zero = __float2bfloat16(0.0f);
+    if (sk == 1) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int v = tid + 256 * i;        // 512 vectors: 128 rows x 4 vectors of 8 k
+            const int rr = v >> 2, kq = (v & 3) * 8;
+            const int gr = r0 + rr, gk = k0 + kq;
+            if (vec && gr < R && gk + 8 <= K) {
+                *reinterpret_cast<uint4*>(&S[rr][kq]) = *reinterpret_cast<const uint4*>(G + gr * sr + gk);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    S[rr][kq + j] = (gr < R && gk + j < K) ? G[gr * sr + gk + j] : zero;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int v = tid + 256 * i;        // 512 vectors: 32 k x 16 vectors of 8 rows
+            const int kk = v >> 4, rq = (v & 15) * 8;
+            const int gr = r0 + rq, gk = k0 + kk;
+            if (vec && gk < K && gr + 8 <= R) {
+                union { uint4 u; __hip_bfloat16 h[8]; } t;
+                t.u = *reinterpret_cast<const uint4*>(G + gk * sk + gr);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) S[rq + j][kk] = t.h[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    S[rq + j][kk] = (gk < K && gr + j < R) ? G[gk * sk + (gr + j) * sr] : zero;
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmParams p) {
-    __shared__ __hip_bfloat16 As[GT_M][GT_K + GT_PAD];
-    __shared__ __hip_bfloat16 Bs[GT_N][GT_K + GT_PAD];
+    __shared__ __attribute__((aligned(16))) __hip_bfloat16 As[GT_M][GT_K + GT_PAD];
+    __shared__ __attribute__((aligned(16))) __hip_bfloat16 Bs[GT_N][GT_K + GT_PAD];
     const int bz = blockIdx.z;
     const int m0 = blockIdx.y * GT_M;
     const int n0 = blockIdx.x * GT_N;
@@ -63,21 +107,9 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmParams p) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-    const __hip_bfloat16 zero = __float2bfloat16(0.0f);
     for (int k0 = 0; k0 < p.K; k0 += GT_K) {
-        // stage A[m0 .. +128][k0 .. +32] and B[k0 .. +32][n0 .. +128] into LDS
-        for (int e = tid; e < GT_M * GT_K; e += 256) {
-            int mm, kk;
-            if (p.sak == 1) { mm = e / GT_K; kk = e % GT_K; } else { kk = e / GT_M; mm = e % GT_M; }
-            const int gm = m0 + mm, gk = k0 + kk;
-            As[mm][kk] = (gm < p.M && gk < p.K) ? A[gm * p.sam + gk * p.sak] : zero;
-        }
-        for (int e = tid; e < GT_N * GT_K; e += 256) {
-            int nn, kk;
-            if (p.sbk == 1) { nn = e / GT_K; kk = e % GT_K; } else { kk = e / GT_N; nn = e % GT_N; }
-            const int gn = n0 + nn, gk = k0 + kk;
-            Bs[nn][kk] = (gn < p.N && gk < p.K) ? B[gk * p.sbk + gn * p.sbn] : zero;
-        }
+        load_tile(As, A, m0, p.M, p.sam, k0, p.K, p.sak, p.vec_a, tid);
+        load_tile(Bs, B, n0, p.N, p.sbn, k0, p.K, p.sbk, p.vec_b, tid);
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < GT_K; ks += 16) {
@@ -201,6 +233,23 @@ __global__ void colsum_kernel(const float* __restrict__ x, float* __restrict__ o
     out[c] = s;
 }
 
+// out[c] = sum_r x[c * col_stride + r] for a column stored contiguously
+// (row_stride 1): one workgroup per column, strided partial sums, then a
+// wavefront-shuffle and LDS reduction in a fixed order (deterministic).
+__global__ void __launch_bounds__(256) colsum_contig_kernel(const float* __restrict__ x, float* __restrict__ out,
+                                                             int64_t rows, int64_t col_stride) {
+    __shared__ float part[4];
+    const int64_t c = blockIdx.x;
+    const float* __restrict__ col = x + c * col_stride;
+    float s = 0.0f;
+    for (int64_t r = threadIdx.x; r < rows; r += 256) s += col[r];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[c] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
 static dim3 blocks1d(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace pgpe
@@ -233,6 +282,13 @@ int pgpe_gemm_bf16(const pgpe_gemm_args* a, void* stream) {
     p.bias_mode = a->bias_mode;
     p.gelu = a->gelu;
     p.out_bf16 = a->out_bf16;
+    // 16-byte vectors along the unit-stride axis need a 16-byte aligned base and strides
+    // that are multiples of 8 elements on the other axes.
+    auto aligned = [](const void* ptr, int64_t s1, int64_t s2) {
+        return ((uintptr_t)ptr % 16 == 0) && (s1 % 8 == 0) && (s2 % 8 == 0);
+    };
+    p.vec_a = a->sak == 1 ? aligned(a->A, a->sam, a->sab) : aligned(a->A, a->sak, a->sab);
+    p.vec_b = a->sbk == 1 ? aligned(a->B, a->sbn, a->sbb) : aligned(a->B, a->sbk, a->sbb);
     const dim3 grid((a->N + GT_N - 1) / GT_N, (a->M + GT_M - 1) / GT_M, a->batch);
     hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream, p);
     PGPE_LAUNCHED("gemm_bf16_kernel");
@@ -289,6 +345,12 @@ int pgpe_colsum(const float* x, float* out, int64_t rows, int64_t cols, int64_t 
                 void* stream) {
     if (rows <= 0 || cols <= 0) return fail(PGPE_ERR_ARG, "pgpe_colsum: empty");
     if (!x || !out) return fail(PGPE_ERR_ARG, "pgpe_colsum: null pointer");
+    if (row_stride == 1 && cols <= 0x7fffffff) {
+        hipLaunchKernelGGL(colsum_contig_kernel, dim3((unsigned)cols), dim3(256), 0, (hipStream_t)stream, x, out, rows,
+                           col_stride);
+        PGPE_LAUNCHED("colsum_contig_kernel");
+        return PGPE_OK;
+    }
     hipLaunchKernelGGL(colsum_kernel, blocks1d(cols), dim3(256), 0, (hipStream_t)stream, x, out, rows, cols,
                        row_stride, col_stride);
     PGPE_LAUNCHED("colsum_kernel");

@@ -72,6 +72,48 @@ __device__ __forceinline__ void reservoir_rhs(const pgpe_member& mb, T rho, T P,
 
 template <int RES> struct ResFields { static constexpr int n = (RES == 0) ? 0 : (RES == 1 ? 1 : 2); };
 
+// Member constants of the scalar right-hand side in the working precision.
+template <typename T>
+struct ScalarConsts {
+    T inv_dx2, inv_dy2, kin, inv_hbar, gC, gR, gI, R, gammaC, eta_c, hb2m;
+    __device__ __forceinline__ explicit ScalarConsts(const pgpe_member& mb)
+        : inv_dx2((T)mb.inv_dx2), inv_dy2((T)mb.inv_dy2), kin((T)mb.kin), inv_hbar((T)mb.inv_hbar),
+          gC((T)mb.g_C), gR((T)mb.g_R), gI((T)mb.g_I), R((T)mb.R), gammaC((T)mb.gamma_C), eta_c((T)mb.eta),
+          hb2m((T)mb.hbar_over_2m) {}
+};
+
+// Scalar condensate and reservoir derivatives at one point from the stage
+// input ps, its four neighbours and the stage reservoir n (model.py:89-127).
+// Both the per-stage and the whole-step kernels call this, so they perform
+// the same operations in the same order.
+template <typename T, int RES>
+__device__ __forceinline__ void scalar_rhs(const pgpe_member& mb, const ScalarConsts<T>& k, const c2<T>& ps,
+                                           const c2<T>& pn, const c2<T>& pso, const c2<T>& pe, const c2<T>& pw,
+                                           const T n[2], T P, c2<T>& d, T dn[2]) {
+    constexpr int NF = ResFields<RES>::n;
+    // (psi[y+1] - 2 psi + psi[y-1]) / dy^2 + (psi[x+1] - 2 psi + psi[x-1]) / dx^2  (model.py:220-224)
+    c2<T> lap;
+    lap.x = ((pn.x - (T)2 * ps.x) + pso.x) * k.inv_dy2 + ((pe.x - (T)2 * ps.x) + pw.x) * k.inv_dx2;
+    lap.y = ((pn.y - (T)2 * ps.y) + pso.y) * k.inv_dy2 + ((pe.y - (T)2 * ps.y) + pw.y) * k.inv_dx2;
+    const T rho = ps.x * ps.x + ps.y * ps.y;
+    dn[0] = (T)0;
+    dn[1] = (T)0;
+    reservoir_rhs<T, RES>(mb, rho, P, n, dn);
+    const T nA = (NF > 0) ? n[0] : (T)0;
+    const T nI = (NF > 1) ? n[1] : (T)0;
+    const T eff = (k.gC * rho + k.gR * nA) + k.gI * nI;
+    const T gain = (k.R * nA - k.gammaC) * (T)0.5;
+    const T Xr = k.kin * lap.x + eff * ps.x;
+    const T Xi = k.kin * lap.y + eff * ps.y;
+    d.x = k.inv_hbar * Xi + gain * ps.x;
+    d.y = -(k.inv_hbar * Xr) + gain * ps.y;
+    if (k.eta_c != (T)0) {
+        const T s = (k.eta_c * nA) * k.hb2m;
+        d.x = d.x + s * lap.x;
+        d.y = d.y + s * lap.y;
+    }
+}
+
 // KIND: 0 = first stage (input is psi0), 1 = middle stage, 2 = final stage.
 template <typename T, int RES, int TH, int KIND>
 __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
@@ -114,11 +156,7 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
     const int lane = threadIdx.x & 63;
     const int gx = x0 + lane;
     if (gx >= nx) return;
-    const T inv_dx2 = (T)mb.inv_dx2, inv_dy2 = (T)mb.inv_dy2;
-    const T kin = (T)mb.kin, inv_hbar = (T)mb.inv_hbar;
-    const T gC = (T)mb.g_C, gR = (T)mb.g_R, gI = (T)mb.g_I, R = (T)mb.R, gammaC = (T)mb.gamma_C;
-    const T eta_c = (T)mb.eta;
-    const T hb2m = (T)mb.hbar_over_2m;
+    const ScalarConsts<T> kc(mb);
     const T coef = p.coef[m];
     const T* __restrict__ env = p.pump + mb.pump_off;
     uint64_t step = p.step;
@@ -129,16 +167,6 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
         if (gy >= ny) break;
         const int64_t idx = (int64_t)gy * nx + gx;
         const int c = (r + 1) * W + lane + 1;
-        const c2<T> ps = s_psi[c];
-        const c2<T> pn = s_psi[c + W];
-        const c2<T> pso = s_psi[c - W];
-        const c2<T> pe = s_psi[c + 1];
-        const c2<T> pw = s_psi[c - 1];
-        // (psi[y+1] - 2 psi + psi[y-1]) / dy^2 + (psi[x+1] - 2 psi + psi[x-1]) / dx^2  (model.py:220-224)
-        c2<T> lap;
-        lap.x = ((pn.x - (T)2 * ps.x) + pso.x) * inv_dy2 + ((pe.x - (T)2 * ps.x) + pw.x) * inv_dx2;
-        lap.y = ((pn.y - (T)2 * ps.y) + pso.y) * inv_dy2 + ((pe.y - (T)2 * ps.y) + pw.y) * inv_dx2;
-
         T n[2] = {(T)0, (T)0};
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
@@ -147,24 +175,9 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
             n[f] = v;
         }
         const T P = env[idx] * coef;
-        const T rho = ps.x * ps.x + ps.y * ps.y;
-        T dn[2] = {(T)0, (T)0};
-        reservoir_rhs<T, RES>(mb, rho, P, n, dn);
-
-        const T nA = (NF > 0) ? n[0] : (T)0;
-        const T nI = (NF > 1) ? n[1] : (T)0;
-        const T eff = (gC * rho + gR * nA) + gI * nI;
-        const T gain = (R * nA - gammaC) * (T)0.5;
-        const T Xr = kin * lap.x + eff * ps.x;
-        const T Xi = kin * lap.y + eff * ps.y;
         c2<T> d;
-        d.x = inv_hbar * Xi + gain * ps.x;
-        d.y = -(inv_hbar * Xr) + gain * ps.y;
-        if (eta_c != (T)0) {
-            const T s = (eta_c * nA) * hb2m;
-            d.x = d.x + s * lap.x;
-            d.y = d.y + s * lap.y;
-        }
+        T dn[2];
+        scalar_rhs<T, RES>(mb, kc, s_psi[c], s_psi[c + W], s_psi[c - W], s_psi[c + 1], s_psi[c - 1], n, P, d, dn);
 
         if constexpr (KIND != 2) {
             p.kout[off + idx] = d;
@@ -363,6 +376,215 @@ __global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T
     }
 }
 
+// ------------------------------------------------------------------ whole step
+// All four stages of one RK4 step in one launch (temporal blocking).  A
+// workgroup loads psi0, the reservoir and the pump of a 64 x TH output tile
+// plus a 4-point halo (72 x (TH + 8) points, periodic wrap), keeps the
+// per-point state in registers and only the stage input psi_s in LDS (double
+// buffered), and recomputes the halo: after stage s the outer s rings are
+// stale, so after four stages exactly the 64 x TH interior is exact.  HBM
+// sees psi0, n0 and the pump once (times the halo overhead) and the new state
+// once per step, instead of the 13 complex + 17 real arrays per point of the
+// four stage launches.  Each point runs the same operations in the same order
+// as the stage kernels (scalar_rhs; the final combination accumulates
+// ((k1 + 2 k2) + 2 k3) + k4 left to right), so the two paths agree bit for bit.
+template <typename T>
+struct Rk4StepParams {
+    const pgpe_member* table;
+    const int32_t* tile_map;
+    int num_tiles;
+    int noise;
+    int members;
+    T a_half, a_full, dt6;
+    const c2<T>* psi0;
+    c2<T>* psi_out;
+    const T* res0[2];
+    T* res_out[2];
+    const T* pump;
+    const T* coef;       // 4 x members: pump coefficient of stage s of member m at coef[s * members + m]
+    uint64_t step;
+    const uint64_t* step_base;
+};
+
+constexpr int kStepHalo = 4;
+
+template <typename T, int RES, int TH>
+__global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p) {
+    constexpr int NF = ResFields<RES>::n;
+    constexpr int EW = kTileW + 2 * kStepHalo;
+    constexpr int EH = TH + 2 * kStepHalo;
+    constexpr int NP = EW * EH;
+    constexpr int PPT = (NP + 255) / 256;
+    extern __shared__ unsigned char smem_raw[];
+    c2<T>* buf0 = reinterpret_cast<c2<T>*>(smem_raw);
+    c2<T>* buf1 = buf0 + NP;
+
+    const int tile = xcd_tile(blockIdx.x, p.num_tiles);
+    if (tile < 0) return;
+    const int m = p.tile_map[2 * tile];
+    const pgpe_member& mb = p.table[m];
+    const int nx = mb.nx, ny = mb.ny;
+    const int local = p.tile_map[2 * tile + 1];
+    const int x0 = (local % mb.tiles_x) * kTileW;
+    const int y0 = (local / mb.tiles_x) * TH;
+    const int64_t off = mb.off;
+    const ScalarConsts<T> kc(mb);
+    T coef[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) coef[s] = p.coef[s * p.members + m];
+
+    c2<T> z[PPT], acc[PPT];
+    T env[PPT], n0[PPT][2], ns[PPT][2], accn[PPT][2];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const int f = tid + 256 * j;
+        if (f < NP) {
+            const int ly = f / EW;
+            const int lx = f - ly * EW;
+            int gy = (y0 + ly - kStepHalo) % ny;
+            int gx = (x0 + lx - kStepHalo) % nx;
+            gy += gy < 0 ? ny : 0;
+            gx += gx < 0 ? nx : 0;
+            const int g = gy * nx + gx;
+            z[j] = p.psi0[off + g];
+            env[j] = p.pump[mb.pump_off + g];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) {
+                n0[j][q] = p.res0[q][off + g];
+                ns[j][q] = n0[j][q];
+            }
+            buf0[f] = z[j];
+        }
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const c2<T>* __restrict__ cur = (s & 1) ? buf1 : buf0;
+        c2<T>* __restrict__ nxt = (s & 1) ? buf0 : buf1;
+        const T a_next = s < 2 ? p.a_half : p.a_full;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const int f = tid + 256 * j;
+            if (f >= NP) continue;
+            // neighbours clamped into the buffer; border points are stale anyway
+            const c2<T> ps = cur[f];
+            const c2<T> pn = cur[f + EW < NP ? f + EW : f];
+            const c2<T> pso = cur[f >= EW ? f - EW : f];
+            const c2<T> pe = cur[f + 1 < NP ? f + 1 : f];
+            const c2<T> pw = cur[f >= 1 ? f - 1 : f];
+            const T P = env[j] * coef[s];
+            c2<T> d;
+            T dn[2];
+            scalar_rhs<T, RES>(mb, kc, ps, pn, pso, pe, pw, ns[j], P, d, dn);
+            if (s == 0) {
+                acc[j] = d;
+#pragma unroll
+                for (int q = 0; q < NF; ++q) accn[j][q] = dn[q];
+            } else if (s < 3) {
+                acc[j].x = acc[j].x + (T)2 * d.x;
+                acc[j].y = acc[j].y + (T)2 * d.y;
+#pragma unroll
+                for (int q = 0; q < NF; ++q) accn[j][q] = accn[j][q] + (T)2 * dn[q];
+            }
+            if (s < 3) {
+                c2<T> v;
+                v.x = z[j].x + a_next * d.x;
+                v.y = z[j].y + a_next * d.y;
+                nxt[f] = v;
+#pragma unroll
+                for (int q = 0; q < NF; ++q) ns[j][q] = n0[j][q] + a_next * dn[q];
+            } else {
+                const int ly = f / EW;
+                const int lx = f - ly * EW;
+                const int oy = y0 + ly - kStepHalo, ox = x0 + lx - kStepHalo;
+                if (lx < kStepHalo || lx >= kStepHalo + kTileW || ly < kStepHalo || ly >= kStepHalo + TH ||
+                    ox >= nx || oy >= ny)
+                    continue;
+                const int64_t idx = (int64_t)oy * nx + ox;
+                c2<T> out;
+                out.x = z[j].x + p.dt6 * (acc[j].x + d.x);
+                out.y = z[j].y + p.dt6 * (acc[j].y + d.y);
+                T n_new[2] = {(T)0, (T)0};
+#pragma unroll
+                for (int q = 0; q < NF; ++q) {
+                    T v = n0[j][q] + p.dt6 * (accn[j][q] + dn[q]);
+                    v = v > (T)0 ? v : (T)0;  // set_state clamp (single_reservoir.py:70-73)
+                    n_new[q] = v;
+                    p.res_out[q][off + idx] = v;
+                }
+                if (p.noise) {
+                    uint64_t step = p.step;
+                    if (p.step_base) step += *p.step_base;
+                    wigner_site<T>(out.x, out.y, mb.seed, (uint32_t)idx, step, 1u, (T)mb.noise_strength,
+                                   (T)mb.noise_gamma, (T)mb.noise_R, NF > 0 ? n_new[0] : (T)0,
+                                   (T)mb.noise_dt_over_4v);
+                }
+                p.psi_out[off + idx] = out;
+            }
+        }
+        if (s < 3) __syncthreads();
+    }
+}
+
+template <typename T, int RES, int TH>
+static int launch_step_th(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
+    constexpr int NP = (kTileW + 2 * kStepHalo) * (TH + 2 * kStepHalo);
+    const size_t shm = 2 * NP * sizeof(c2<T>);
+    if (shm > 160 * 1024) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: tile too tall for LDS at this precision");
+    auto kern = rk4_step_kernel<T, RES, TH>;
+    if (shm > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(rk4_step_kernel)");
+    }
+    hipLaunchKernelGGL(kern, dim3(xcd_grid(a->num_tiles)), dim3(256), shm, s, p);
+    PGPE_LAUNCHED("rk4_step_kernel");
+    return PGPE_OK;
+}
+
+template <typename T, int RES>
+static int launch_step_res(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
+    switch (a->tile_y) {
+        case 16: return launch_step_th<T, RES, 16>(a, p, s);
+        case 32: return launch_step_th<T, RES, 32>(a, p, s);
+        case 64:
+            if constexpr (sizeof(T) == 4) return launch_step_th<T, RES, 64>(a, p, s);
+            return fail(PGPE_ERR_ARG, "pgpe_rk4_step: tile_y 64 is single precision only");
+        default: return fail(PGPE_ERR_ARG, "pgpe_rk4_step: tile_y must be 16, 32 or 64");
+    }
+}
+
+template <typename T>
+static int launch_step_typed(const pgpe_rk4_args* a, hipStream_t s) {
+    Rk4StepParams<T> p{};
+    p.table = a->table;
+    p.tile_map = a->tile_map;
+    p.num_tiles = a->num_tiles;
+    p.noise = a->noise;
+    p.members = a->members;
+    p.a_half = (T)(0.5 * a->dt);
+    p.a_full = (T)(1.0 * a->dt);
+    p.dt6 = (T)(a->dt / 6.0);
+    p.psi0 = (const c2<T>*)a->psi0;
+    p.psi_out = (c2<T>*)a->psi_out;
+    for (int f = 0; f < 2; ++f) {
+        p.res0[f] = (const T*)a->res0[f];
+        p.res_out[f] = (T*)a->res_out[f];
+    }
+    p.pump = (const T*)a->pump;
+    p.coef = (const T*)a->coef;
+    p.step = a->step;
+    p.step_base = a->step_base;
+    switch (a->reservoir) {
+        case PGPE_RES_NONE: return launch_step_res<T, PGPE_RES_NONE>(a, p, s);
+        case PGPE_RES_SINGLE: return launch_step_res<T, PGPE_RES_SINGLE>(a, p, s);
+        case PGPE_RES_DOUBLE: return launch_step_res<T, PGPE_RES_DOUBLE>(a, p, s);
+        case PGPE_RES_QDOUBLE: return launch_step_res<T, PGPE_RES_QDOUBLE>(a, p, s);
+        default: return fail(PGPE_ERR_ARG, "pgpe_rk4_step: unknown reservoir kind");
+    }
+}
+
 template <typename T, int RES, int TH>
 static int launch_kind(const pgpe_rk4_args* a, const Rk4Params<T>& p, hipStream_t s) {
     const dim3 grid(xcd_grid(a->num_tiles)), block(256);
@@ -476,4 +698,22 @@ extern "C" int pgpe_rk4_stage(const pgpe_rk4_args* a, void* stream) {
     if (a->precision == 0) return launch_typed<float>(a, (hipStream_t)stream);
     if (a->precision == 1) return launch_typed<double>(a, (hipStream_t)stream);
     return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: precision must be 0 or 1");
+}
+
+extern "C" int pgpe_rk4_step(const pgpe_rk4_args* a, void* stream) {
+    if (!a) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: null args");
+    if (a->num_tiles <= 0 || a->members <= 0) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: empty member table");
+    if (a->components != 1) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: scalar fields only (components 1)");
+    if (!a->table || !a->tile_map || !a->psi0 || !a->psi_out || !a->pump || !a->coef)
+        return fail(PGPE_ERR_ARG, "pgpe_rk4_step: null table, field, pump or coefficient");
+    if (a->psi_out == a->psi0) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: psi_out must not alias psi0");
+    const int nf = a->reservoir == PGPE_RES_NONE ? 0 : (a->reservoir == PGPE_RES_SINGLE ? 1 : 2);
+    for (int f = 0; f < nf; ++f) {
+        if (!a->res0[f] || !a->res_out[f]) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: null reservoir field");
+        // the halo reads neighbouring tiles' reservoir, so the update cannot be in place
+        if (a->res_out[f] == a->res0[f]) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: res_out must not alias res0");
+    }
+    if (a->precision == 0) return launch_step_typed<float>(a, (hipStream_t)stream);
+    if (a->precision == 1) return launch_step_typed<double>(a, (hipStream_t)stream);
+    return fail(PGPE_ERR_ARG, "pgpe_rk4_step: precision must be 0 or 1");
 }

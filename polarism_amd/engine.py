@@ -1,8 +1,11 @@
 """RK4 launch engine over a packed member table.
 
-Owns the stage buffers of one batch (uniform or uneven grids) and issues the
-four fused ``pgpe_rk4_stage`` launches of a step.  Used by the
-``rk4-fdm-hip`` solver and by the packed sweep driver.
+Owns the buffers of one batch (uniform or uneven grids) and issues either the
+four fused ``pgpe_rk4_stage`` launches of a step or, with ``whole_step``, one
+``pgpe_rk4_step`` launch that runs all four stages on halo tiles (same
+result bit for bit; scalar fields; the reservoir then ping-pongs between two
+buffers like the field).  Used by the ``rk4-fdm-hip`` solver and by the
+packed sweep driver.
 """
 from __future__ import annotations
 
@@ -25,7 +28,7 @@ def complex_dtype(precision):
 
 class Rk4Engine:
     def __init__(self, table: MemberTable, pump: DeviceArray, precision: str, reservoir: str,
-                 stream: Stream | None = None):
+                 stream: Stream | None = None, whole_step: bool = False):
         self.table = table
         self.pump = pump
         self.precision = precision
@@ -40,10 +43,20 @@ class Rk4Engine:
         n = table.total_points * self.components
         nr = table.total_points * self.res_components
         c, r = self.cdtype, self.rdtype
+        self.whole_step = bool(whole_step)
+        if self.whole_step:
+            if self.components != 1:
+                raise ValueError("whole_step runs scalar fields only")
+            if table.tile_h not in (16, 32, 64) or (table.tile_h == 64 and precision != "single"):
+                raise ValueError("whole_step needs tile_h 16 or 32 (or 64 in single precision)")
+        elif table.tile_h not in (8, 16, 32):
+            raise ValueError("the stage kernels need tile_h 8, 16 or 32")
         self.psi = [DeviceArray.empty((n,), c), DeviceArray.empty((n,), c)]
-        self.k = [DeviceArray.empty((n,), c) for _ in range(3)]
-        self.kres = [[DeviceArray.empty((nr,), r) for _ in range(self.nres)] for _ in range(3)]
+        stage_bufs = 0 if self.whole_step else 3
+        self.k = [DeviceArray.empty((n,), c) for _ in range(stage_bufs)]
+        self.kres = [[DeviceArray.empty((nr,), r) for _ in range(self.nres)] for _ in range(stage_bufs)]
         self.res = [DeviceArray.zeros((nr,), r) for _ in range(self.nres)]
+        self._res_alt = [DeviceArray.zeros((nr,), r) for _ in range(self.nres)] if self.whole_step else None
         self.step_base = DeviceArray.zeros((1,), np.uint64)
         # active tile list: starts as every tile; the sweep driver may retire members
         self.tile_map = table.tile_map_dev
@@ -62,7 +75,9 @@ class Rk4Engine:
         self._graphs.clear()
 
     def launch_step(self, psi_in: DeviceArray, psi_out: DeviceArray, coef_ptrs, noise: bool, step: int,
-                    step_base_ptr=None, stages=(0, 1, 2, 3)) -> None:
+                    step_base_ptr=None, stages=(0, 1, 2, 3), res_in=None, res_out=None) -> None:
+        """One step from ``psi_in`` into ``psi_out``.  The stage path updates ``self.res``
+        in place; the whole-step path reads ``res_in`` and writes ``res_out``."""
         if self.num_tiles == 0:
             return
         tab = self.table
@@ -79,9 +94,20 @@ class Rk4Engine:
         a.tile_map = self.tile_map.ptr
         a.psi0 = psi_in.ptr
         a.pump = self.pump.ptr
+        lib = N.lib()
+        if self.whole_step:
+            for f in range(self.nres):
+                a.res0[f] = res_in[f].ptr
+                a.res_out[f] = res_out[f].ptr
+            a.psi_out = psi_out.ptr
+            a.coef = coef_ptrs[0]          # the 4 stage rows of this step are contiguous
+            a.noise = 1 if noise else 0
+            a.step = int(step)
+            a.step_base = step_base_ptr
+            N.check(lib.pgpe_rk4_step(C.byref(a), self.stream.handle), "pgpe_rk4_step")
+            return
         for f in range(self.nres):
             a.res0[f] = self.res[f].ptr
-        lib = N.lib()
         for s in stages:
             a.stage = s
             a.coef = coef_ptrs[s]
@@ -130,14 +156,17 @@ class Rk4Engine:
             if graphs and rem >= 2:
                 chunk = rem - rem % 2
                 p0, p1 = self.psi
-                key = (chunk, const, noise, self.num_tiles, self.tile_map.ptr, p0.ptr, coef.ptr)
+                key = (chunk, const, noise, self.num_tiles, self.tile_map.ptr, p0.ptr, coef.ptr,
+                       self.res[0].ptr if self.whole_step and self.nres else 0)
                 g = self._graphs.get(key)
                 if g is None:
+                    r0, r1 = self.res, self._res_alt
                     def body():
                         for k in range(chunk):
                             src, dst = (p0, p1) if k % 2 == 0 else (p1, p0)
+                            rsrc, rdst = (r0, r1) if k % 2 == 0 else (r1, r0)
                             self.launch_step(src, dst, self.coef_ptrs(coef, 0 if const else k), noise,
-                                             k, self.step_base.ptr)
+                                             k, self.step_base.ptr, res_in=rsrc, res_out=rdst)
                     self.stream.synchronize()
                     g = Graph.capture(self.stream, body)
                     if len(self._graphs) >= 8:
@@ -148,6 +177,8 @@ class Rk4Engine:
                 done += chunk
             else:
                 self.launch_step(self.psi[0], self.psi[1], self.coef_ptrs(coef, 0 if const else done), noise,
-                                 step0 + done, None)
+                                 step0 + done, None, res_in=self.res, res_out=self._res_alt)
                 self.psi.reverse()
+                if self.whole_step:
+                    self.res, self._res_alt = self._res_alt, self.res
                 done += 1

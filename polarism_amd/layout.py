@@ -43,8 +43,8 @@ class MemberSpec:
 
 class MemberTable:
     def __init__(self, specs: list[MemberSpec], tile_h: int = 16):
-        if tile_h not in (8, 16, 32):
-            raise ValueError("tile_h must be 8, 16 or 32")
+        if tile_h not in (8, 16, 32, 64):
+            raise ValueError("tile_h must be 8, 16, 32 or 64")
         self.specs = list(specs)
         self.tile_h = tile_h
         self.members = len(self.specs)

@@ -25,10 +25,35 @@ def gemm(A, B, C_out, M, N_, K, *, batch=1, sa=(None, None, 0), sb=(None, None, 
     """C[b](m, n) = alpha sum_k A[b](m, k) B[b](k, n) + beta C + bias + residual.
 
     ``sa = (sam, sak, sab)``, ``sb = (sbk, sbn, sbb)``, ``sc = (scm, scn, scb)`` in elements;
-    A and B are bf16 tensors, C is fp32 or bf16.
+    A and B are bf16 tensors, C is fp32 or bf16.  A long reduction (K >= 2^16) into a
+    small contiguous fp32 output (the weight gradients, K = pixels of the batch) is split
+    over K into partial products reduced in a fixed order, so it spreads over the chip.
     """
     if A.dtype != torch.bfloat16 or B.dtype != torch.bfloat16:
         raise TypeError("gemm operands must be bfloat16")
+    splits = _k_splits(M, N_, K, batch, sc, beta, bias, residual, gelu, C_out)
+    if splits > 1:
+        kc = K // splits
+        part = torch.empty((splits, M, N_), dtype=torch.float32, device=C_out.device)
+        _launch(A, B, part, M, N_, kc, splits, (sa[0], sa[1], kc * sa[1]), (sb[0], sb[1], kc * sb[0]),
+                (N_, 1, M * N_), alpha, 0.0, None, 0, None, 0, False)
+        colsum(part, C_out, splits, M * N_, M * N_, 1)
+        return
+    _launch(A, B, C_out, M, N_, K, batch, sa, sb, sc, alpha, beta, bias, bias_mode, residual, srb, gelu)
+
+
+def _k_splits(M, N_, K, batch, sc, beta, bias, residual, gelu, C_out):
+    tiles = ((M + 127) // 128) * ((N_ + 127) // 128) * batch
+    if (batch != 1 or K < (1 << 16) or tiles >= 64 or beta != 0.0 or bias is not None or residual is not None
+            or gelu or C_out.dtype != torch.float32 or sc[0] != N_ or sc[1] != 1):
+        return 1
+    splits = 1
+    while splits < 256 and K % (2 * splits) == 0 and (K // (2 * splits)) % 32 == 0 and K // (2 * splits) >= 2048:
+        splits *= 2
+    return splits
+
+
+def _launch(A, B, C_out, M, N_, K, batch, sa, sb, sc, alpha, beta, bias, bias_mode, residual, srb, gelu):
     a = N.GemmArgs()
     a.A, a.B, a.C = A.data_ptr(), B.data_ptr(), C_out.data_ptr()
     a.bias = _ptr(bias)

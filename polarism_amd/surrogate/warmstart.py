@@ -126,6 +126,7 @@ def train(x, y, epochs=40, batch=16, lr=2e-3, seed=0, log=print, cfg=None, time_
     hist = []
     g = torch.Generator(device="cpu").manual_seed(seed)
     t0 = time.perf_counter()
+    next_log = t0 + 30.0
     step = 0
     for ep in range(epochs):
         perm = torch.randperm(n, generator=g)
@@ -141,8 +142,12 @@ def train(x, y, epochs=40, batch=16, lr=2e-3, seed=0, log=print, cfg=None, time_
             torch.nn.utils.clip_grad_norm_(mod.parameters(), 1.0)
             opt.step()
             sched.step()
-            acc += float(loss)
+            acc += float(loss.detach())
             step += 1
+            if time.perf_counter() > next_log:
+                next_log = time.perf_counter() + 30.0
+                log(f"epoch {ep + 1} step {s + 1}/{steps_per_epoch} loss {acc / (s + 1):.4e} "
+                    f"({time.perf_counter() - t0:.0f} s)")
         hist.append(acc / steps_per_epoch)
         log(f"epoch {ep + 1}/{epochs} loss {hist[-1]:.4e} ({time.perf_counter() - t0:.0f} s)")
         if time_limit and time.perf_counter() - t0 > time_limit:

@@ -54,3 +54,53 @@ def test_gemm_epilogue_bias_residual_gelu_and_beta():
     gemm(A, B, Cb, M, N, K, sa=(K, 1, 0), sb=(N, 1, 0), sc=(N, 1, 0), bias=bias[:N] if False else None)
     torch.cuda.synchronize()
     assert torch.allclose(Cb.float(), A.float() @ B.float(), atol=3e-2, rtol=1e-2)
+
+
+@pytest.mark.parametrize("M,N", [(64, 64), (3, 128), (64, 6)])
+def test_split_k_weight_gradient_matches_torch(M, N):
+    """K = 2^17 pixels into a small fp32 output takes the split-K path (partials + fixed-order sum)."""
+    from polarism_amd.device import init
+    from polarism_amd.surrogate.kernels import _k_splits, gemm
+    init()
+    torch.manual_seed(11)
+    K = 1 << 17
+    dy = torch.randn(M, K, device="cuda").to(torch.bfloat16)
+    x = torch.randn(N, K, device="cuda").to(torch.bfloat16)
+    C = torch.empty(M, N, device="cuda")
+    assert _k_splits(M, N, K, 1, (N, 1, 0), 0.0, None, None, False, C) > 1
+    gemm(dy, x, C, M, N, K, sa=(K, 1, 0), sb=(1, K, 0), sc=(N, 1, 0))
+    torch.cuda.synchronize()
+    ref = dy.float() @ x.float().T
+    err = (C - ref).abs().max().item() / ref.abs().max().item()
+    assert err < 1e-4, err
+
+
+def test_gemm_unaligned_views_use_element_loads():
+    """Operands offset by one element cannot use 16-byte vectors; results must not change."""
+    from polarism_amd.device import init
+    from polarism_amd.surrogate.kernels import gemm
+    init()
+    torch.manual_seed(5)
+    M, N, K = 72, 90, 48
+    Abuf = torch.randn(M * K + 1, device="cuda").to(torch.bfloat16)
+    Bbuf = torch.randn(K * N + 1, device="cuda").to(torch.bfloat16)
+    A, B = Abuf[1:], Bbuf[1:]
+    C = torch.empty(M, N, device="cuda")
+    gemm(A, B, C, M, N, K, sa=(K, 1, 0), sb=(N, 1, 0), sc=(N, 1, 0))
+    torch.cuda.synchronize()
+    ref = A.float().reshape(M, K) @ B.float().reshape(K, N)
+    assert (C - ref).abs().max().item() < 1e-4 * ref.abs().max().item()
+
+
+@pytest.mark.parametrize("rows,cols", [(1 << 20, 64), (1000, 3), (7, 5)])
+def test_colsum_contiguous_columns(rows, cols):
+    from polarism_amd.device import init
+    from polarism_amd.surrogate.kernels import colsum
+    init()
+    torch.manual_seed(2)
+    x = torch.randn(cols, rows, device="cuda")
+    out = torch.empty(cols, device="cuda")
+    colsum(x, out, rows, cols, 1, rows)
+    torch.cuda.synchronize()
+    ref = x.double().sum(dim=1).float()
+    assert torch.allclose(out, ref, rtol=1e-4, atol=1e-3 * rows ** 0.5)

@@ -151,20 +151,24 @@ def test_adaptive_rejection_and_tick_schedule():
     assert np.max(np.abs(ctl.state.psi - _exact_uniform(cfg))) <= 50 * cfg.solver.rtol
 
 
-def test_adaptive_trials_match_oracle_sequence():
+@pytest.mark.parametrize("chunk", [1, 4])
+def test_adaptive_trials_match_oracle_sequence(chunk):
+    """Each advance of ``chunk`` ticks lands on its own target (abstract_solver.py:86-96)."""
     cfg = quickstart_config(solver__method="ifrk-fft-hip", solver__tableau="dp5", grid__nx=32, grid__ny=32,
                             grid__lx=30.0, grid__ly=30.0, solver__dt=0.5, solver__total_time=2.0,
                             solver__adaptive=True, solver__rtol=1e-6, solver__atol=1e-9, laser__P0=0.3,
                             laser__sigma_space=6.0, noise__stochastic="none")
     ctl = _ctl(cfg)
-    ctl.run()
+    for _ in range(4 // chunk):
+        ctl.solver.advance(ctl.model, ctl.state, chunk)
+    assert ctl.state.step == 4 and ctl.state.t == ctl.state.t0 + 2.0
     g = O.Grid(32, 32, 30.0, 30.0)
     p = oracle_physics(cfg)
     psi = O.initial_psi(g.shape, cfg.noise.initial_amplitude, cfg.noise.seed).reshape(1, 1, 32, 32)
     res = (np.zeros((1, 1, 32, 32)),)
     env = O.gaussian_profile(g, 6.0)
     prob = O.LawsonProblem("single", p, g, tableau="dp5", adaptive=True, rtol=1e-6, atol=1e-9)
-    psi, res, ctl_o = O.lawson_advance_adaptive(prob, psi, res, lambda t: env * 0.3, 0.0, 0.5, 4)
+    psi, res, ctl_o = O.lawson_advance_adaptive(prob, psi, res, lambda t: env * 0.3, 0.0, 0.5, 4, chunk=chunk)
     sc = ctl.solver.step_controller
     assert (sc.accepted, sc.rejected) == (ctl_o.accepted, ctl_o.rejected)
     assert O.rel_l2(ctl.state.field.get(), psi) <= 1e-10

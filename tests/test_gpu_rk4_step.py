@@ -1,0 +1,87 @@
+"""Whole-step RK4 kernel (pgpe_rk4_step) against the four stage launches, bit for bit."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(64, 64), (96, 80), (200, 72), (128, 128), (8, 40)]
+
+
+def _engines(sizes, reservoir, precision, tile_h, noise_strength=0.0):
+    from polarism_amd.config import PhysicsConstants
+    from polarism_amd.device import DeviceArray, init
+    from polarism_amd.engine import Rk4Engine, real_dtype
+    from polarism_amd.layout import MemberSpec, MemberTable
+    init()
+    p = PhysicsConstants()
+    p.g_I = 0.0005
+    p.kinetic_relaxation_eta = 0.02
+    rng = np.random.default_rng(7)
+    specs, envs, off = [], [], 0
+    for i, (nx, ny) in enumerate(sizes):
+        specs.append(MemberSpec(nx=nx, ny=ny, dx=0.5, dy=0.6, physics=p, reservoir=reservoir, pump_off=off,
+                                seed=31 + i, noise_strength=noise_strength, dt=0.05))
+        envs.append(rng.uniform(0.0, 1.0, nx * ny))
+        off += nx * ny
+    r = real_dtype(precision)
+    pump = DeviceArray.from_numpy(np.concatenate(envs).astype(r))
+    coef = DeviceArray.from_numpy(np.tile(rng.uniform(0.02, 0.08, len(sizes)), (4, 1)).astype(r))
+    stage_tab = MemberTable(specs, 16 if tile_h == 64 else tile_h)
+    step_tab = MemberTable(specs, tile_h)
+    e_stage = Rk4Engine(stage_tab, pump, precision, reservoir)
+    e_step = Rk4Engine(step_tab, pump, precision, reservoir, whole_step=True)
+    total = stage_tab.total_points
+    psi = (0.3 * (rng.standard_normal(total) + 1j * rng.standard_normal(total))).astype(e_stage.cdtype)
+    res = [rng.uniform(0.0, 3.0, total).astype(r) for _ in range(e_stage.nres)]
+    for e in (e_stage, e_step):
+        e.set_dt(0.05)
+        e.psi[0].upload(psi)
+        for f in range(e.nres):
+            e.res[f].upload(res[f])
+    return e_stage, e_step, coef
+
+
+def _compare(e_stage, e_step):
+    e_stage.stream.synchronize()
+    e_step.stream.synchronize()
+    a, b = e_stage.psi[0].get(), e_step.psi[0].get()
+    assert np.all(np.isfinite(a))
+    np.testing.assert_array_equal(a, b)
+    for f in range(e_stage.nres):
+        np.testing.assert_array_equal(e_stage.res[f].get(), e_step.res[f].get())
+
+
+@pytest.mark.parametrize("reservoir", ["none", "single", "double", "quadratic-double"])
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_whole_step_equals_stage_launches_bitwise(reservoir, precision):
+    e_stage, e_step, coef = _engines(SIZES, reservoir, precision, 16)
+    for e in (e_stage, e_step):
+        e.run_steps(5, coef, True, False, 0, graphs=False)
+    _compare(e_stage, e_step)
+
+
+@pytest.mark.parametrize("tile_h,precision", [(32, "single"), (64, "single"), (32, "double")])
+def test_whole_step_tile_heights_and_graph_chunks(tile_h, precision):
+    """Graph-captured chunks (even) plus a trailing single step, with Wigner noise fused in."""
+    e_stage, e_step, coef = _engines(SIZES, "single", precision, tile_h, noise_strength=1.0)
+    for e in (e_stage, e_step):
+        e.run_steps(7, coef, True, True, 3, graphs=True)
+    _compare(e_stage, e_step)
+
+
+def test_whole_step_with_retired_members():
+    """A tile map covering a subset of members leaves the others untouched."""
+    e_stage, e_step, coef = _engines(SIZES, "double", "single", 32)
+    keep = [0, 2, 4]
+    for e in (e_stage, e_step):
+        e.set_active_tiles(e.table.tiles_of(np.asarray(keep)))
+        e.run_steps(4, coef, True, False, 0, graphs=True)
+    _compare(e_stage, e_step)
+
+
+def test_whole_step_rejects_aliased_reservoir():
+    from polarism_amd import _native as N
+    e_stage, e_step, coef = _engines(SIZES[:1], "single", "single", 16)
+    with pytest.raises(RuntimeError, match="alias"):
+        e_step.launch_step(e_step.psi[0], e_step.psi[1], e_step.coef_ptrs(coef, 0), False, 0,
+                           res_in=e_step.res, res_out=e_step.res)
