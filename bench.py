@@ -20,7 +20,19 @@ Roofline leg: the middle RK4 stage kernel (FP32, single reservoir) on one
 8192 x 8192 member (537 MB per complex field, beyond the 256 MB Infinity
 Cache), timed with HIP events on the solver stream.  Algorithmic bytes per
 point: psi0, k_prev (2 x 8 B), n0, kn_prev, P (3 x 4 B) in; k (8 B), kn (4 B)
-out = 40 B.
+out = 40 B.  The whole-step kernel (all four stages on halo tiles, one
+launch per step) is timed on the same member; its effective rate uses the
+per-stage traffic model (172 B per point-step), its own algorithmic bytes
+are psi0, n0, P in and psi1, n1 out = 28 B per point-step.
+
+Extra legs (reported beside the headline, not part of value):
+  uneven: 256 members with grids drawn from {64..512}^2, double reservoir,
+          complex64, dt 1e-3 ps, 2000 steps in 1000-step graph chunks
+          (grid-point-stage updates/s).
+  fft:    split-step FFT (CPU operator order, 4 transforms per step) and the
+          K-outer order (2 per step), 512 x 512, single reservoir, Wigner
+          noise seed 1234, dt 5e-3 ps, complex64 (steps/s and FFT GB/s with
+          2 x 8 B per point per transform).
 
 cpu_baseline: the NumPy oracle's rk4-fdm step (complex64, one 256 x 256
 member) timed on one host core for a bounded number of steps; its
@@ -44,7 +56,7 @@ SWEEP = dict(members=256, n=256, L=128.0, dt=0.05, check_every=1.0, rtol=1e-6, m
 WEIGHTS = os.path.join(ROOT, "polarism_amd", "surrogate", "weights", "fno_steady_256.npz")
 
 
-def run_sweep(members, tile_h=8):
+def run_sweep(members, tile_h="auto"):
     from polarism_amd.sweep import PackedSweep
     sw = PackedSweep(members, reservoir="single", precision="single", dt=SWEEP["dt"],
                      check_every=SWEEP["check_every"], rtol=SWEEP["rtol"], max_time=SWEEP["max_time"],
@@ -63,39 +75,181 @@ def summarise(res, extra_s=0.0):
             "effective_GBps": round(res.point_steps * 172.0 / res.wall_s / 1e9, 1)}
 
 
-def roofline_leg(n=8192, tile_h=16, reps=20):
-    """Average duration of the FP32 middle-stage kernel on an n x n member."""
+def _member_engine(n, tile_h, whole_step, reservoir="single"):
     from polarism_amd.config import PhysicsConstants
-    from polarism_amd.device import DeviceArray, Event
+    from polarism_amd.device import DeviceArray
     from polarism_amd.engine import Rk4Engine
     from polarism_amd.layout import MemberSpec, MemberTable
     p = PhysicsConstants()
-    tab = MemberTable([MemberSpec(n, n, 0.5, 0.5, p, reservoir="single", dt=0.05)], tile_h)
+    tab = MemberTable([MemberSpec(n, n, 0.5, 0.5, p, reservoir=reservoir, dt=0.05)], tile_h)
     pump = DeviceArray.from_numpy(np.full(n * n, 0.01, dtype=np.float32))
-    e = Rk4Engine(tab, pump, "single", "single")
+    e = Rk4Engine(tab, pump, "single", reservoir, whole_step=whole_step)
     e.set_dt(0.05)
     rng = np.random.default_rng(0)
     e.psi[0].upload((1e-3 * (rng.standard_normal(n * n) + 1j * rng.standard_normal(n * n))).astype(np.complex64))
     for buf in e.k + [e.psi[1]] + [x for row in e.kres for x in row]:
         buf.fill_zero(e.stream)
     coef = DeviceArray.from_numpy(np.ones(4, dtype=np.float32))
-    ptrs = e.coef_ptrs(coef, 0)
-    e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0)
+    return e, coef
+
+
+def _time_launches(e, launch, reps):
+    from polarism_amd.device import Event
+    launch()
     e.stream.synchronize()
     start, stop = Event(), Event()
     total = 0.0
     for _ in range(reps):
         start.record(e.stream)
-        e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0, stages=(1,))
+        launch()
         stop.record(e.stream)
+        e.stream.synchronize()
         total += start.elapsed_ms(stop)
-    avg_ms = total / reps
+    return total / reps
+
+
+def roofline_leg(n=8192, reps=20):
+    """Average durations of the FP32 middle-stage kernel (fastest tile height) and of the whole-step kernel."""
+    stage = {}
+    for th in (16, 32):
+        e, coef = _member_engine(n, th, False)
+        ptrs = e.coef_ptrs(coef, 0)
+        e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0)
+        stage[th] = _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0, stages=(1,)), reps)
+        del e
+    tile_h = min(stage, key=stage.get)
+    avg_ms = stage[tile_h]
     bytes_per_launch = 40.0 * n * n
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": f"rk4_stage_kernel<float, SINGLE, {tile_h}, middle>", "grid": f"{n}x{n}",
-            "avg_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+            "avg_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch),
+            "avg_us_by_tile_h": {str(k): round(v * 1e3, 2) for k, v in stage.items()}}
+    del e
+    ws = {}
+    for th in (32, 64):
+        e, coef = _member_engine(n, th, True)
+        ptrs = e.coef_ptrs(coef, 0)
+        r0, r1 = e.res, e._res_alt
+        ms = _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0, res_in=r0, res_out=r1),
+                            reps)
+        pts = float(n) * n
+        ws[f"tile_h_{th}"] = {"avg_us": round(ms * 1e3, 2),
+                              "point_steps_per_s": round(pts / (ms * 1e-3), 1),
+                              "effective_GBps_172B_model": round(172.0 * pts / (ms * 1e-3) / 1e9, 1),
+                              "algorithmic_GBps_28B": round(28.0 * pts / (ms * 1e-3) / 1e9, 1)}
+        del e
+    stage_step_ms = None
+    e, coef = _member_engine(n, tile_h, False)
+    ptrs = e.coef_ptrs(coef, 0)
+    stage_step_ms = _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0), reps)
+    del e
+    roof["whole_step_kernel"] = ws
+    roof["stage_path_step_us"] = round(stage_step_ms * 1e3, 2)
+    roof["stage_path_effective_GBps"] = round(172.0 * n * n / (stage_step_ms * 1e-3) / 1e9, 1)
+    return roof
+
+
+def copy_leg(nbytes=1 << 30, reps=10):
+    """Device-to-device copy bandwidth (read + write bytes), the measured ceiling beside the 8 TB/s peak."""
+    from polarism_amd.device import DeviceArray, Event, Stream
+    s = Stream()
+    a = DeviceArray.zeros((nbytes // 4,), np.float32)
+    b = DeviceArray.zeros((nbytes // 4,), np.float32)
+    b.copy_from(a, s)
+    s.synchronize()
+    start, stop = Event(), Event()
+    start.record(s)
+    for _ in range(reps):
+        b.copy_from(a, s)
+    stop.record(s)
+    s.synchronize()
+    ms = start.elapsed_ms(stop) / reps
+    return {"bytes_per_copy": nbytes, "avg_us": round(ms * 1e3, 2),
+            "GBps_read_plus_write": round(2.0 * nbytes / (ms * 1e-3) / 1e9, 1)}
+
+
+def uneven_leg(steps=2000, seed=4):
+    """256 members of uneven grids, double reservoir, complex64, dt 1e-3 ps, 1000-step graph chunks."""
+    from polarism_amd.config import PhysicsConstants
+    from polarism_amd.device import DeviceArray, Event
+    from polarism_amd.engine import Rk4Engine
+    from polarism_amd.layout import MemberSpec, MemberTable
+    from polarism_amd.pumps import threshold
+    from polarism_amd.autotune import tune_rk4
+    p = PhysicsConstants()
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([64, 96, 128, 192, 256, 384, 512], size=256)
+    specs, envs, off = [], [], 0
+    pth = threshold(p, "double")
+    for i, nn in enumerate(sizes):
+        nn = int(nn)
+        specs.append(MemberSpec(nn, nn, 0.5, 0.5, p, reservoir="double", pump_off=off, seed=i, dt=1e-3))
+        x = (np.arange(nn) - (nn - 1) / 2) * 0.5
+        sig = rng.uniform(8, 25)
+        envs.append(np.exp(-0.5 * (x[None, :] ** 2 + x[:, None] ** 2) / sig ** 2).ravel())
+        off += nn * nn
+    choice = tune_rk4(specs, "single", "double")
+    tab = MemberTable(specs, choice.tile_h)
+    pump = DeviceArray.from_numpy(np.concatenate(envs).astype(np.float32))
+    coef = DeviceArray.from_numpy(np.tile(rng.uniform(0.5, 3.0, 256) * pth, (4, 1)).astype(np.float32))
+    e = Rk4Engine(tab, pump, "single", "double", whole_step=choice.whole_step)
+    e.set_dt(1e-3)
+    e.psi[0].upload((1e-3 * (rng.standard_normal(off) + 1j * rng.standard_normal(off))).astype(np.complex64))
+    e.run_steps(1000, coef, True, False, 0)
+    e.stream.synchronize()
+    start, stop = Event(), Event()
+    start.record(e.stream)
+    done = 0
+    while done < steps:
+        e.run_steps(1000, coef, True, False, done)
+        done += 1000
+    stop.record(e.stream)
+    e.stream.synchronize()
+    ms = start.elapsed_ms(stop)
+    pt_steps = float(off) * steps
+    return {"members": 256, "points": int(off), "steps": steps, "form": choice.form, "tile_h": choice.tile_h,
+            "wall_ms": round(ms, 1), "point_steps_per_s": round(pt_steps / (ms * 1e-3), 1),
+            "point_stage_updates_per_s": round(4 * pt_steps / (ms * 1e-3), 1),
+            "effective_GBps_224B_model": round(224.0 * pt_steps / (ms * 1e-3) / 1e9, 1)}
+
+
+def fft_leg(steps=1000):
+    """Split-step FFT at 512^2 (complex64, Wigner noise), both operator orders."""
+    from polarism_amd import Config, SimulationController
+    from polarism_amd.config import GridParameters
+    from polarism_amd.device import Event
+    out = {}
+    for method, transforms in (("split-step-fft-hip", 4), ("split-step-fft-kouter-hip", 2)):
+        cfg = Config()
+        cfg.grid = GridParameters(nx=512, ny=512, lx=200.0, ly=200.0)
+        cfg.laser.laser_type = "continuous-gaussian"
+        cfg.laser.sigma_space = 20.0
+        cfg.laser.P0 = 2 * 0.02075
+        cfg.reservoir.reservoir_type = "single"
+        cfg.solver.method = method
+        cfg.solver.dt = 5e-3
+        cfg.solver.total_time = steps * 5e-3
+        cfg.solver.precision = "single"
+        cfg.noise.seed = 1234
+        cfg.noise.stochastic = "wigner"
+        ctl = SimulationController(cfg)
+        sol = ctl.solver
+        sol.advance(ctl.model, ctl.state, 100)
+        stream = sol.stream
+        stream.synchronize()
+        start, stop = Event(), Event()
+        start.record(stream)
+        sol.advance(ctl.model, ctl.state, steps)
+        stop.record(stream)
+        stream.synchronize()
+        ms = start.elapsed_ms(stop)
+        pts = 512 * 512
+        out[method] = {"steps_per_s": round(steps / (ms * 1e-3), 1), "transforms_per_step": transforms,
+                       "fft_GBps": round(transforms * 16.0 * pts * steps / (ms * 1e-3) / 1e9, 1),
+                       "step_us": round(ms * 1e3 / steps, 2)}
+    return out
 
 
 def cpu_baseline(point_steps, converged, steps=300):
@@ -139,7 +293,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--members", type=int, default=SWEEP["members"])
-    ap.add_argument("--legs", default="sweep,roofline,cpu")
+    ap.add_argument("--legs", default="sweep,copy,roofline,uneven,fft,cpu")
     ap.add_argument("--weights", default=WEIGHTS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -184,8 +338,14 @@ def main():
             value = warm["members_per_s"]
         result.update(value=value, ms_per_step=round(wall / args.steps * 1e3, 1))
         result["sweep"] = sweep
+    if "copy" in legs:
+        result["device_copy"] = copy_leg()
     if "roofline" in legs:
         result["roofline"] = roofline_leg()
+    if "uneven" in legs:
+        result["uneven_grid_sweep"] = uneven_leg()
+    if "fft" in legs:
+        result["split_step_fft"] = fft_leg()
     result["config"] = {"workload": "steady-state pump sweep, 256 members of 256x256, rk4-fdm-hip complex64, "
                                     "single reservoir, dt 0.05 ps; warm start (FNO) vs plain cold start",
                         "members": len(members), "grid": "256x256", "lx_um": SWEEP["L"], "dt_ps": SWEEP["dt"],

@@ -102,7 +102,18 @@ typedef struct pgpe_member {
     double noise_gamma, noise_R, noise_strength, noise_dt_over_4v;
     uint64_t seed;          /* Philox key of this member */
     double pump_w_plus, pump_w_minus; /* spinor pump weights (1 +/- c)/2 */
+    int64_t pot_off;        /* offset of its potential in the packed complex potential array (points) */
+    int32_t laplacian;      /* pgpe_laplacian_kind */
+    int32_t boundary;       /* pgpe_boundary_kind */
 } pgpe_member;
+
+/* Finite-difference Laplacian of the rk4 kernels: the periodic or mirrored 5-point
+ * stencil (model.py:210-248) or the isotropic 9-point stencil of rk4-cuda
+ * (rk4_cuda_solver.py:719-770; needs dx == dy). */
+enum pgpe_laplacian_kind { PGPE_LAP_FIVE = 0, PGPE_LAP_NINE = 1 };
+/* Grid boundary of the rk4 kernels: periodic wrap, or the closed interval with
+ * mirrored (Neumann) neighbours psi[-1] = psi[1] (model.py:227-248). */
+enum pgpe_boundary_kind { PGPE_BC_PERIODIC = 0, PGPE_BC_CLOSED = 1 };
 
 /* ------------------------------------------------------------------ */
 /* Philox4x32-10 and the truncated-Wigner increment                    */
@@ -157,6 +168,8 @@ typedef struct pgpe_rk4_args {
     const void* coef;                /* real, members entries: pump coefficient of this stage */
     uint64_t step;                   /* noise counter step = (*step_base if set) + step */
     const uint64_t* step_base;       /* device or NULL */
+    const void* potential;           /* complex V packed per member at pot_off, or NULL for V = 0;
+                                        enters as (V + gC|psi|^2 + gR nA + gI nI) psi (model.py:89-127) */
 } pgpe_rk4_args;
 
 int pgpe_rk4_stage(const pgpe_rk4_args* args, void* stream);

@@ -178,17 +178,28 @@ def initial_psi(shape, eps, seed, precision="double"):
 
 @dataclass
 class Grid:
-    """Periodic 2-D grid (grid/periodic_grid.py:35-69); x is the fastest axis."""
+    """2-D grid, x the fastest axis.  Periodic (grid/periodic_grid.py:35-69): dx = lx/nx,
+    centred X = (i - (n-1)/2) dx.  Closed interval (grid/closed_interval_grid.py:35-80):
+    dx = lx/(nx-1), X = linspace(-lx/2, lx/2, nx)."""
     nx: int
     ny: int
     lx: float
     ly: float
+    boundary: str = "periodic"
 
     def __post_init__(self):
-        self.dx = self.lx / self.nx
-        self.dy = self.ly / self.ny
-        x = (np.arange(self.nx, dtype=np.float64) - (self.nx - 1) * 0.5) * self.dx
-        y = (np.arange(self.ny, dtype=np.float64) - (self.ny - 1) * 0.5) * self.dy
+        if self.boundary == "periodic":
+            self.dx = self.lx / self.nx
+            self.dy = self.ly / self.ny
+            x = (np.arange(self.nx, dtype=np.float64) - (self.nx - 1) * 0.5) * self.dx
+            y = (np.arange(self.ny, dtype=np.float64) - (self.ny - 1) * 0.5) * self.dy
+        elif self.boundary == "closed-interval":
+            self.dx = self.lx / (self.nx - 1)
+            self.dy = self.ly / (self.ny - 1)
+            x = np.linspace(-self.lx / 2, self.lx / 2, self.nx, dtype=np.float64)
+            y = np.linspace(-self.ly / 2, self.ly / 2, self.ny, dtype=np.float64)
+        else:
+            raise ValueError(self.boundary)
         self.X, self.Y = np.meshgrid(x, y, indexing="xy")
         self.kx = 2 * np.pi * np.fft.fftfreq(self.nx, d=self.dx)
         self.ky = 2 * np.pi * np.fft.fftfreq(self.ny, d=self.dy)
@@ -304,10 +315,74 @@ def reservoir_rk2(kind, p, dt, psi, pump, state):
 # ---------------------------------------------------------------------------
 # Finite-difference RHS (model.py:89-248)
 # ---------------------------------------------------------------------------
-def laplacian(f, dx, dy):
-    """Periodic 5-point Laplacian (model.py:214-225); x is the last axis."""
-    return ((np.roll(f, -1, axis=-2) - 2 * f + np.roll(f, 1, axis=-2)) / (dy ** 2)
-            + (np.roll(f, -1, axis=-1) - 2 * f + np.roll(f, 1, axis=-1)) / (dx ** 2))
+def laplacian(f, dx, dy, boundary="periodic", kind="five-point"):
+    """Finite-difference Laplacian; x is the last axis.
+
+    five-point, periodic: model.py:214-225.  five-point, closed interval
+    (mirrored Neumann edges written as 2 (f[1] - f[0]) / h^2): model.py:227-248.
+    isotropic-9pt (dx == dy): rk4_cuda_solver.py:719-770, periodic wrap or the
+    closed interval's mirrored neighbour indices.
+    """
+    if kind == "isotropic-9pt":
+        return _laplacian9(f, dx, boundary)
+    if boundary == "periodic":
+        return ((np.roll(f, -1, axis=-2) - 2 * f + np.roll(f, 1, axis=-2)) / (dy ** 2)
+                + (np.roll(f, -1, axis=-1) - 2 * f + np.roll(f, 1, axis=-1)) / (dx ** 2))
+    if boundary != "closed-interval":
+        raise ValueError(boundary)
+    ny, nx = f.shape[-2:]
+    out = np.zeros_like(f)
+    c = (slice(None),) * (f.ndim - 2)
+    if ny > 2 and nx > 2:
+        out[c + (slice(1, -1), slice(1, -1))] = (
+            (f[c + (slice(2, None), slice(1, -1))] - 2 * f[c + (slice(1, -1), slice(1, -1))]
+             + f[c + (slice(None, -2), slice(1, -1))]) / dy ** 2
+            + (f[c + (slice(1, -1), slice(2, None))] - 2 * f[c + (slice(1, -1), slice(1, -1))]
+               + f[c + (slice(1, -1), slice(None, -2))]) / dx ** 2)
+    if nx > 2:
+        for row, nb in ((0, 1), (-1, -2)):
+            out[c + (row, slice(1, -1))] = (
+                2 * (f[c + (nb, slice(1, -1))] - f[c + (row, slice(1, -1))]) / dy ** 2
+                + (f[c + (row, slice(2, None))] - 2 * f[c + (row, slice(1, -1))]
+                   + f[c + (row, slice(None, -2))]) / dx ** 2)
+    if ny > 2:
+        for col, nb in ((0, 1), (-1, -2)):
+            out[c + (slice(1, -1), col)] = (
+                (f[c + (slice(2, None), col)] - 2 * f[c + (slice(1, -1), col)]
+                 + f[c + (slice(None, -2), col)]) / dy ** 2
+                + 2 * (f[c + (slice(1, -1), nb)] - f[c + (slice(1, -1), col)]) / dx ** 2)
+    for row, nby in ((0, 1), (-1, -2)):
+        for col, nbx in ((0, 1), (-1, -2)):
+            out[c + (row, col)] = (2 * (f[c + (nby, col)] - f[c + (row, col)]) / dy ** 2
+                                   + 2 * (f[c + (row, nbx)] - f[c + (row, col)]) / dx ** 2)
+    return out
+
+
+def _neighbour(n, boundary):
+    """Index arrays of the -1 / +1 neighbours along an axis of length n."""
+    i = np.arange(n)
+    if boundary == "periodic":
+        return (i - 1) % n, (i + 1) % n
+    lo = np.where(i > 0, i - 1, 1 if n > 1 else 0)
+    hi = np.where(i < n - 1, i + 1, n - 2 if n > 1 else 0)
+    return lo, hi
+
+
+def _laplacian9(f, h, boundary):
+    """(4 (dn + up + rt + lt) + ul + ur + dl + dr - 20 f) / (6 h^2) in the kernel's order."""
+    ny, nx = f.shape[-2:]
+    up, dn = _neighbour(ny, boundary)
+    lt, rt = _neighbour(nx, boundary)
+    rows = lambda idx: np.take(f, idx, axis=-2)
+    cols = lambda a, idx: np.take(a, idx, axis=-1)
+    fdn, fup = rows(dn), rows(up)
+    inv_6h2 = (1.0 / (h * h)) / 6.0
+    card = ((fdn + fup) + cols(f, rt)) + cols(f, lt)
+    acc = 4.0 * card + cols(fup, lt)
+    acc = acc + cols(fup, rt)
+    acc = acc + cols(fdn, lt)
+    acc = acc + cols(fdn, rt)
+    return (acc - 20.0 * f) * inv_6h2
 
 
 def te_tm_energy(p, field, dx, dy):
@@ -326,14 +401,14 @@ def te_tm_energy(p, field, dx, dy):
 
 
 def fdm_rhs(kind, p, grid, psi, res, pump, potential=0.0, pump_weights=None,
-            reservoir_components=1):
+            reservoir_components=1, laplacian_kind="five-point"):
     """Scalar or spinor FDM right-hand side; returns (dpsi, dres).
 
     Scalar: model.py:89-127.  Spinor: model.py:129-173 (Zeeman +/-Z/2,
     alpha_2 cross density, TE-TM 9-point stencil).
     """
     spinor = psi.shape[1] == 2
-    lap = laplacian(psi, grid.dx, grid.dy)
+    lap = laplacian(psi, grid.dx, grid.dy, getattr(grid, "boundary", "periodic"), laplacian_kind)
     if spinor:
         if reservoir_components == 1:
             rho_res = (psi.real * psi.real + psi.imag * psi.imag).sum(axis=1, keepdims=True)

@@ -14,6 +14,10 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libpgpe_hip.so"
 
 PGPE_RES_NONE, PGPE_RES_SINGLE, PGPE_RES_DOUBLE, PGPE_RES_QDOUBLE = 0, 1, 2, 3
+PGPE_LAP_FIVE, PGPE_LAP_NINE = 0, 1
+LAPLACIAN_KIND = {"five-point": PGPE_LAP_FIVE, "isotropic-9pt": PGPE_LAP_NINE}
+PGPE_BC_PERIODIC, PGPE_BC_CLOSED = 0, 1
+BOUNDARY_KIND = {"periodic": PGPE_BC_PERIODIC, "closed-interval": PGPE_BC_CLOSED}
 RESERVOIR_KIND = {"none": PGPE_RES_NONE, "single": PGPE_RES_SINGLE, "double": PGPE_RES_DOUBLE,
                   "quadratic-double": PGPE_RES_QDOUBLE}
 RESERVOIR_FIELDS = {"none": 0, "single": 1, "double": 2, "quadratic-double": 2}
@@ -40,6 +44,7 @@ class Member(C.Structure):
         ("noise_dt_over_4v", C.c_double),
         ("seed", C.c_uint64),
         ("pump_w_plus", C.c_double), ("pump_w_minus", C.c_double),
+        ("pot_off", C.c_int64), ("laplacian", C.c_int32), ("boundary", C.c_int32),
     ]
 
 
@@ -57,6 +62,7 @@ class Rk4Args(C.Structure):
         ("kres_out", vp * 2), ("res_out", vp * 2),
         ("pump", vp), ("coef", vp),
         ("step", C.c_uint64), ("step_base", vp),
+        ("potential", vp),
     ]
 
 

@@ -46,6 +46,7 @@ struct Rk4Params {
     const T* coef;
     uint64_t step;
     const uint64_t* step_base;
+    const c2<T>* potential;   // complex V packed per member at pot_off, or null
 };
 
 // Reservoir right-hand side with the condensate frozen.  Rate sums are
@@ -82,29 +83,77 @@ struct ScalarConsts {
           hb2m((T)mb.hbar_over_2m) {}
 };
 
+// Neighbour index along one axis: periodic wrap, or the closed interval's
+// mirror psi[-k] = psi[k], psi[n-1+k] = psi[n-1-k] (model.py:227-248), which
+// for the 5-point stencil gives the reference's 2 (psi[1] - psi[0]) / h^2 edge
+// rows and stays exact for deeper halos (the mirrored field evolves as the
+// mirror image).
+__device__ __forceinline__ int map_index(int g, int n, int boundary) {
+    if (boundary == PGPE_BC_CLOSED) {
+        if (n == 1) return 0;
+        const int period = 2 * (n - 1);
+        int r = g % period;
+        r += r < 0 ? period : 0;
+        return r < n ? r : period - r;
+    }
+    if (g >= 0 && g < n) return g;
+    if (g < 0 && g >= -n) return g + n;
+    int r = g % n;
+    return r < 0 ? r + n : r;
+}
+
+// 5-point Laplacian (psi[y+1] - 2 psi + psi[y-1]) / dy^2 + (psi[x+1] - 2 psi + psi[x-1]) / dx^2
+// (model.py:220-224) from the point and its north / south / east / west neighbours.
+template <typename T>
+__device__ __forceinline__ c2<T> lap5(const c2<T>& ps, const c2<T>& pn, const c2<T>& pso, const c2<T>& pe,
+                                      const c2<T>& pw, T inv_dx2, T inv_dy2) {
+    c2<T> lap;
+    lap.x = ((pn.x - (T)2 * ps.x) + pso.x) * inv_dy2 + ((pe.x - (T)2 * ps.x) + pw.x) * inv_dx2;
+    lap.y = ((pn.y - (T)2 * ps.y) + pso.y) * inv_dy2 + ((pe.y - (T)2 * ps.y) + pw.y) * inv_dx2;
+    return lap;
+}
+
+// Isotropic 9-point Laplacian (dx == dy = h) of rk4_cuda_solver.py:760-770:
+// (4 (n + s + e + w) + sw + se + nw + ne - 20 psi) / (6 h^2), where the
+// diagonal order is (y-1, x-1), (y-1, x+1), (y+1, x-1), (y+1, x+1).
+template <typename T>
+__device__ __forceinline__ c2<T> lap9(const c2<T>& ps, const c2<T>& pn, const c2<T>& pso, const c2<T>& pe,
+                                      const c2<T>& pw, const c2<T>& psw, const c2<T>& pse, const c2<T>& pnw,
+                                      const c2<T>& pne, T inv_6h2) {
+    c2<T> lap;
+    lap.x = ((((((T)4 * (((pn.x + pso.x) + pe.x) + pw.x) + psw.x) + pse.x) + pnw.x) + pne.x) - (T)20 * ps.x) * inv_6h2;
+    lap.y = ((((((T)4 * (((pn.y + pso.y) + pe.y) + pw.y) + psw.y) + pse.y) + pnw.y) + pne.y) - (T)20 * ps.y) * inv_6h2;
+    return lap;
+}
+
 // Scalar condensate and reservoir derivatives at one point from the stage
-// input ps, its four neighbours and the stage reservoir n (model.py:89-127).
+// input ps, its Laplacian and the stage reservoir n (model.py:89-127), with
+// an optional complex potential V in (V + gC|psi|^2 + gR nA + gI nI) psi.
 // Both the per-stage and the whole-step kernels call this, so they perform
 // the same operations in the same order.
 template <typename T, int RES>
 __device__ __forceinline__ void scalar_rhs(const pgpe_member& mb, const ScalarConsts<T>& k, const c2<T>& ps,
-                                           const c2<T>& pn, const c2<T>& pso, const c2<T>& pe, const c2<T>& pw,
-                                           const T n[2], T P, c2<T>& d, T dn[2]) {
+                                           const c2<T>& lap, const T n[2], T P, const c2<T>* V, c2<T>& d,
+                                           T dn[2]) {
     constexpr int NF = ResFields<RES>::n;
-    // (psi[y+1] - 2 psi + psi[y-1]) / dy^2 + (psi[x+1] - 2 psi + psi[x-1]) / dx^2  (model.py:220-224)
-    c2<T> lap;
-    lap.x = ((pn.x - (T)2 * ps.x) + pso.x) * k.inv_dy2 + ((pe.x - (T)2 * ps.x) + pw.x) * k.inv_dx2;
-    lap.y = ((pn.y - (T)2 * ps.y) + pso.y) * k.inv_dy2 + ((pe.y - (T)2 * ps.y) + pw.y) * k.inv_dx2;
     const T rho = ps.x * ps.x + ps.y * ps.y;
     dn[0] = (T)0;
     dn[1] = (T)0;
     reservoir_rhs<T, RES>(mb, rho, P, n, dn);
     const T nA = (NF > 0) ? n[0] : (T)0;
     const T nI = (NF > 1) ? n[1] : (T)0;
-    const T eff = (k.gC * rho + k.gR * nA) + k.gI * nI;
     const T gain = (k.R * nA - k.gammaC) * (T)0.5;
-    const T Xr = k.kin * lap.x + eff * ps.x;
-    const T Xi = k.kin * lap.y + eff * ps.y;
+    T Xr, Xi;
+    if (V == nullptr) {
+        const T eff = (k.gC * rho + k.gR * nA) + k.gI * nI;
+        Xr = k.kin * lap.x + eff * ps.x;
+        Xi = k.kin * lap.y + eff * ps.y;
+    } else {
+        const T er = ((V->x + k.gC * rho) + k.gR * nA) + k.gI * nI;
+        const T ei = V->y;
+        Xr = k.kin * lap.x + (er * ps.x - ei * ps.y);
+        Xi = k.kin * lap.y + (er * ps.y + ei * ps.x);
+    }
     d.x = k.inv_hbar * Xi + gain * ps.x;
     d.y = -(k.inv_hbar * Xr) + gain * ps.y;
     if (k.eta_c != (T)0) {
@@ -132,16 +181,13 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
     const int64_t off = mb.off;
     const c2<T>* __restrict__ psi0 = p.psi0 + off;
 
-    // ---- stage input with halo into LDS (periodic wrap) ----
+    // ---- stage input with halo into LDS (periodic wrap or mirrored edges) ----
+    const int bc = mb.boundary;
     for (int i = threadIdx.x; i < (TH + 2) * W; i += 256) {
         const int ly = i / W;
         const int lx = i - ly * W;
-        int gy = y0 + ly - 1;
-        int gx = x0 + lx - 1;
-        gy = gy < 0 ? gy + ny : gy;
-        gx = gx < 0 ? gx + nx : gx;
-        if (gy >= ny) gy %= ny;
-        if (gx >= nx) gx %= nx;
+        const int gy = map_index(y0 + ly - 1, ny, bc);
+        const int gx = map_index(x0 + lx - 1, nx, bc);
         const int64_t g = (int64_t)gy * nx + gx;
         c2<T> v = psi0[g];
         if constexpr (KIND != 0) {
@@ -157,6 +203,8 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
     const int gx = x0 + lane;
     if (gx >= nx) return;
     const ScalarConsts<T> kc(mb);
+    const bool nine = mb.laplacian == PGPE_LAP_NINE;
+    const T inv_6h2 = kc.inv_dx2 / (T)6;
     const T coef = p.coef[m];
     const T* __restrict__ env = p.pump + mb.pump_off;
     uint64_t step = p.step;
@@ -175,9 +223,16 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
             n[f] = v;
         }
         const T P = env[idx] * coef;
+        const c2<T> ps = s_psi[c];
+        const c2<T> lap = nine ? lap9<T>(ps, s_psi[c + W], s_psi[c - W], s_psi[c + 1], s_psi[c - 1], s_psi[c - W - 1],
+                                         s_psi[c - W + 1], s_psi[c + W - 1], s_psi[c + W + 1], inv_6h2)
+                               : lap5<T>(ps, s_psi[c + W], s_psi[c - W], s_psi[c + 1], s_psi[c - 1], kc.inv_dx2,
+                                         kc.inv_dy2);
+        c2<T> vpot;
+        if (p.potential) vpot = p.potential[mb.pot_off + idx];
         c2<T> d;
         T dn[2];
-        scalar_rhs<T, RES>(mb, kc, s_psi[c], s_psi[c + W], s_psi[c - W], s_psi[c + 1], s_psi[c - 1], n, P, d, dn);
+        scalar_rhs<T, RES>(mb, kc, ps, lap, n, P, p.potential ? &vpot : nullptr, d, dn);
 
         if constexpr (KIND != 2) {
             p.kout[off + idx] = d;
@@ -429,6 +484,8 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
     const int y0 = (local / mb.tiles_x) * TH;
     const int64_t off = mb.off;
     const ScalarConsts<T> kc(mb);
+    const bool nine = mb.laplacian == PGPE_LAP_NINE;
+    const T inv_6h2 = kc.inv_dx2 / (T)6;
     T coef[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) coef[s] = p.coef[s * p.members + m];
@@ -442,10 +499,8 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
         if (f < NP) {
             const int ly = f / EW;
             const int lx = f - ly * EW;
-            int gy = (y0 + ly - kStepHalo) % ny;
-            int gx = (x0 + lx - kStepHalo) % nx;
-            gy += gy < 0 ? ny : 0;
-            gx += gx < 0 ? nx : 0;
+            const int gy = map_index(y0 + ly - kStepHalo, ny, mb.boundary);
+            const int gx = map_index(x0 + lx - kStepHalo, nx, mb.boundary);
             const int g = gy * nx + gx;
             z[j] = p.psi0[off + g];
             env[j] = p.pump[mb.pump_off + g];
@@ -470,14 +525,27 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
             if (f >= NP) continue;
             // neighbours clamped into the buffer; border points are stale anyway
             const c2<T> ps = cur[f];
-            const c2<T> pn = cur[f + EW < NP ? f + EW : f];
-            const c2<T> pso = cur[f >= EW ? f - EW : f];
+            const int fn = f + EW < NP ? f + EW : f;
+            const int fs = f >= EW ? f - EW : f;
+            const c2<T> pn = cur[fn];
+            const c2<T> pso = cur[fs];
             const c2<T> pe = cur[f + 1 < NP ? f + 1 : f];
             const c2<T> pw = cur[f >= 1 ? f - 1 : f];
+            c2<T> lap;
+            if (nine) {
+                // corners: clamped into the buffer like the edges (stale only on the outer ring)
+                const c2<T> psw = cur[fs >= 1 ? fs - 1 : fs];
+                const c2<T> pse = cur[fs + 1 < NP ? fs + 1 : fs];
+                const c2<T> pnw = cur[fn >= 1 ? fn - 1 : fn];
+                const c2<T> pne = cur[fn + 1 < NP ? fn + 1 : fn];
+                lap = lap9<T>(ps, pn, pso, pe, pw, psw, pse, pnw, pne, inv_6h2);
+            } else {
+                lap = lap5<T>(ps, pn, pso, pe, pw, kc.inv_dx2, kc.inv_dy2);
+            }
             const T P = env[j] * coef[s];
             c2<T> d;
             T dn[2];
-            scalar_rhs<T, RES>(mb, kc, ps, pn, pso, pe, pw, ns[j], P, d, dn);
+            scalar_rhs<T, RES>(mb, kc, ps, lap, ns[j], P, nullptr, d, dn);
             if (s == 0) {
                 acc[j] = d;
 #pragma unroll
@@ -647,6 +715,7 @@ static int launch_typed(const pgpe_rk4_args* a, hipStream_t s) {
     p.coef = (const T*)a->coef;
     p.step = a->step;
     p.step_base = a->step_base;
+    p.potential = (const c2<T>*)a->potential;
     switch (a->reservoir) {
         case PGPE_RES_NONE: return launch_th<T, PGPE_RES_NONE>(a, p, s);
         case PGPE_RES_SINGLE: return launch_th<T, PGPE_RES_SINGLE>(a, p, s);
@@ -664,6 +733,8 @@ static int check_args(const pgpe_rk4_args* a) {
     if (a->components != 1 && a->components != 2) return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: components must be 1 or 2");
     if (a->components == 2 && a->res_components != 1 && a->res_components != 2)
         return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: res_components must be 1 or 2");
+    if (a->components == 2 && a->potential)
+        return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: spinor fields take no potential here");
     if (!a->table || !a->tile_map || !a->psi0 || !a->pump || !a->coef)
         return fail(PGPE_ERR_ARG, "pgpe_rk4_stage: null table, field, pump or coefficient");
     const int nf = a->reservoir == PGPE_RES_NONE ? 0 : (a->reservoir == PGPE_RES_SINGLE ? 1 : 2);
@@ -707,6 +778,7 @@ extern "C" int pgpe_rk4_step(const pgpe_rk4_args* a, void* stream) {
     if (!a->table || !a->tile_map || !a->psi0 || !a->psi_out || !a->pump || !a->coef)
         return fail(PGPE_ERR_ARG, "pgpe_rk4_step: null table, field, pump or coefficient");
     if (a->psi_out == a->psi0) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: psi_out must not alias psi0");
+    if (a->potential) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: a potential needs the stage kernels (pgpe_rk4_stage)");
     const int nf = a->reservoir == PGPE_RES_NONE ? 0 : (a->reservoir == PGPE_RES_SINGLE ? 1 : 2);
     for (int f = 0; f < nf; ++f) {
         if (!a->res0[f] || !a->res_out[f]) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: null reservoir field");

@@ -191,6 +191,14 @@ def sync(stream=None) -> None:
     N.call("pgpe_stream_sync", _sp(stream))
 
 
+def device_name(device: int = 0) -> str:
+    """Marketing name of the device (part of the autotuner's cache key)."""
+    init()
+    buf = C.create_string_buffer(256)
+    N.call("pgpe_device_name", device, buf, C.c_size_t(256))
+    return buf.value.decode(errors="replace")
+
+
 class Event:
     def __init__(self):
         init()

@@ -58,6 +58,7 @@ class Rk4Engine:
         self.res = [DeviceArray.zeros((nr,), r) for _ in range(self.nres)]
         self._res_alt = [DeviceArray.zeros((nr,), r) for _ in range(self.nres)] if self.whole_step else None
         self.step_base = DeviceArray.zeros((1,), np.uint64)
+        self.potential = None      # packed complex potential (DeviceArray) or None; stage kernels only
         # active tile list: starts as every tile; the sweep driver may retire members
         self.tile_map = table.tile_map_dev
         self.num_tiles = table.num_tiles
@@ -94,6 +95,7 @@ class Rk4Engine:
         a.tile_map = self.tile_map.ptr
         a.psi0 = psi_in.ptr
         a.pump = self.pump.ptr
+        a.potential = self.potential.ptr if self.potential is not None else None
         lib = N.lib()
         if self.whole_step:
             for f in range(self.nres):
@@ -157,7 +159,8 @@ class Rk4Engine:
                 chunk = rem - rem % 2
                 p0, p1 = self.psi
                 key = (chunk, const, noise, self.num_tiles, self.tile_map.ptr, p0.ptr, coef.ptr,
-                       self.res[0].ptr if self.whole_step and self.nres else 0)
+                       self.res[0].ptr if self.whole_step and self.nres else 0,
+                       self.potential.ptr if self.potential is not None else 0)
                 g = self._graphs.get(key)
                 if g is None:
                     r0, r1 = self.res, self._res_alt

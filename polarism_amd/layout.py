@@ -39,6 +39,9 @@ class MemberSpec:
     noise_strength: float = 0.0
     dt: float = 1e-3
     circular_polarization: float = 0.0
+    laplacian: str = "five-point"     # or "isotropic-9pt" (dx == dy)
+    boundary: str = "periodic"        # or "closed-interval" (mirrored Neumann edges)
+    pot_off: int = 0                  # points into the packed complex potential, if any
 
 
 class MemberTable:
@@ -78,6 +81,13 @@ class MemberTable:
             row.res_a, row.res_b, row.res_c, row.res_d = reservoir_rates(s.reservoir, get)
             row.alpha_2, row.zeeman, row.te_tm = get("alpha_2"), get("zeeman"), get("te_tm")
             row.inv_4dxdy = 1.0 / (4 * s.dx * s.dy)
+            row.laplacian = N.LAPLACIAN_KIND[s.laplacian]
+            row.boundary = N.BOUNDARY_KIND[s.boundary]
+            row.pot_off = s.pot_off
+            if s.laplacian == "isotropic-9pt" and abs(s.dx - s.dy) > 1e-8 * max(s.dx, s.dy):
+                raise ValueError(f"laplacian 'isotropic-9pt' requires dx == dy, got dx={s.dx}, dy={s.dy}")
+            if s.components == 2 and (s.laplacian != "five-point" or s.boundary != "periodic"):
+                raise ValueError("spinor members run the periodic five-point stencil")
             row.noise_gamma = get("gamma_C")
             row.noise_R = get("R")
             row.noise_strength = float(s.noise_strength)
@@ -115,6 +125,13 @@ class MemberTable:
     def member_points(self, m: int) -> int:
         s = self.specs[m]
         return s.nx * s.ny
+
+    def set_pot_offsets(self, offsets) -> None:
+        """Point member m at offsets[m] of the packed potential and re-upload the table."""
+        for m, o in enumerate(offsets):
+            self.specs[m].pot_off = int(o)
+            self.rows[m].pot_off = int(o)
+        self.refresh()
 
     def refresh(self) -> None:
         """Re-upload the table after host-side edits of ``rows``."""

@@ -14,7 +14,7 @@ import numpy as np
 
 from ..device import DeviceArray
 from ..engine import Rk4Engine
-from ..solver_api import ACCEPTED, SolverCapabilities, register_solver
+from ..solver_api import ACCEPTED, SolverCapabilities, SolverCompatibilityError, register_solver
 from .base import HipSolverBase
 
 _ALL_RES = frozenset({"single", "double", "quadratic-double", "none"})
@@ -23,13 +23,14 @@ _ALL_RES = frozenset({"single", "double", "quadratic-double", "none"})
 @register_solver("rk4-fdm-hip")
 class RK4FDMHipSolver(HipSolverBase):
     capabilities = SolverCapabilities(
-        components=frozenset({1, 2}), ndim=frozenset({2}), grid_types=frozenset({"periodic"}),
-        reservoirs=_ALL_RES, custom_reservoirs="unsupported", potential=False,
+        components=frozenset({1, 2}), ndim=frozenset({2}), grid_types=frozenset({"periodic", "closed-interval"}),
+        reservoirs=_ALL_RES, custom_reservoirs="unsupported", potential=True,
         kinetic_relaxation=True, dispersion=frozenset({"parabolic"}), terms=False,
         noise=frozenset({"none", "wigner"}), quantitative_reservoirs=_ALL_RES,
         description="Stage-coupled FDM RK4 on gfx950: one fused HIP launch per RK stage over all "
                     "members, LDS halo tiles, in-kernel Wigner noise, HIP-graph chunks; scalar or spinor "
-                    "(TE-TM 9-point, Zeeman, alpha_2). Periodic 2-D, zero potential.")
+                    "(TE-TM 9-point, Zeeman, alpha_2). 2-D periodic or closed-interval grids, 5-point or "
+                    "isotropic 9-point Laplacian, static complex potential.")
     injects_noise = True   # advance() adds the Wigner increment inside the final stage kernel
     max_graph_chunk = 1000
 
@@ -38,12 +39,62 @@ class RK4FDMHipSolver(HipSolverBase):
         self.engine = None
         self._const_coef = None
         self._coef_chunk = None
+        self._pot_key = None
+        self._pot_dev = None
+        # rk4_cuda_solver.py:2280-2294: the two stencils, and the 9-point one needs dx == dy
+        self.laplacian = getattr(config.solver, "laplacian", "five-point")
+        if self.laplacian not in ("five-point", "isotropic-9pt"):
+            raise ValueError(f"solver.laplacian='{self.laplacian}' is not recognised. "
+                             "Valid values: 'five-point', 'isotropic-9pt'.")
+        if self.laplacian == "isotropic-9pt":
+            rel = abs(grid.dx - grid.dy) / max(grid.dx, grid.dy)
+            if rel > 1e-8:
+                raise ValueError(f"solver.laplacian='isotropic-9pt' requires dx == dy, got dx={grid.dx}, "
+                                 f"dy={grid.dy} (relative difference {rel:.2e}).")
+        self.boundary = getattr(grid, "boundary", "periodic")
+        if self.components == 2 and (self.laplacian != "five-point" or self.boundary != "periodic"):
+            raise SolverCompatibilityError("rk4-fdm-hip runs spinor fields on periodic grids with the "
+                                           "five-point stencil")
+
+    def member_specs(self, model, batch):
+        specs = super().member_specs(model, batch)
+        for s in specs:
+            s.laplacian = self.laplacian
+            s.boundary = self.boundary
+        return specs
+
+    def _potential(self, model, batch):
+        """Packed complex potential (shared or one per member) or None for V = 0."""
+        pot = model.potential
+        if pot is None or np.ndim(pot) == 0:
+            if pot not in (None, 0, 0.0):
+                raise ValueError("a constant nonzero potential must be given as an array")
+            return None
+        if self.components == 2:
+            raise SolverCompatibilityError("rk4-fdm-hip takes no potential for spinor fields")
+        pot = np.asarray(pot)
+        key = (id(model.potential), pot.shape, pot.dtype.str, batch)
+        if self._pot_key != key:
+            ny, nx = self.grid.shape
+            arr = pot.reshape(-1, ny, nx) if pot.size % (ny * nx) == 0 else np.broadcast_to(pot, (ny, nx))[None]
+            if arr.shape[0] not in (1, batch):
+                raise ValueError(f"potential holds {arr.shape[0]} members for a batch of {batch}")
+            self._pot_dev = DeviceArray.from_numpy(np.ascontiguousarray(arr, dtype=self.cdtype).reshape(-1))
+            self._pot_shared = arr.shape[0] == 1
+            self._pot_key = key
+        return self._pot_dev
 
     def _ensure_engine(self, model, batch):
+        pot = self._potential(model, batch)
         tab = self.ensure_table(model, batch)
+        if pot is not None:
+            per = 0 if self._pot_shared else self.grid.nx * self.grid.ny
+            if any(s.pot_off != m * per for m, s in enumerate(tab.specs)):
+                tab.set_pot_offsets([m * per for m in range(batch)])
         if self.engine is None or self.engine.table is not tab:
             self.engine = Rk4Engine(tab, self._pump_dev, self.precision, self.reservoir_kind, self.stream)
             self._const_coef = None
+        self.engine.potential = pot
         if getattr(self.engine, "_dt", None) != self.config.solver.dt:
             self.engine.set_dt(self.config.solver.dt)
         return self.engine

@@ -99,7 +99,7 @@ def generate(members, dt=0.05, rtol=1e-4, max_time=3000.0, log=print):
     """Cold-start steady states; returns (features, targets, converged mask, result)."""
     t0 = time.perf_counter()
     sw = PackedSweep(members, reservoir="single", precision="single", dt=dt, check_every=1.0, rtol=rtol,
-                     max_time=max_time, tile_h=8, progress=log)
+                     max_time=max_time, progress=log)
     res = sw.run()
     conv = np.isfinite(res.stop_times)
     norm = default_normaliser()

@@ -57,8 +57,12 @@ class SweepResult:
 
 class PackedSweep:
     def __init__(self, members: list[SweepMember], *, reservoir="single", precision="single", dt=0.05,
-                 check_every=1.0, rtol=1e-6, min_time=0.0, max_time=1000.0, tile_h=16,
+                 check_every=1.0, rtol=1e-6, min_time=0.0, max_time=1000.0, tile_h="auto", form="auto",
                  initial_amplitude=1e-3, retire=True, graphs=True, progress=None, progress_every_s=30.0):
+        """``form`` is "step" (whole-step kernel), "stage" (four stage kernels) or "auto";
+        ``tile_h`` a tile height or "auto", which asks the autotuner (cached per size mix).
+        An explicit tile height with ``form="auto"`` takes the whole-step kernel where that
+        height allows it (16, 32, 64)."""
         init()
         self.members = members
         self.reservoir = reservoir
@@ -83,11 +87,22 @@ class PackedSweep:
             specs.append(MemberSpec(nx=m.nx, ny=m.ny, dx=m.lx / m.nx, dy=m.ly / m.ny, physics=m.physics,
                                     reservoir=reservoir, pump_off=off, seed=m.seed, dt=self.dt))
             off += m.nx * m.ny
-        self.table = MemberTable(specs, tile_h)
+        if tile_h == "auto":
+            from .autotune import tune_rk4
+            choice = tune_rk4(specs, precision, reservoir)
+            if form in ("auto", choice.form):
+                form, tile_h = choice.form, choice.tile_h
+            else:
+                tile_h = 32 if form == "step" else 16
+        elif form == "auto":
+            form = "step" if tile_h in (16, 32, 64) else "stage"
+        self.form, self.tile_h = form, int(tile_h)
+        self.table = MemberTable(specs, self.tile_h)
         r = real_dtype(precision)
         env = np.concatenate([np.asarray(m.envelope, dtype=np.float64).reshape(-1) for m in members])
         self.pump = DeviceArray.from_numpy(env.astype(r))
-        self.engine = Rk4Engine(self.table, self.pump, precision, reservoir, self.stream)
+        self.engine = Rk4Engine(self.table, self.pump, precision, reservoir, self.stream,
+                                whole_step=(form == "step"))
         self.engine.set_dt(self.dt)
         coef = np.tile(np.asarray([m.P0 for m in members], dtype=np.float64), (4, 1)).astype(r)
         self.coef = DeviceArray.from_numpy(coef)

@@ -36,9 +36,12 @@ def oracle_physics(cfg) -> O.Physics:
                      zeeman=p.zeeman, te_tm=p.te_tm)
 
 
-def oracle_rk4_run(cfg, n_steps, psi0=None, res0=None, noise=False):
-    """Run the oracle rk4-fdm on a single-member config; returns (psi (1,1,ny,nx), res tuple)."""
-    g = O.Grid(cfg.grid.nx, cfg.grid.ny, cfg.grid.lx, cfg.grid.ly)
+def oracle_rk4_run(cfg, n_steps, psi0=None, res0=None, noise=False, potential=0.0):
+    """Run the oracle rk4-fdm on a single-member config; returns (psi (1,1,ny,nx), res tuple).
+
+    The grid type and ``solver.laplacian`` come from the config."""
+    g = O.Grid(cfg.grid.nx, cfg.grid.ny, cfg.grid.lx, cfg.grid.ly, boundary=cfg.grid.grid_type)
+    kw = dict(potential=potential, laplacian_kind=cfg.solver.laplacian)
     prec = cfg.solver.precision
     real = np.float32 if prec == "single" else np.float64
     kind = cfg.reservoir.reservoir_type
@@ -63,7 +66,7 @@ def oracle_rk4_run(cfg, n_steps, psi0=None, res0=None, noise=False):
     psi, res = psi0, res0
     dt = cfg.solver.dt
     for s in range(n_steps):
-        psi, res = O.rk4_fdm_step(kind, p, g, psi, res, pump_at, s * dt, dt)
+        psi, res = O.rk4_fdm_step(kind, p, g, psi, res, pump_at, s * dt, dt, **kw)
         if nz is not None:
             O.wigner_add(psi, res[0] if res else 0, dt=dt, step=s, **nz)
     return psi, res

@@ -1,0 +1,41 @@
+"""Autotuner cache logic (host only: the timer is injected)."""
+from polarism_amd.autotune import Autotuner, LaunchChoice, candidates, tune_key
+
+
+def test_key_depends_on_size_multiset_not_order():
+    a = tune_key("rk4-fdm-hip", "dev", "single", "double", 1, [(64, 64), (128, 96), (64, 64)])
+    b = tune_key("rk4-fdm-hip", "dev", "single", "double", 1, [(128, 96), (64, 64), (64, 64)])
+    c = tune_key("rk4-fdm-hip", "dev", "single", "double", 1, [(128, 96), (64, 64)])
+    d = tune_key("rk4-fdm-hip", "dev", "double", "double", 1, [(128, 96), (64, 64), (64, 64)])
+    assert a == b and a != c and a != d
+
+
+def test_candidates_respect_kernel_limits():
+    assert LaunchChoice("step", 64) in candidates("single", 1)
+    assert LaunchChoice("step", 64) not in candidates("double", 1)
+    assert all(c.form == "stage" for c in candidates("single", 2))
+
+
+def test_choice_is_fastest_and_persisted(tmp_path):
+    path = tmp_path / "tune.json"
+    times = {LaunchChoice("stage", 8): 3.0, LaunchChoice("stage", 16): 2.0, LaunchChoice("step", 32): 1.0}
+    calls = []
+
+    def timer(opt):
+        calls.append(opt)
+        return times[opt]
+
+    t = Autotuner(path)
+    best = t.choose("k", list(times), timer)
+    assert best == LaunchChoice("step", 32) and len(calls) == 3
+    again = Autotuner(path).choose("k", list(times), lambda o: 1 / 0)     # cached: no timing
+    assert again == best
+    assert Autotuner(path).lookup("other") is None
+
+
+def test_corrupt_cache_is_ignored(tmp_path):
+    path = tmp_path / "tune.json"
+    path.write_text("{not json")
+    t = Autotuner(path)
+    assert t.lookup("k") is None
+    assert t.choose("k", [LaunchChoice("stage", 16)], lambda o: 1.0) == LaunchChoice("stage", 16)

@@ -163,3 +163,41 @@ def test_spinor_without_coupling_equals_two_scalar_runs():
         half.noise.seed = seed
         ref = _run(half)
         np.testing.assert_allclose(s.state.field.get()[0, c], ref.state.field.get()[0, 0], rtol=1e-12, atol=1e-18)
+
+
+@pytest.mark.parametrize("grid_type", ["periodic", "closed-interval"])
+@pytest.mark.parametrize("laplacian", ["five-point", "isotropic-9pt"])
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("single", 1e-5)])
+def test_boundaries_and_stencils_match_oracle(grid_type, laplacian, precision, tol):
+    """Closed-interval mirrored edges (model.py:227-248) and the isotropic 9-point stencil
+    (rk4_cuda_solver.py:719-770) against the oracle, with the pump off-centre so the edges matter."""
+    cfg = quickstart_config(grid__nx=72, grid__ny=72, grid__lx=36.0, grid__ly=36.0, grid__grid_type=grid_type,
+                            solver__laplacian=laplacian, solver__precision=precision, solver__total_time=0.3,
+                            laser__P0=0.3, laser__sigma_space=6.0, laser__x0=12.0, laser__y0=-9.0,
+                            noise__initial_amplitude=0.05)
+    cfg.solver.dt = 2e-3
+    ctl = _run(cfg)
+    psi_ref, res_ref = oracle_rk4_run(cfg, 150)
+    e = O.rel_l2(ctl.state.field.get(), psi_ref)
+    print(f"{grid_type} {laplacian} {precision}: rel-L2 psi={e:.3e}")
+    assert e <= tol
+    assert O.rel_l2(ctl.reservoir.get_state()[0].get(), res_ref[0]) <= tol
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("single", 1e-5)])
+def test_complex_potential_matches_oracle(precision, tol):
+    """A static complex potential V enters (V + gC|psi|^2 + ...) psi (model.py:89-127)."""
+    from polarism_amd import SimulationController
+    cfg = quickstart_config(grid__nx=64, grid__ny=64, grid__lx=40.0, grid__ly=40.0, solver__precision=precision,
+                            solver__total_time=0.2, laser__P0=0.2, noise__initial_amplitude=0.05)
+    cfg.solver.dt = 2e-3
+    g = O.Grid(64, 64, 40.0, 40.0)
+    pot = (0.3 * np.exp(-(g.X ** 2 + g.Y ** 2) / 50.0) - 0.02j * (g.X / 20.0) ** 2).astype(np.complex128)
+    ctl = SimulationController(cfg)
+    ctl.model.potential = pot
+    ctl.run()
+    real = np.float32 if precision == "single" else np.float64
+    cplx = np.complex64 if precision == "single" else np.complex128
+    psi_ref, res_ref = oracle_rk4_run(cfg, 100, potential=pot.astype(cplx))
+    assert O.rel_l2(ctl.state.field.get(), psi_ref) <= tol
+    assert O.rel_l2(ctl.reservoir.get_state()[0].get(), res_ref[0].astype(real)) <= tol

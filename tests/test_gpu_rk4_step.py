@@ -7,7 +7,8 @@ pytestmark = pytest.mark.gpu
 SIZES = [(64, 64), (96, 80), (200, 72), (128, 128), (8, 40)]
 
 
-def _engines(sizes, reservoir, precision, tile_h, noise_strength=0.0):
+def _engines(sizes, reservoir, precision, tile_h, noise_strength=0.0, laplacian="five-point",
+             boundary="periodic"):
     from polarism_amd.config import PhysicsConstants
     from polarism_amd.device import DeviceArray, init
     from polarism_amd.engine import Rk4Engine, real_dtype
@@ -19,8 +20,9 @@ def _engines(sizes, reservoir, precision, tile_h, noise_strength=0.0):
     rng = np.random.default_rng(7)
     specs, envs, off = [], [], 0
     for i, (nx, ny) in enumerate(sizes):
-        specs.append(MemberSpec(nx=nx, ny=ny, dx=0.5, dy=0.6, physics=p, reservoir=reservoir, pump_off=off,
-                                seed=31 + i, noise_strength=noise_strength, dt=0.05))
+        specs.append(MemberSpec(nx=nx, ny=ny, dx=0.5, dy=0.5 if laplacian == "isotropic-9pt" else 0.6,
+                                physics=p, reservoir=reservoir, pump_off=off, seed=31 + i,
+                                noise_strength=noise_strength, dt=0.05, laplacian=laplacian, boundary=boundary))
         envs.append(rng.uniform(0.0, 1.0, nx * ny))
         off += nx * ny
     r = real_dtype(precision)
@@ -66,6 +68,16 @@ def test_whole_step_tile_heights_and_graph_chunks(tile_h, precision):
     e_stage, e_step, coef = _engines(SIZES, "single", precision, tile_h, noise_strength=1.0)
     for e in (e_stage, e_step):
         e.run_steps(7, coef, True, True, 3, graphs=True)
+    _compare(e_stage, e_step)
+
+
+@pytest.mark.parametrize("laplacian", ["five-point", "isotropic-9pt"])
+@pytest.mark.parametrize("boundary", ["periodic", "closed-interval"])
+def test_whole_step_stencils_and_boundaries(laplacian, boundary):
+    """The 4-point halo of the whole-step kernel is exact for the mirrored closed interval too."""
+    e_stage, e_step, coef = _engines(SIZES, "double", "double", 16, laplacian=laplacian, boundary=boundary)
+    for e in (e_stage, e_step):
+        e.run_steps(6, coef, True, False, 0, graphs=True)
     _compare(e_stage, e_step)
 
 

@@ -70,3 +70,32 @@ def test_retirement_does_not_change_running_members():
     b = _run(members[1:], reservoir="single", max_time=3.0, rtol=1e-12)
     for i in range(1, 4):
         np.testing.assert_array_equal(a.final_psi[i], b.final_psi[i - 1])
+
+
+def test_whole_step_sweep_equals_stage_sweep_bitwise():
+    members = _members(SIZES[:5], reservoir="double")
+    from polarism_amd.sweep import PackedSweep
+    runs = []
+    for form, tile_h in (("stage", 16), ("step", 32)):
+        sw = PackedSweep(members, reservoir="double", dt=0.05, check_every=0.5, rtol=1e-3, max_time=3.0,
+                         tile_h=tile_h, form=form)
+        runs.append(sw.run())
+    a, b = runs
+    np.testing.assert_array_equal(a.stop_times, b.stop_times)
+    for i in range(len(members)):
+        np.testing.assert_array_equal(a.final_psi[i], b.final_psi[i])
+        for x, y in zip(a.final_res[i], b.final_res[i]):
+            np.testing.assert_array_equal(x, y)
+
+
+def test_autotuner_picks_a_valid_form_and_caches_it(tmp_path):
+    from polarism_amd.autotune import Autotuner, candidates, tune_rk4
+    from polarism_amd.layout import MemberSpec
+    from polarism_amd.config import PhysicsConstants
+    p = PhysicsConstants()
+    specs = [MemberSpec(nx=n, ny=n, dx=0.5, dy=0.5, physics=p, reservoir="single", dt=0.05) for n in (64, 128, 96)]
+    tuner = Autotuner(tmp_path / "tune.json")
+    choice = tune_rk4(specs, "single", "single", steps=4, tuner=tuner)
+    assert choice in candidates("single", 1)
+    assert (tmp_path / "tune.json").exists()
+    assert tune_rk4(specs, "single", "single", steps=4, tuner=Autotuner(tmp_path / "tune.json")) == choice

@@ -335,3 +335,51 @@ def test_pulse_train_peaks_and_cutoffs():
     assert O.pulse_gaussian_coeff(first + 0.29, **sparse) > 0.0
     assert O.pulse_gaussian_coeff(first + 0.31, **sparse) == 0.0          # beyond 3 sigma, before the next pulse
     assert O.pulse_gaussian_coeff(first + 0.6, **{**kw, "n_pulses": 1}) == 0.0
+
+
+# ---------------------------------------------------------------- closed interval and 9-point stencils
+def _sym5(k, h):
+    return -(2 - 2 * np.cos(k * h)) / h ** 2
+
+
+def _sym9(kx, ky, h):
+    a, b = kx * h, ky * h
+    return (4 * (2 * np.cos(a) + 2 * np.cos(b)) + 4 * np.cos(a) * np.cos(b) - 20) / (6 * h ** 2)
+
+
+@pytest.mark.parametrize("kind", ["five-point", "isotropic-9pt"])
+def test_closed_interval_neumann_cosines_are_eigenfunctions(kind):
+    """cos(m pi (x + L/2) / L) on the nodes is even about both ends, so the mirrored stencil keeps it."""
+    g = O.Grid(33, 33, 16.0, 16.0, boundary="closed-interval")
+    assert g.dx == 0.5 and g.X[0, 0] == -8.0 and g.X[0, -1] == 8.0
+    kx, ky = 3 * np.pi / g.lx, 2 * np.pi / g.ly
+    f = np.cos(kx * (g.X + g.lx / 2)) * np.cos(ky * (g.Y + g.ly / 2))
+    lap = O.laplacian(f, g.dx, g.dy, "closed-interval", kind)
+    sym = _sym5(kx, g.dx) + _sym5(ky, g.dy) if kind == "five-point" else _sym9(kx, ky, g.dx)
+    np.testing.assert_allclose(lap, sym * f, atol=1e-12 * abs(sym))
+    # the constant is in the kernel (Neumann): Laplacian exactly zero
+    np.testing.assert_array_equal(O.laplacian(np.full(g.shape, 2.5), g.dx, g.dy, "closed-interval", kind), 0.0)
+
+
+def test_nine_point_symbol_and_isotropy():
+    """Periodic 9-point stencil on plane cosines gives its symbol; its O(h^2) error is isotropic."""
+    g = O.Grid(48, 48, 24.0, 24.0)
+    h = g.dx
+    kx, ky = 2 * np.pi * 5 / g.lx, 2 * np.pi * 3 / g.ly
+    f = np.cos(kx * g.X) * np.cos(ky * g.Y)
+    np.testing.assert_allclose(O.laplacian(f, h, h, "periodic", "isotropic-9pt"), _sym9(kx, ky, h) * f,
+                               atol=1e-12 * abs(_sym9(kx, ky, h)))
+    k = 0.6
+    along = _sym9(k, 0.0, h) + k ** 2
+    diag = _sym9(k / np.sqrt(2), k / np.sqrt(2), h) + k ** 2
+    five_along = _sym5(k, h) + k ** 2
+    five_diag = 2 * _sym5(k / np.sqrt(2), h) + k ** 2
+    assert abs(along - diag) < 0.01 * abs(along)                 # 9-point: direction-free at O(h^2)
+    assert abs(five_along - five_diag) > 0.3 * abs(five_along)    # 5-point: not
+
+
+def test_closed_grid_matches_reference_geometry():
+    g = O.Grid(5, 3, 4.0, 2.0, boundary="closed-interval")
+    assert (g.dx, g.dy) == (1.0, 1.0)
+    np.testing.assert_array_equal(g.X[0], [-2, -1, 0, 1, 2])
+    np.testing.assert_array_equal(g.Y[:, 0], [-1, 0, 1])
