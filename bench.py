@@ -110,44 +110,42 @@ def _time_launches(e, launch, reps):
 
 def roofline_leg(n=8192, reps=20):
     """Average durations of the FP32 middle-stage kernel (fastest tile height) and of the whole-step kernel."""
-    stage = {}
-    for th in (16, 32):
+    def stage_kernel_ms(th):
         e, coef = _member_engine(n, th, False)
         ptrs = e.coef_ptrs(coef, 0)
         e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0)
-        stage[th] = _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0, stages=(1,)), reps)
-        del e
-    tile_h = min(stage, key=stage.get)
-    avg_ms = stage[tile_h]
+        mid = _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0, stages=(1,)), reps)
+        whole = _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0), reps)
+        return mid, whole
+
+    def step_kernel_ms(th):
+        e, coef = _member_engine(n, th, True)
+        ptrs = e.coef_ptrs(coef, 0)
+        r0, r1 = e.res, e._res_alt
+        return _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0, res_in=r0, res_out=r1),
+                              reps)
+
+    stage = {th: stage_kernel_ms(th) for th in (16, 32)}
+    tile_h = min(stage, key=lambda th: stage[th][0])
+    avg_ms, stage_step_ms = stage[tile_h]
     bytes_per_launch = 40.0 * n * n
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    pts = float(n) * n
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": f"rk4_stage_kernel<float, SINGLE, {tile_h}, middle>", "grid": f"{n}x{n}",
             "avg_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch),
-            "avg_us_by_tile_h": {str(k): round(v * 1e3, 2) for k, v in stage.items()}}
-    del e
+            "avg_us_by_tile_h": {str(k): round(v[0] * 1e3, 2) for k, v in stage.items()},
+            "stage_path_step_us": round(stage_step_ms * 1e3, 2),
+            "stage_path_effective_GBps_172B_model": round(172.0 * pts / (stage_step_ms * 1e-3) / 1e9, 1)}
     ws = {}
     for th in (32, 64):
-        e, coef = _member_engine(n, th, True)
-        ptrs = e.coef_ptrs(coef, 0)
-        r0, r1 = e.res, e._res_alt
-        ms = _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0, res_in=r0, res_out=r1),
-                            reps)
-        pts = float(n) * n
+        ms = step_kernel_ms(th)
         ws[f"tile_h_{th}"] = {"avg_us": round(ms * 1e3, 2),
                               "point_steps_per_s": round(pts / (ms * 1e-3), 1),
                               "effective_GBps_172B_model": round(172.0 * pts / (ms * 1e-3) / 1e9, 1),
                               "algorithmic_GBps_28B": round(28.0 * pts / (ms * 1e-3) / 1e9, 1)}
-        del e
-    stage_step_ms = None
-    e, coef = _member_engine(n, tile_h, False)
-    ptrs = e.coef_ptrs(coef, 0)
-    stage_step_ms = _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0), reps)
-    del e
     roof["whole_step_kernel"] = ws
-    roof["stage_path_step_us"] = round(stage_step_ms * 1e3, 2)
-    roof["stage_path_effective_GBps"] = round(172.0 * n * n / (stage_step_ms * 1e-3) / 1e9, 1)
     return roof
 
 

@@ -176,7 +176,8 @@ int pgpe_rk4_stage(const pgpe_rk4_args* args, void* stream);
 
 /* One whole RK4 step (all four stages, the combination, the clamp and the optional
  * Wigner increment) in a single launch over the same member table and tile map; the
- * same result as the four pgpe_rk4_stage launches, bit for bit.  Replaces the
+ * same result as the four pgpe_rk4_stage launches (bit for bit on periodic grids, to
+ * rounding on closed intervals, whose mirrored halo points are recomputed).  Replaces the
  * reference's one-launch tiled step (rk4_cuda_solver.py:1835-2194).  Scalar fields
  * (components 1); tile_y is the table's tile_h: 16, 32, or 64 (single precision).
  * Uses psi0, res0, pump, psi_out, res_out, noise, step, step_base and coef, which here

@@ -4,8 +4,9 @@ For a batch of members it times every launch form the device supports --
 the four fused stage launches (1-point halo, tile heights 8 / 16 / 32) and
 the whole-step launch (4-point halo recomputed on chip, tile heights 16 / 32
 and 64 in single precision) -- on the batch's own member table, and keeps the
-fastest.  Every form gives the same result bit for bit (the kernels share
-their arithmetic), so the choice only changes speed.  Choices are stored in a
+fastest.  Every form gives the same result (bit for bit on periodic grids,
+to rounding on closed intervals; the kernels share their arithmetic), so the
+choice only changes speed.  Choices are stored in a
 JSON file keyed by device, method, precision, reservoir, components and the
 multiset of grid sizes; later runs on the same key skip the timing.
 

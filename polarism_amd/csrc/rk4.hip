@@ -442,7 +442,10 @@ __global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T
 // once per step, instead of the 13 complex + 17 real arrays per point of the
 // four stage launches.  Each point runs the same operations in the same order
 // as the stage kernels (scalar_rhs; the final combination accumulates
-// ((k1 + 2 k2) + 2 k3) + k4 left to right), so the two paths agree bit for bit.
+// ((k1 + 2 k2) + 2 k3) + k4 left to right), so on periodic grids the two paths
+// agree bit for bit.  On a closed interval the halo points beyond a wall are
+// mirror images computed here rather than copied, and their stencil sums run
+// in mirrored order, so the two paths agree to rounding there.
 template <typename T>
 struct Rk4StepParams {
     const pgpe_member* table;

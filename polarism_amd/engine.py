@@ -3,7 +3,7 @@
 Owns the buffers of one batch (uniform or uneven grids) and issues either the
 four fused ``pgpe_rk4_stage`` launches of a step or, with ``whole_step``, one
 ``pgpe_rk4_step`` launch that runs all four stages on halo tiles (same
-result bit for bit; scalar fields; the reservoir then ping-pongs between two
+result, bit for bit on periodic grids; scalar fields; the reservoir then ping-pongs between two
 buffers like the field).  Used by the ``rk4-fdm-hip`` solver and by the
 packed sweep driver.
 """

@@ -1,10 +1,10 @@
 """Fourier neural operator for steady-state warm starts, on hand-written kernels.
 
-Maps a pump map plus three scalars (P0 / P_th, gamma_C, g_C) on an H x W grid
-to the steady state (Re psi, Im psi, n_R).  Architecture (width 64, 4 Fourier
+Maps a pump map plus three scalars (P0 / P_th, gamma_C, g_C), given as four
+input channels on an H x W grid, to the steady state (Re psi, Im psi, n_R).  Architecture (width 64, 4 Fourier
 layers, 32 x 32 kept modes: kx in [0, 32), ky in [-16, 16)):
 
-    v0 = Lift(features)                                   pointwise 6 -> 64
+    v0 = Lift(features)                                   pointwise 4 -> 64
     v_{l+1} = GELU(Wp_l v_l + b_l + S_l v_l)  (no GELU after the last layer)
     S_l v   = real inverse DFT of  M_l[ky] (d_l[ky, kx] * DFT(v)[ky, kx])
     out     = Q2 GELU(Q1 v_4 + q1) + q2                    pointwise 64 -> 128 -> 3
@@ -35,7 +35,7 @@ BF = torch.bfloat16
 
 
 class FNOConfig:
-    def __init__(self, H=256, W=256, width=64, layers=4, modes_x=32, modes_y=32, in_features=6, hidden=128,
+    def __init__(self, H=256, W=256, width=64, layers=4, modes_x=32, modes_y=32, in_features=4, hidden=128,
                  out_features=3):
         self.H, self.W, self.width, self.layers = H, W, width, layers
         self.mx, self.my = modes_x, modes_y

@@ -19,8 +19,8 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--rtol", type=float, default=1e-4)
-    ap.add_argument("--max-time", type=float, default=3000.0)
+    ap.add_argument("--rtol", type=float, default=1e-6)
+    ap.add_argument("--max-time", type=float, default=4000.0)
     ap.add_argument("--train-seconds", type=float, default=None)
     ap.add_argument("--out", required=True)
     args = ap.parse_args(argv)

@@ -11,7 +11,14 @@
 * ``train``         -- AdamW on the mean squared error of normalised targets;
   the network's forward and backward run on the HIP kernels.
 * ``warm_start``    -- predicted (psi, n_R) plus the same noise field a cold
-  start would get, as initial state of the same steady-state run.
+  start would get, as initial state of the same steady-state run; a member
+  predicted to hold no condensate (peak density below ``ZERO_STATE_PEAK``)
+  starts from psi = 0 with the predicted reservoir.
+
+The inputs carry no coordinate channels, so the network is equivariant to
+the translations of the periodic box, and training augments every batch with
+a random circular shift and one of the eight symmetries of the square (the
+GPE on a square periodic grid with dx = dy is invariant under both).
 """
 from __future__ import annotations
 
@@ -23,11 +30,15 @@ import numpy as np
 
 from ..config import PhysicsConstants
 from ..noise import make_initial_psi
-from ..pumps import coords, random_pump, threshold
+from ..pumps import random_pump, threshold
 from ..sweep import PackedSweep, SweepMember
 
 GRID = dict(n=256, L=128.0)
-IN_FEATURES = 6
+IN_FEATURES = 4
+# A member whose predicted peak density stays below this (|psi|^2, in the units of the solver)
+# is started from psi = 0, the steady state of a pump below threshold: a predicted residue of
+# order 1e-3 would otherwise have to decay to the norm < 1e-30 test like a cold start does.
+ZERO_STATE_PEAK = 1e-4
 
 
 def make_members(count, seed, n=GRID["n"], L=GRID["L"], ratio=(0.8, 4.0)):
@@ -44,10 +55,9 @@ def make_members(count, seed, n=GRID["n"], L=GRID["L"], ratio=(0.8, 4.0)):
 
 
 def features(members) -> np.ndarray:
-    """(6, B, H, W) float32: local pump / P_th, P0 / P_th, gamma_C / 0.083, g_C / 0.001, x / (lx/2), y / (ly/2)."""
+    """(4, B, H, W) float32: local pump / P_th, P0 / P_th, gamma_C / 0.083, g_C / 0.001."""
     m0 = members[0]
     H, W = m0.ny, m0.nx
-    X, Y = coords(m0.nx, m0.ny, m0.lx, m0.ly)
     out = np.empty((IN_FEATURES, len(members), H, W), dtype=np.float32)
     for b, m in enumerate(members):
         pth = threshold(m.physics, "single")
@@ -56,8 +66,6 @@ def features(members) -> np.ndarray:
         out[1, b] = ratio
         out[2, b] = m.physics.gamma_C / 0.083
         out[3, b] = m.physics.g_C / 0.001
-        out[4, b] = X / (m.lx / 2)
-        out[5, b] = Y / (m.ly / 2)
     return out
 
 
@@ -95,7 +103,7 @@ def default_normaliser() -> Normaliser:
     return Normaliser(psi_scale=math.sqrt(p.gamma_C / p.g_C) * 0.1, n_scale=p.gamma_C / p.R)
 
 
-def generate(members, dt=0.05, rtol=1e-4, max_time=3000.0, log=print):
+def generate(members, dt=0.05, rtol=1e-6, max_time=4000.0, log=print):
     """Cold-start steady states; returns (features, targets, converged mask, result)."""
     t0 = time.perf_counter()
     sw = PackedSweep(members, reservoir="single", precision="single", dt=dt, check_every=1.0, rtol=rtol,
@@ -133,8 +141,7 @@ def train(x, y, epochs=40, batch=16, lr=2e-3, seed=0, log=print, cfg=None, time_
         acc = 0.0
         for s in range(steps_per_epoch):
             idx = perm[s * batch:(s + 1) * batch].to(dev)
-            xb = X[:, idx].contiguous()
-            yb = Y[:, idx].contiguous()
+            xb, yb = _augment(X[:, idx], Y[:, idx], g)
             out = mod(xb)
             loss = ((out - yb) ** 2).mean()
             opt.zero_grad(set_to_none=True)
@@ -155,6 +162,24 @@ def train(x, y, epochs=40, batch=16, lr=2e-3, seed=0, log=print, cfg=None, time_
             break
     params = {k: v.detach().cpu().numpy() for k, v in mod.params.items()}
     return params, hist, cfg
+
+
+def _augment(xb, yb, gen):
+    """One random symmetry of the square (flips, transpose) and one random circular shift,
+    applied to inputs and targets alike."""
+    import torch
+    k = int(torch.randint(0, 8, (1,), generator=gen))
+    sy, sx = (int(v) for v in torch.randint(0, xb.shape[2], (2,), generator=gen))
+    out = []
+    for t in (xb, yb):
+        if k & 1:
+            t = torch.flip(t, dims=(3,))
+        if k & 2:
+            t = torch.flip(t, dims=(2,))
+        if k & 4:
+            t = t.transpose(2, 3)
+        out.append(torch.roll(t, shifts=(sy, sx), dims=(2, 3)).contiguous())
+    return out[0], out[1]
 
 
 def predict(params, x, cfg=None, batch=16):
@@ -179,9 +204,16 @@ def warm_start(members, y_pred, norm: Normaliser | None = None, noise_amplitude=
     out = []
     for b, m in enumerate(members):
         psi, n = norm.fields(y_pred, b)
-        noise = make_initial_psi((m.ny, m.nx), noise_amplitude, seed=m.seed, precision="single")
+        label = dict(m.label)
+        if float(np.max(np.abs(psi) ** 2)) < ZERO_STATE_PEAK:
+            psi0 = np.zeros((m.ny, m.nx), np.complex64)
+            label["warm"] = "zero"
+        else:
+            noise = make_initial_psi((m.ny, m.nx), noise_amplitude, seed=m.seed, precision="single")
+            psi0 = (psi + noise).astype(np.complex64)
+            label["warm"] = "predicted"
         out.append(SweepMember(m.nx, m.ny, m.lx, m.ly, m.physics, m.envelope, m.P0, seed=m.seed,
-                               psi0=(psi + noise).astype(np.complex64), res0=(n,), label=dict(m.label)))
+                               psi0=psi0, res0=(n,), label=label))
     return out
 
 
