@@ -71,8 +71,10 @@ def summarise(res, extra_s=0.0):
             "converged": int(conv.sum()), "members": int(conv.size),
             "wall_to_last_s": round(t_last, 3), "wall_total_s": round(res.wall_s + extra_s, 3),
             "median_stop_ps": float(np.nanmedian(res.stop_times)) if conv.any() else None,
+            "last_stop_ps": float(np.nanmax(res.stop_times)) if conv.any() else None,
             "steps_run": res.steps_run, "point_steps": res.point_steps,
-            "effective_GBps": round(res.point_steps * 172.0 / res.wall_s / 1e9, 1)}
+            "point_steps_per_s": round(res.point_steps / res.wall_s, 1),
+            "effective_GBps_172B_model": round(res.point_steps * 172.0 / res.wall_s / 1e9, 1)}
 
 
 def _member_engine(n, tile_h, whole_step, reservoir="single"):
@@ -250,8 +252,8 @@ def fft_leg(steps=1000):
     return out
 
 
-def cpu_baseline(point_steps, converged, steps=300):
-    """Oracle rk4-fdm on the host (one core, complex64 256^2) scaled to the sweep's point-steps."""
+def _oracle_rate(steps):
+    """Point-steps/s of the oracle rk4-fdm step (complex64, one 256^2 member) on this core."""
     from oracle import gpe_oracle as O
     g = O.Grid(SWEEP["n"], SWEEP["n"], SWEEP["L"], SWEEP["L"])
     p = O.Physics()
@@ -263,12 +265,45 @@ def cpu_baseline(point_steps, converged, steps=300):
     t0 = time.perf_counter()
     for _ in range(steps):
         psi, res = O.rk4_fdm_step("single", p, g, psi, res, pump_at, 0.0, SWEEP["dt"])
-    dt = time.perf_counter() - t0
-    rate = steps * g.nx * g.ny / dt
-    cpu_time = point_steps / rate
-    return {"value": round(converged / cpu_time, 6), "unit": "members/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} oracle rk4-fdm steps, complex64 256x256, {dt:.1f} s on one core "
-                      f"({rate / 1e6:.2f} Mpt-step/s), scaled to the plain sweep's {point_steps:.3e} point-steps"}
+    return steps * g.nx * g.ny / (time.perf_counter() - t0)
+
+
+def cpu_worker(steps, procs):
+    """Runs in a child process that never touches the GPU: one core, then a pool of ``procs``."""
+    import multiprocessing as mp
+    single = _oracle_rate(steps)
+    with mp.get_context("fork").Pool(procs) as pool:
+        t0 = time.perf_counter()
+        rates = pool.map(_oracle_rate, [steps] * procs)
+        wall = time.perf_counter() - t0
+    pooled = procs * steps * SWEEP["n"] ** 2 / wall
+    print(json.dumps({"single": single, "pooled": pooled, "per_proc": rates, "procs": procs}))
+
+
+def cpu_baseline(point_steps, converged, steps=400):
+    """The oracle rk4-fdm (the reference's NumPy algorithm) on the host cores, in a child process:
+    one member per core, scaled to the plain sweep's point-steps."""
+    import subprocess
+    procs = max(1, min(16, len(os.sched_getaffinity(0))))
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", str(steps), str(procs)],
+                         capture_output=True, text=True, timeout=600, cwd=ROOT)
+    if out.returncode != 0:
+        raise RuntimeError(f"cpu worker failed: {out.stderr[-2000:]}")
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    pooled, single = r["pooled"], r["single"]
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    return {"value": round(converged / (point_steps / pooled), 6), "unit": "members/s", "cores": procs,
+            "kind": "port",
+            "sample": f"oracle rk4-fdm (NumPy restatement of rk4_fdm_solver.py), complex64 256x256, {steps} steps "
+                      f"per process on {procs} processes ({pooled / 1e6:.1f} Mpt-step/s pooled, "
+                      f"{single / 1e6:.2f} on one core), scaled to the plain sweep's {point_steps:.3e} point-steps",
+            "single_core_members_per_s": round(converged / (point_steps / single), 6),
+            "cpu_model": cpu_model, "affinity_cores": len(os.sched_getaffinity(0))}
 
 
 def density_agreement(cold, warm):
@@ -286,6 +321,9 @@ def density_agreement(cold, warm):
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-worker":
+        cpu_worker(int(sys.argv[2]), int(sys.argv[3]))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
@@ -330,6 +368,7 @@ def main():
         if have_weights:
             warm = summarise(warm_runs[-1], extra_s=infer_s[-1])
             warm["inference_s"] = round(infer_s[-1], 3)
+            warm["zero_starts"] = int(sum(m.label.get("warm") == "zero" for m in wm))
             sweep["warm"] = warm
             sweep["speedup_warm_over_plain"] = round(warm["members_per_s"] / max(plain["members_per_s"], 1e-12), 3)
             sweep["density_agreement"] = density_agreement(plain_runs[-1], warm_runs[-1])

@@ -17,7 +17,10 @@
 // array once per stage; the 5-point Laplacian then reads LDS.  Wavefront w
 // handles rows w, w+4, ... so every global access of a wavefront is one
 // contiguous 64-point row segment.  Arithmetic follows the reference's
-// operation order; the library is built with -ffp-contract=off.
+// operation order; the library is built with -ffp-contract=off, and the RK4
+// point arithmetic (stencils, right-hand side, stage combinations) opts into
+// fused multiply-adds per source expression (clang fp contract(on)), so the
+// stage and whole-step kernels fuse the same operations.
 #include "common.hpp"
 
 namespace pgpe {
@@ -54,6 +57,7 @@ struct Rk4Params {
 // before touching the array (double_reservoir.py:268-269).
 template <typename T, int RES>
 __device__ __forceinline__ void reservoir_rhs(const pgpe_member& mb, T rho, T P, const T n[2], T d[2]) {
+#pragma clang fp contract(on)
     if constexpr (RES == PGPE_RES_SINGLE) {
         // dnR = P - (gamma_R + R rho) nR
         d[0] = P - ((T)mb.res_a + (T)mb.R * rho) * n[0];
@@ -69,6 +73,36 @@ __device__ __forceinline__ void reservoir_rhs(const pgpe_member& mb, T rho, T P,
         d[0] = (transfer - (T)mb.res_b * nR) - (T)mb.R * nR * rho;
         d[1] = (P - transfer) - (T)mb.res_c * nI;
     }
+}
+
+// Streaming accesses of the stage kernels: arrays each point touches once per
+// launch (reservoir terms, pump, outputs).  Built with PGPE_NT_STREAMS they
+// carry the non-temporal hint so they do not displace the psi / k rows that
+// neighbouring tiles re-read as halo.
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+#ifdef PGPE_NT_STREAMS
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+template <typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) {
+#ifdef PGPE_NT_STREAMS
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+template <typename T>
+__device__ __forceinline__ void st_stream_c(c2<T>* p, const c2<T>& v) {
+#ifdef PGPE_NT_STREAMS
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+#else
+    *p = v;
+#endif
 }
 
 template <int RES> struct ResFields { static constexpr int n = (RES == 0) ? 0 : (RES == 1 ? 1 : 2); };
@@ -107,6 +141,7 @@ __device__ __forceinline__ int map_index(int g, int n, int boundary) {
 template <typename T>
 __device__ __forceinline__ c2<T> lap5(const c2<T>& ps, const c2<T>& pn, const c2<T>& pso, const c2<T>& pe,
                                       const c2<T>& pw, T inv_dx2, T inv_dy2) {
+#pragma clang fp contract(on)
     c2<T> lap;
     lap.x = ((pn.x - (T)2 * ps.x) + pso.x) * inv_dy2 + ((pe.x - (T)2 * ps.x) + pw.x) * inv_dx2;
     lap.y = ((pn.y - (T)2 * ps.y) + pso.y) * inv_dy2 + ((pe.y - (T)2 * ps.y) + pw.y) * inv_dx2;
@@ -120,6 +155,7 @@ template <typename T>
 __device__ __forceinline__ c2<T> lap9(const c2<T>& ps, const c2<T>& pn, const c2<T>& pso, const c2<T>& pe,
                                       const c2<T>& pw, const c2<T>& psw, const c2<T>& pse, const c2<T>& pnw,
                                       const c2<T>& pne, T inv_6h2) {
+#pragma clang fp contract(on)
     c2<T> lap;
     lap.x = ((((((T)4 * (((pn.x + pso.x) + pe.x) + pw.x) + psw.x) + pse.x) + pnw.x) + pne.x) - (T)20 * ps.x) * inv_6h2;
     lap.y = ((((((T)4 * (((pn.y + pso.y) + pe.y) + pw.y) + psw.y) + pse.y) + pnw.y) + pne.y) - (T)20 * ps.y) * inv_6h2;
@@ -135,6 +171,7 @@ template <typename T, int RES>
 __device__ __forceinline__ void scalar_rhs(const pgpe_member& mb, const ScalarConsts<T>& k, const c2<T>& ps,
                                            const c2<T>& lap, const T n[2], T P, const c2<T>* V, c2<T>& d,
                                            T dn[2]) {
+#pragma clang fp contract(on)
     constexpr int NF = ResFields<RES>::n;
     const T rho = ps.x * ps.x + ps.y * ps.y;
     dn[0] = (T)0;
@@ -166,6 +203,7 @@ __device__ __forceinline__ void scalar_rhs(const pgpe_member& mb, const ScalarCo
 // KIND: 0 = first stage (input is psi0), 1 = middle stage, 2 = final stage.
 template <typename T, int RES, int TH, int KIND>
 __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
+#pragma clang fp contract(on)
     constexpr int NF = ResFields<RES>::n;
     constexpr int W = kTileW + 2;
     __shared__ c2<T> s_psi[(TH + 2) * W];
@@ -218,11 +256,11 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
         T n[2] = {(T)0, (T)0};
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            T v = p.res0[f][off + idx];
-            if constexpr (KIND != 0) v = v + p.a * p.kres_prev[f][off + idx];
+            T v = ld_stream(&p.res0[f][off + idx]);
+            if constexpr (KIND != 0) v = v + p.a * ld_stream(&p.kres_prev[f][off + idx]);
             n[f] = v;
         }
-        const T P = env[idx] * coef;
+        const T P = ld_stream(&env[idx]) * coef;
         const c2<T> ps = s_psi[c];
         const c2<T> lap = nine ? lap9<T>(ps, s_psi[c + W], s_psi[c - W], s_psi[c + 1], s_psi[c - 1], s_psi[c - W - 1],
                                          s_psi[c - W + 1], s_psi[c + W - 1], s_psi[c + W + 1], inv_6h2)
@@ -235,9 +273,9 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
         scalar_rhs<T, RES>(mb, kc, ps, lap, n, P, p.potential ? &vpot : nullptr, d, dn);
 
         if constexpr (KIND != 2) {
-            p.kout[off + idx] = d;
+            st_stream_c<T>(&p.kout[off + idx], d);
 #pragma unroll
-            for (int f = 0; f < NF; ++f) p.kres_out[f][off + idx] = dn[f];
+            for (int f = 0; f < NF; ++f) st_stream(&p.kres_out[f][off + idx], dn[f]);
         } else {
             const c2<T> a1 = p.k1[off + idx];
             const c2<T> a2 = p.k2[off + idx];
@@ -468,14 +506,19 @@ constexpr int kStepHalo = 4;
 
 template <typename T, int RES, int TH>
 __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p) {
+#pragma clang fp contract(on)
     constexpr int NF = ResFields<RES>::n;
     constexpr int EW = kTileW + 2 * kStepHalo;
     constexpr int EH = TH + 2 * kStepHalo;
     constexpr int NP = EW * EH;
     constexpr int PPT = (NP + 255) / 256;
+    // Each buffer has a one-row-plus-one pad on both sides, so every stencil
+    // neighbour f +- EW +- 1 is an LDS address without clamping; the pads and
+    // the outer ring hold stale values that only reach stale halo points.
+    constexpr int PAD = EW + 1;
     extern __shared__ unsigned char smem_raw[];
-    c2<T>* buf0 = reinterpret_cast<c2<T>*>(smem_raw);
-    c2<T>* buf1 = buf0 + NP;
+    c2<T>* buf0 = reinterpret_cast<c2<T>*>(smem_raw) + PAD;
+    c2<T>* buf1 = buf0 + NP + 2 * PAD;
 
     const int tile = xcd_tile(blockIdx.x, p.num_tiles);
     if (tile < 0) return;
@@ -528,20 +571,14 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
             if (f >= NP) continue;
             // neighbours clamped into the buffer; border points are stale anyway
             const c2<T> ps = cur[f];
-            const int fn = f + EW < NP ? f + EW : f;
-            const int fs = f >= EW ? f - EW : f;
-            const c2<T> pn = cur[fn];
-            const c2<T> pso = cur[fs];
-            const c2<T> pe = cur[f + 1 < NP ? f + 1 : f];
-            const c2<T> pw = cur[f >= 1 ? f - 1 : f];
+            const c2<T> pn = cur[f + EW];
+            const c2<T> pso = cur[f - EW];
+            const c2<T> pe = cur[f + 1];
+            const c2<T> pw = cur[f - 1];
             c2<T> lap;
             if (nine) {
-                // corners: clamped into the buffer like the edges (stale only on the outer ring)
-                const c2<T> psw = cur[fs >= 1 ? fs - 1 : fs];
-                const c2<T> pse = cur[fs + 1 < NP ? fs + 1 : fs];
-                const c2<T> pnw = cur[fn >= 1 ? fn - 1 : fn];
-                const c2<T> pne = cur[fn + 1 < NP ? fn + 1 : fn];
-                lap = lap9<T>(ps, pn, pso, pe, pw, psw, pse, pnw, pne, inv_6h2);
+                lap = lap9<T>(ps, pn, pso, pe, pw, cur[f - EW - 1], cur[f - EW + 1], cur[f + EW - 1],
+                              cur[f + EW + 1], inv_6h2);
             } else {
                 lap = lap5<T>(ps, pn, pso, pe, pw, kc.inv_dx2, kc.inv_dy2);
             }
@@ -602,7 +639,8 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
 template <typename T, int RES, int TH>
 static int launch_step_th(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
     constexpr int NP = (kTileW + 2 * kStepHalo) * (TH + 2 * kStepHalo);
-    const size_t shm = 2 * NP * sizeof(c2<T>);
+    constexpr int PAD = kTileW + 2 * kStepHalo + 1;
+    const size_t shm = 2 * (NP + 2 * PAD) * sizeof(c2<T>);
     if (shm > 160 * 1024) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: tile too tall for LDS at this precision");
     auto kern = rk4_step_kernel<T, RES, TH>;
     if (shm > 65536) {

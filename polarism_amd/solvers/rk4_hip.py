@@ -2,8 +2,10 @@
 
 Drop-in for the reference's ``rk4-fdm`` / ``rk4-cuda`` methods
 (solver/rk4_fdm_solver.py:14-77, solver/rk4_cuda_solver.py:2197-3642):
-the same RK4 stages, reservoir clamp and noise semantics, with each stage one
-fused launch of ``pgpe_rk4_stage`` over every member of the batch.  Chunks
+the same RK4 stages, reservoir clamp and noise semantics, with one
+``pgpe_rk4_step`` launch per step over every member of the batch (all four
+stages on halo tiles) for scalar fields without a potential, and four fused
+``pgpe_rk4_stage`` launches per step otherwise (same result).  Chunks
 of up to 1000 steps are captured as HIP graphs whose pump coefficients and
 noise step base live in device buffers, so replaying a chunk needs no host
 synchronisation inside the chunk.
@@ -91,13 +93,30 @@ class RK4FDMHipSolver(HipSolverBase):
             per = 0 if self._pot_shared else self.grid.nx * self.grid.ny
             if any(s.pot_off != m * per for m, s in enumerate(tab.specs)):
                 tab.set_pot_offsets([m * per for m in range(batch)])
-        if self.engine is None or self.engine.table is not tab:
-            self.engine = Rk4Engine(tab, self._pump_dev, self.precision, self.reservoir_kind, self.stream)
+        # one whole-step launch per step where it applies (scalar, no potential); the four
+        # stage launches otherwise -- both give the same result
+        whole = self.components == 1 and pot is None
+        if self.engine is None or self.engine.table is not tab or self.engine.whole_step != whole:
+            if self.engine is not None and self.engine.table is tab:
+                self._carry_state(whole)
+            else:
+                self.engine = Rk4Engine(tab, self._pump_dev, self.precision, self.reservoir_kind, self.stream,
+                                        whole_step=whole)
             self._const_coef = None
         self.engine.potential = pot
         if getattr(self.engine, "_dt", None) != self.config.solver.dt:
             self.engine.set_dt(self.config.solver.dt)
         return self.engine
+
+    def _carry_state(self, whole):
+        """Switch launch form on the same table, keeping the current field and reservoir."""
+        old = self.engine
+        self.engine = Rk4Engine(old.table, self._pump_dev, self.precision, self.reservoir_kind, self.stream,
+                                whole_step=whole)
+        self.engine.psi[0].copy_from(old.psi[0], self.stream)
+        for a, b in zip(self.engine.res, old.res):
+            a.copy_from(b, self.stream)
+        self.stream.synchronize()
 
     def _adopt_state(self, model, state):
         e = self.engine

@@ -10,10 +10,9 @@
   of the U(1)-symmetric GPE is defined only up to a global phase).
 * ``train``         -- AdamW on the mean squared error of normalised targets;
   the network's forward and backward run on the HIP kernels.
-* ``warm_start``    -- predicted (psi, n_R) plus the same noise field a cold
-  start would get, as initial state of the same steady-state run; a member
-  predicted to hold no condensate (peak density below ``ZERO_STATE_PEAK``)
-  starts from psi = 0 with the predicted reservoir.
+* ``warm_start``    -- predicted (psi, n_R) as the initial state of the same
+  steady-state run; a member predicted to hold no condensate (peak density
+  below ``ZERO_STATE_PEAK``) starts from psi = 0 with the predicted reservoir.
 
 The inputs carry no coordinate channels, so the network is equivariant to
 the translations of the periodic box, and training augments every batch with
@@ -29,7 +28,6 @@ from dataclasses import dataclass
 import numpy as np
 
 from ..config import PhysicsConstants
-from ..noise import make_initial_psi
 from ..pumps import random_pump, threshold
 from ..sweep import PackedSweep, SweepMember
 
@@ -38,7 +36,9 @@ IN_FEATURES = 4
 # A member whose predicted peak density stays below this (|psi|^2, in the units of the solver)
 # is started from psi = 0, the steady state of a pump below threshold: a predicted residue of
 # order 1e-3 would otherwise have to decay to the norm < 1e-30 test like a cold start does.
-ZERO_STATE_PEAK = 1e-4
+# Condensed steady states of the training distribution peak above ~0.05; the surrogate's
+# residue on below-threshold members stays under ~0.01.
+ZERO_STATE_PEAK = 1e-2
 
 
 def make_members(count, seed, n=GRID["n"], L=GRID["L"], ratio=(0.8, 4.0)):
@@ -198,22 +198,24 @@ def predict(params, x, cfg=None, batch=16):
         return torch.cat(outs, dim=1).cpu().numpy()
 
 
-def warm_start(members, y_pred, norm: Normaliser | None = None, noise_amplitude=1e-3):
-    """Members whose initial state is the prediction plus the cold start's noise field."""
+def warm_start(members, y_pred, norm: Normaliser | None = None):
+    """Members whose initial state is the predicted (psi, n_R).
+
+    The cold start's noise field (RMS 1e-3) is not added: the prediction's own error is larger,
+    so it seeds any symmetry breaking the dynamics needs.  A member predicted to hold no
+    condensate (peak density below ``ZERO_STATE_PEAK``) starts from psi = 0 exactly."""
     norm = norm or default_normaliser()
     out = []
     for b, m in enumerate(members):
         psi, n = norm.fields(y_pred, b)
         label = dict(m.label)
-        if float(np.max(np.abs(psi) ** 2)) < ZERO_STATE_PEAK:
-            psi0 = np.zeros((m.ny, m.nx), np.complex64)
+        if float(np.max(psi.real ** 2 + psi.imag ** 2)) < ZERO_STATE_PEAK:
+            psi = np.zeros((m.ny, m.nx), np.complex64)
             label["warm"] = "zero"
         else:
-            noise = make_initial_psi((m.ny, m.nx), noise_amplitude, seed=m.seed, precision="single")
-            psi0 = (psi + noise).astype(np.complex64)
             label["warm"] = "predicted"
         out.append(SweepMember(m.nx, m.ny, m.lx, m.ly, m.physics, m.envelope, m.P0, seed=m.seed,
-                               psi0=psi0, res0=(n,), label=label))
+                               psi0=psi, res0=(n,), label=label))
     return out
 
 
