@@ -110,6 +110,19 @@ def _time_launches(e, launch, reps):
     return total / reps
 
 
+def _profiled_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "rk4_stage_fp32_8192_traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    entry = data.get("kernels", {}).get(kernel)
+    return None if entry is None else int(entry["traffic_bytes"])
+
+
 def roofline_leg(n=8192, reps=20):
     """Average durations of the FP32 middle-stage kernel (fastest tile height) and of the whole-step kernel."""
     def stage_kernel_ms(th):
@@ -133,8 +146,9 @@ def roofline_leg(n=8192, reps=20):
     bytes_per_launch = 40.0 * n * n
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     pts = float(n) * n
+    traffic = _profiled_traffic(f"rk4_stage_kernel<float, 1, {tile_h}, 1>")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": f"rk4_stage_kernel<float, SINGLE, {tile_h}, middle>", "grid": f"{n}x{n}",
             "avg_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch),
             "avg_us_by_tile_h": {str(k): round(v[0] * 1e3, 2) for k, v in stage.items()},

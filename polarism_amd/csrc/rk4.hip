@@ -76,12 +76,13 @@ __device__ __forceinline__ void reservoir_rhs(const pgpe_member& mb, T rho, T P,
 }
 
 // Streaming accesses of the stage kernels: arrays each point touches once per
-// launch (reservoir terms, pump, outputs).  Built with PGPE_NT_STREAMS they
-// carry the non-temporal hint so they do not displace the psi / k rows that
-// neighbouring tiles re-read as halo.
+// launch (reservoir terms, pump, outputs).  They carry the non-temporal hint so
+// they do not displace the psi / k rows that neighbouring tiles re-read as halo
+// (measured on the 8192^2 FP32 middle stage: 503 -> 473 us).  Building with
+// PGPE_TEMPORAL_STREAMS drops the hint for comparison.
 template <typename T>
 __device__ __forceinline__ T ld_stream(const T* p) {
-#ifdef PGPE_NT_STREAMS
+#ifndef PGPE_TEMPORAL_STREAMS
     return __builtin_nontemporal_load(p);
 #else
     return *p;
@@ -89,7 +90,7 @@ __device__ __forceinline__ T ld_stream(const T* p) {
 }
 template <typename T>
 __device__ __forceinline__ void st_stream(T* p, T v) {
-#ifdef PGPE_NT_STREAMS
+#ifndef PGPE_TEMPORAL_STREAMS
     __builtin_nontemporal_store(v, p);
 #else
     *p = v;
@@ -97,7 +98,7 @@ __device__ __forceinline__ void st_stream(T* p, T v) {
 }
 template <typename T>
 __device__ __forceinline__ void st_stream_c(c2<T>* p, const c2<T>& v) {
-#ifdef PGPE_NT_STREAMS
+#ifndef PGPE_TEMPORAL_STREAMS
     __builtin_nontemporal_store(v.x, &p->x);
     __builtin_nontemporal_store(v.y, &p->y);
 #else

@@ -95,7 +95,7 @@ class RK4FDMHipSolver(HipSolverBase):
                 tab.set_pot_offsets([m * per for m in range(batch)])
         # one whole-step launch per step where it applies (scalar, no potential); the four
         # stage launches otherwise -- both give the same result
-        whole = self.components == 1 and pot is None
+        whole = self.components == 1 and pot is None and tab.tile_h in (16, 32, 64)
         if self.engine is None or self.engine.table is not tab or self.engine.whole_step != whole:
             if self.engine is not None and self.engine.table is tab:
                 self._carry_state(whole)

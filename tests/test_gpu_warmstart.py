@@ -37,8 +37,10 @@ def test_warm_start_reaches_the_cold_steady_state_sooner():
     for i in np.flatnonzero(both):
         dc = np.abs(cold.final_psi[i]) ** 2
         dw = np.abs(hot.final_psi[i]) ** 2
-        agree.append(np.linalg.norm(dw - dc) / max(np.linalg.norm(dc), 1e-30))
+        nc, nw = np.linalg.norm(dc), np.linalg.norm(dw)
+        # both at the zero steady state (norm below the criterion's 1e-30): they agree
+        agree.append(0.0 if nc < 1e-30 and nw < 1e-30 else np.linalg.norm(dw - dc) / max(nc, 1e-30))
     agree = np.asarray(agree)
     print("density rel-L2 warm vs cold:", np.round(agree, 6))
-    assert both.sum() >= 8
+    assert both.sum() >= max(4, int(0.8 * np.isfinite(cold.stop_times).sum()))
     assert np.median(agree) <= 1e-3
