@@ -250,7 +250,7 @@ def fft_leg(steps=1000):
         cfg.noise.stochastic = "wigner"
         ctl = SimulationController(cfg)
         sol = ctl.solver
-        sol.advance(ctl.model, ctl.state, 100)
+        sol.advance(ctl.model, ctl.state, steps)     # captures the chunk's graph outside the timed region
         stream = sol.stream
         stream.synchronize()
         start, stop = Event(), Event()
@@ -320,6 +320,54 @@ def cpu_baseline(point_steps, converged, steps=400):
             "cpu_model": cpu_model, "affinity_cores": len(os.sched_getaffinity(0))}
 
 
+def lawson_leg(ticks=5):
+    """Spinor Lawson IFRK, DP5 adaptive (rtol 1e-6, 1 ps ticks), 1024^2, lx = ly = 300 um, TE-TM 0.05,
+    Zeeman 0.02, alpha_2 = -0.1 g_C, elliptic ring pump (r 15 um, width 3 um), double reservoir,
+    complex64 fields with FP64 error norms: a window of ``ticks`` ps after one warm-up tick."""
+    from polarism_amd import Config, SimulationController
+    from polarism_amd.config import GridParameters
+    from polarism_amd.device import Event
+    cfg = Config()
+    cfg.grid = GridParameters(nx=1024, ny=1024, lx=300.0, ly=300.0)
+    cfg.physics.spin = "spinor"
+    cfg.physics.te_tm = 0.05
+    cfg.physics.zeeman = 0.02
+    cfg.physics.alpha_2 = -0.1 * cfg.physics.g_C
+    cfg.laser.laser_type = "profile"
+    cfg.laser.profile = {"type": "ring", "radius": 15.0, "width": 3.0}
+    cfg.laser.P0 = 0.1
+    cfg.laser.circular_polarization = 0.5
+    cfg.reservoir.reservoir_type = "double"
+    cfg.solver.method = "ifrk-fft-hip"
+    cfg.solver.tableau = "dp5"
+    cfg.solver.adaptive = True
+    cfg.solver.rtol = 1e-6
+    cfg.solver.dt = 1.0
+    cfg.solver.total_time = float(ticks + 1)
+    cfg.solver.precision = "single"
+    cfg.noise.seed = 3
+    ctl = SimulationController(cfg)
+    sol = ctl.solver
+    sol.advance(ctl.model, ctl.state, 1)
+    sc = sol.step_controller
+    a0, r0 = sc.accepted, sc.rejected
+    sol.stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(ticks):
+        sol.advance(ctl.model, ctl.state, 1)
+    sol.stream.synchronize()
+    wall = time.perf_counter() - t0
+    trials = (sc.accepted - a0) + (sc.rejected - r0)
+    pts = 1024 * 1024 * 2
+    # per trial: 7 stages, each a forward and an inverse 2-D transform (2 passes each), plus
+    # the final combination's inverse (+ the error field's inverse) -> ~ (2 * 7 + 2) transforms
+    transforms = 2 * 7 + 2
+    return {"ticks": ticks, "trials": trials, "accepted": sc.accepted - a0, "rejected": sc.rejected - r0,
+            "ms_per_trial": round(wall * 1e3 / max(trials, 1), 3), "ps_per_s": round(ticks / wall, 3),
+            "dt_last_ps": sc.dt_last,
+            "fft_GBps_16_transforms_per_trial": round(transforms * 16.0 * pts * trials / wall / 1e9, 1)}
+
+
 def density_agreement(cold, warm):
     both = np.isfinite(cold.stop_times) & np.isfinite(warm.stop_times)
     errs = []
@@ -343,7 +391,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--members", type=int, default=SWEEP["members"])
-    ap.add_argument("--legs", default="sweep,copy,roofline,uneven,fft,cpu")
+    ap.add_argument("--legs", default="sweep,copy,roofline,uneven,fft,lawson,cpu")
     ap.add_argument("--weights", default=WEIGHTS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -397,6 +445,8 @@ def main():
         result["uneven_grid_sweep"] = uneven_leg()
     if "fft" in legs:
         result["split_step_fft"] = fft_leg()
+    if "lawson" in legs:
+        result["lawson_spinor_dp5"] = lawson_leg()
     result["config"] = {"workload": "steady-state pump sweep, 256 members of 256x256, rk4-fdm-hip complex64, "
                                     "single reservoir, dt 0.05 ps; warm start (FNO) vs plain cold start",
                         "members": len(members), "grid": "256x256", "lx_um": SWEEP["L"], "dt_ps": SWEEP["dt"],

@@ -116,8 +116,9 @@ def generate(members, dt=0.05, rtol=1e-6, max_time=4000.0, log=print):
     return features(members), y, conv, res
 
 
-def train(x, y, epochs=40, batch=16, lr=2e-3, seed=0, log=print, cfg=None, time_limit=None):
-    """Train on (6, N, H, W) / (3, N, H, W) arrays; returns (params as numpy dict, history)."""
+def train(x, y, epochs=40, batch=16, lr=2e-3, seed=0, log=print, cfg=None, time_limit=None, init=None):
+    """Train on (4, N, H, W) / (3, N, H, W) arrays, from ``init`` (a parameter dict) if given;
+    returns (params as numpy dict, history, cfg)."""
     import torch
     from .fno import FNOConfig, FNOModule
     cfg = cfg or FNOConfig(H=x.shape[2], W=x.shape[3])
@@ -127,6 +128,10 @@ def train(x, y, epochs=40, batch=16, lr=2e-3, seed=0, log=print, cfg=None, time_
     Y = torch.tensor(y, device=dev)
     n = X.shape[1]
     mod = FNOModule(cfg, seed=seed, device=dev)
+    if init is not None:
+        with torch.no_grad():
+            for k, v in init.items():
+                mod.params[k].copy_(torch.tensor(v))
     opt = torch.optim.AdamW(mod.parameters(), lr=lr, weight_decay=1e-5)
     steps_per_epoch = max(1, n // batch)
     total = epochs * steps_per_epoch
