@@ -321,9 +321,11 @@ def cpu_baseline(point_steps, converged, steps=400):
 
 
 def lawson_leg(ticks=5):
-    """Spinor Lawson IFRK, DP5 adaptive (rtol 1e-6, 1 ps ticks), 1024^2, lx = ly = 300 um, TE-TM 0.05,
+    """Spinor Lawson IFRK, DP5 adaptive (1 ps ticks), 1024^2, lx = ly = 300 um, TE-TM 0.05,
     Zeeman 0.02, alpha_2 = -0.1 g_C, elliptic ring pump (r 15 um, width 3 um), double reservoir,
-    complex64 fields with FP64 error norms: a window of ``ticks`` ps after one warm-up tick."""
+    complex64 fields with FP64 error norms: a window of ``ticks`` ps after one warm-up tick.
+    rtol is 1e-5, the smallest the reference's validation accepts in single precision
+    (validation.py:241-242); 1e-6 is accepted in double precision only."""
     from polarism_amd import Config, SimulationController
     from polarism_amd.config import GridParameters
     from polarism_amd.device import Event
@@ -341,7 +343,7 @@ def lawson_leg(ticks=5):
     cfg.solver.method = "ifrk-fft-hip"
     cfg.solver.tableau = "dp5"
     cfg.solver.adaptive = True
-    cfg.solver.rtol = 1e-6
+    cfg.solver.rtol = 1e-5
     cfg.solver.dt = 1.0
     cfg.solver.total_time = float(ticks + 1)
     cfg.solver.precision = "single"
@@ -362,7 +364,7 @@ def lawson_leg(ticks=5):
     # per trial: 7 stages, each a forward and an inverse 2-D transform (2 passes each), plus
     # the final combination's inverse (+ the error field's inverse) -> ~ (2 * 7 + 2) transforms
     transforms = 2 * 7 + 2
-    return {"ticks": ticks, "trials": trials, "accepted": sc.accepted - a0, "rejected": sc.rejected - r0,
+    return {"ticks": ticks, "rtol": 1e-5, "trials": trials, "accepted": sc.accepted - a0, "rejected": sc.rejected - r0,
             "ms_per_trial": round(wall * 1e3 / max(trials, 1), 3), "ps_per_s": round(ticks / wall, 3),
             "dt_last_ps": sc.dt_last,
             "fft_GBps_16_transforms_per_trial": round(transforms * 16.0 * pts * trials / wall / 1e9, 1)}

@@ -278,22 +278,38 @@ static void wavenumbers(int n, double d, std::vector<double>& k) {
     }
 }
 
+template <typename T, int V, int MODE, int RES>
+static int launch_rows_v(const pgpe_fft_plan* plan, const SplitParams<T>& p, hipStream_t s) {
+    constexpr int E = 256 * V;
+    const int rows_per = E / plan->nx;
+    const int total_rows = p.planes * p.ny;
+    const dim3 grid((total_rows + rows_per - 1) / rows_per);
+    hipLaunchKernelGGL((fft_rows_kernel<T, V, MODE, RES>), grid, dim3(256), (size_t)E * sizeof(c2<T>), s, p);
+    PGPE_LAUNCHED("fft_rows_kernel");
+    return PGPE_OK;
+}
+
 template <typename T, int MODE, int RES>
 static int launch_rows(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipStream_t s) {
     SplitParams<T> p = p0;
     p.dims = plan->dx_dims;
     p.tw = (const c2<T>*)plan->tw_x;
-    const int V = plan->vx;
-    const int E = 256 * V;
-    const int rows_per = E / plan->nx;
-    const int total_rows = p.planes * p.ny;
-    const dim3 grid((total_rows + rows_per - 1) / rows_per);
-    const size_t shm = (size_t)E * sizeof(c2<T>);
-    if (V == 16)
-        hipLaunchKernelGGL((fft_rows_kernel<T, 16, MODE, RES>), grid, dim3(256), shm, s, p);
-    else
-        hipLaunchKernelGGL((fft_rows_kernel<T, 12, MODE, RES>), grid, dim3(256), shm, s, p);
-    PGPE_LAUNCHED("fft_rows_kernel");
+    switch (plan->rv) {
+        case 4: return launch_rows_v<T, 4, MODE, RES>(plan, p, s);
+        case 8: return launch_rows_v<T, 8, MODE, RES>(plan, p, s);
+        case 12: return launch_rows_v<T, 12, MODE, RES>(plan, p, s);
+        default: return launch_rows_v<T, 16, MODE, RES>(plan, p, s);
+    }
+}
+
+template <typename T, int V, int MODE>
+static int launch_cols_v(const pgpe_fft_plan* plan, const SplitParams<T>& p, hipStream_t s) {
+    constexpr int E = 256 * V;
+    const int cols_per = E / (plan->ny + 1);
+    const int groups = (plan->nx + cols_per - 1) / cols_per;
+    hipLaunchKernelGGL((fft_cols_kernel<T, V, MODE>), dim3(groups * p.planes), dim3(256), (size_t)E * sizeof(c2<T>), s,
+                       p);
+    PGPE_LAUNCHED("fft_cols_kernel");
     return PGPE_OK;
 }
 
@@ -302,18 +318,12 @@ static int launch_cols(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipS
     SplitParams<T> p = p0;
     p.dims = plan->dy_dims;
     p.tw = (const c2<T>*)plan->tw_y;
-    const int V = plan->vy;
-    const int E = 256 * V;
-    const int cols_per = E / (plan->ny + 1);
-    const int groups = (plan->nx + cols_per - 1) / cols_per;
-    const dim3 grid(groups * p.planes);
-    const size_t shm = (size_t)E * sizeof(c2<T>);
-    if (V == 16)
-        hipLaunchKernelGGL((fft_cols_kernel<T, 16, MODE>), grid, dim3(256), shm, s, p);
-    else
-        hipLaunchKernelGGL((fft_cols_kernel<T, 12, MODE>), grid, dim3(256), shm, s, p);
-    PGPE_LAUNCHED("fft_cols_kernel");
-    return PGPE_OK;
+    switch (plan->cv) {
+        case 4: return launch_cols_v<T, 4, MODE>(plan, p, s);
+        case 8: return launch_cols_v<T, 8, MODE>(plan, p, s);
+        case 12: return launch_cols_v<T, 12, MODE>(plan, p, s);
+        default: return launch_cols_v<T, 16, MODE>(plan, p, s);
+    }
 }
 
 template <typename T>
@@ -412,6 +422,9 @@ int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pg
     plan->dy_dims = fy;
     plan->vx = vx;
     plan->vy = vy;
+    plan->rv = small_v(nx, nx);
+    plan->cv = small_v(ny, 2 * (ny + 1));
+    if (256 * plan->cv < ny + 1) plan->cv = vy;          // tall columns: the full group
     std::vector<double> kx, ky;
     wavenumbers(nx, dx, kx);
     wavenumbers(ny, dy, ky);

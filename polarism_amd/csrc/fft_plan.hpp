@@ -12,7 +12,10 @@ struct pgpe_fft_plan {
     void* tw_y;
     double* kx;   // 2 pi fftfreq(nx, dx)
     double* ky;
-    int vx, vy;   // elements per thread of the row / column kernels
+    int vx, vy;   // elements per thread of the Lawson row / column kernels (16, or 12 with a radix 3)
+    int rv, cv;   // elements per thread of the FFT / split-step row and column kernels, sized for
+                  // parallelism: rows take the fewest elements per thread that hold one row, columns
+                  // the fewest that hold two padded columns (more workgroups on small grids)
 };
 
 namespace pgpe {
@@ -33,6 +36,15 @@ inline bool factor(int n, FftDims& d, int& v) {
     if (threes) d.radix[d.npass++] = 3;
     v = threes ? 12 : 16;
     return n <= 256 * v && d.npass <= kFftMaxPasses;
+}
+
+// Elements per thread of a 256-thread group holding `need` elements: 4, 8, 16 (power-of-two
+// lengths; every pass needs V divisible by its radix) or 12 (lengths with a radix 3).
+inline int small_v(int n, int need) {
+    if (n % 3 == 0) return 12;
+    for (int v = 4; v <= 16; v *= 2)
+        if (need <= 256 * v) return v;
+    return 16;
 }
 
 

@@ -97,6 +97,14 @@ __device__ __forceinline__ void st_stream(T* p, T v) {
 #endif
 }
 template <typename T>
+__device__ __forceinline__ c2<T> ld_stream_c(const c2<T>* p) {
+#ifndef PGPE_TEMPORAL_STREAMS
+    return make_c<T>(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+#else
+    return *p;
+#endif
+}
+template <typename T>
 __device__ __forceinline__ void st_stream_c(c2<T>* p, const c2<T>& v) {
 #ifndef PGPE_TEMPORAL_STREAMS
     __builtin_nontemporal_store(v.x, &p->x);
@@ -221,24 +229,49 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
     const c2<T>* __restrict__ psi0 = p.psi0 + off;
 
     // ---- stage input with halo into LDS (periodic wrap or mirrored edges) ----
+    // Wavefront w loads the interior rows w, w + 4, ... it later computes (one point per
+    // lane, one coalesced 64-point segment per row) and keeps psi0 and k_prev of those rows
+    // in registers for the final combination; waves 0 / 1 load the halo rows above / below
+    // (lanes 0 and 1 also the corners), waves 2 / 3 the halo columns left / right.
+    constexpr int RPW = TH / 4;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int bc = mb.boundary;
-    for (int i = threadIdx.x; i < (TH + 2) * W; i += 256) {
-        const int ly = i / W;
-        const int lx = i - ly * W;
-        const int gy = map_index(y0 + ly - 1, ny, bc);
-        const int gx = map_index(x0 + lx - 1, nx, bc);
-        const int64_t g = (int64_t)gy * nx + gx;
-        c2<T> v = psi0[g];
-        if constexpr (KIND != 0) {
-            const c2<T> k = p.kprev[off + g];
-            v.x = v.x + p.a * k.x;
-            v.y = v.y + p.a * k.y;
+    c2<T> z[RPW], kp[RPW];
+    {
+        const int gxi = map_index(x0 + lane, nx, bc);
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) {
+            const int r = wave + 4 * q;
+            const int64_t g = (int64_t)map_index(y0 + r, ny, bc) * nx + gxi;
+            z[q] = psi0[g];
+            c2<T> v = z[q];
+            if constexpr (KIND != 0) {
+                kp[q] = p.kprev[off + g];
+                v.x = v.x + p.a * kp[q].x;
+                v.y = v.y + p.a * kp[q].y;
+            }
+            s_psi[(r + 1) * W + lane + 1] = v;
         }
-        s_psi[i] = v;
+        auto halo = [&](int ly, int lx) {
+            const int64_t g = (int64_t)map_index(y0 + ly, ny, bc) * nx + map_index(x0 + lx, nx, bc);
+            c2<T> v = psi0[g];
+            if constexpr (KIND != 0) {
+                const c2<T> k = p.kprev[off + g];
+                v.x = v.x + p.a * k.x;
+                v.y = v.y + p.a * k.y;
+            }
+            s_psi[(ly + 1) * W + lx + 1] = v;
+        };
+        if (wave < 2) {
+            const int ly = wave == 0 ? -1 : TH;
+            halo(ly, lane);
+            if (lane < 2) halo(ly, lane == 0 ? -1 : kTileW);
+        } else if (lane < TH) {
+            halo(lane, wave == 2 ? -1 : kTileW);
+        }
     }
     __syncthreads();
 
-    const int lane = threadIdx.x & 63;
     const int gx = x0 + lane;
     if (gx >= nx) return;
     const ScalarConsts<T> kc(mb);
@@ -249,16 +282,22 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
     uint64_t step = p.step;
     if (KIND == 2 && p.noise && p.step_base) step += *p.step_base;
 
-    for (int r = threadIdx.x >> 6; r < TH; r += 4) {
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        const int r = wave + 4 * q;
         const int gy = y0 + r;
         if (gy >= ny) break;
         const int64_t idx = (int64_t)gy * nx + gx;
         const int c = (r + 1) * W + lane + 1;
-        T n[2] = {(T)0, (T)0};
+        T n[2] = {(T)0, (T)0}, r0[2] = {(T)0, (T)0}, b3[2] = {(T)0, (T)0};
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            T v = ld_stream(&p.res0[f][off + idx]);
-            if constexpr (KIND != 0) v = v + p.a * ld_stream(&p.kres_prev[f][off + idx]);
+            r0[f] = ld_stream(&p.res0[f][off + idx]);
+            T v = r0[f];
+            if constexpr (KIND != 0) {
+                b3[f] = ld_stream(&p.kres_prev[f][off + idx]);
+                v = v + p.a * b3[f];
+            }
             n[f] = v;
         }
         const T P = ld_stream(&env[idx]) * coef;
@@ -278,21 +317,18 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
 #pragma unroll
             for (int f = 0; f < NF; ++f) st_stream(&p.kres_out[f][off + idx], dn[f]);
         } else {
-            const c2<T> a1 = p.k1[off + idx];
-            const c2<T> a2 = p.k2[off + idx];
-            const c2<T> a3 = p.kprev[off + idx];
-            const c2<T> z = psi0[idx];
+            const c2<T> a1 = ld_stream_c<T>(&p.k1[off + idx]);
+            const c2<T> a2 = ld_stream_c<T>(&p.k2[off + idx]);
+            const c2<T> a3 = kp[q];
             c2<T> out;
-            out.x = z.x + p.dt6 * (((a1.x + (T)2 * a2.x) + (T)2 * a3.x) + d.x);
-            out.y = z.y + p.dt6 * (((a1.y + (T)2 * a2.y) + (T)2 * a3.y) + d.y);
+            out.x = z[q].x + p.dt6 * (((a1.x + (T)2 * a2.x) + (T)2 * a3.x) + d.x);
+            out.y = z[q].y + p.dt6 * (((a1.y + (T)2 * a2.y) + (T)2 * a3.y) + d.y);
             T n_new[2] = {(T)0, (T)0};
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
-                const T r0 = p.res0[f][off + idx];
-                const T b1 = p.kres1[f][off + idx];
-                const T b2 = p.kres2[f][off + idx];
-                const T b3 = p.kres_prev[f][off + idx];
-                T v = r0 + p.dt6 * (((b1 + (T)2 * b2) + (T)2 * b3) + dn[f]);
+                const T b1 = ld_stream(&p.kres1[f][off + idx]);
+                const T b2 = ld_stream(&p.kres2[f][off + idx]);
+                T v = r0[f] + p.dt6 * (((b1 + (T)2 * b2) + (T)2 * b3[f]) + dn[f]);
                 v = v > (T)0 ? v : (T)0;  // set_state clamp (single_reservoir.py:70-73)
                 n_new[f] = v;
                 p.res_out[f][off + idx] = v;

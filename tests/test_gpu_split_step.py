@@ -86,3 +86,24 @@ def test_split_step_graph_equals_direct_launches_bitwise():
     a = _run(_cfg(noise__stochastic="wigner"))
     b = _run(_cfg(noise__stochastic="wigner", solver__cuda_graphs=False))
     np.testing.assert_array_equal(a.state.field.get(), b.state.field.get())
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("single", 1e-5)])
+def test_split_step_with_noise_config_window_matches_oracle(precision, tol):
+    """The split-step-with-noise configuration at full size (512^2, lx = ly = 200 um, Gaussian
+    sigma 20 um, P0 = 2 P_th, Wigner noise seed 1234, dt 5e-3 ps) over a 60-step window.
+
+    The psi tolerance is the reference's CPU-GPU bar (1e-10 FP64, 1e-5 FP32,
+    tests/crosscheck/test_cpu_gpu_agreement.py:12-13); the density error of a field is twice
+    its relative field error to first order, so the FP32 density bar is 2e-5 (240 FP32
+    transforms leave ~7e-6 on psi against the oracle's own FP32 transforms)."""
+    cfg = _cfg(grid__nx=512, grid__ny=512, grid__lx=200.0, grid__ly=200.0, laser__sigma_space=20.0,
+               laser__P0=2 * 0.02075, solver__precision=precision, solver__total_time=0.3,
+               noise__stochastic="wigner", noise__seed=1234)
+    ctl = _run(cfg)
+    psi_ref, res_ref = _oracle_split(cfg, 60, noise=True)
+    e = O.rel_l2(ctl.state.field.get(), psi_ref)
+    d = O.rel_l2(np.abs(ctl.state.field.get()) ** 2, np.abs(psi_ref) ** 2)
+    print(f"512^2 split-step + noise {precision}: psi {e:.3e} density {d:.3e}")
+    assert e <= tol and d <= 2 * tol
+    assert O.rel_l2(ctl.reservoir.get_state()[0].get(), res_ref[0]) <= tol
