@@ -122,11 +122,13 @@ __global__ void __launch_bounds__(256) fft_rows_kernel(const SplitParams<T> p) {
     c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
     constexpr int E = 256 * V;
     const int n = p.nx;
+    c2<T>* s_tw = buf + E;  // twiddles staged once per group: every pass reads them from LDS
     const int rows_per = E / n;
     const int total_rows = p.planes * p.ny;
     const int row0 = blockIdx.x * rows_per;
     const int count = min(rows_per, total_rows - row0);
     if (count <= 0) return;
+    for (int i = threadIdx.x; i < n; i += 256) s_tw[i] = p.tw[i];
     const int elems = count * n;
     const int64_t g0 = (int64_t)row0 * n;
     const bool res_here = RES != 0;
@@ -151,7 +153,7 @@ __global__ void __launch_bounds__(256) fft_rows_kernel(const SplitParams<T> p) {
     }
     __syncthreads();
     const bool first_inverse = (MODE == ROW_INV || MODE == ROW_SS_MID || MODE == ROW_SS_LAST || MODE == ROW_KO_MID);
-    lds_fft<T, V>(buf, p.dims, n, count, p.tw, first_inverse);
+    lds_fft<T, V>(buf, p.dims, n, count, s_tw, first_inverse);
 
     if constexpr (MODE == ROW_SS_MID || MODE == ROW_KO_MID) {
         // inverse done: scale, local physics, then forward again in place
@@ -182,7 +184,7 @@ __global__ void __launch_bounds__(256) fft_rows_kernel(const SplitParams<T> p) {
             buf[i] = v;
         }
         __syncthreads();
-        lds_fft<T, V>(buf, p.dims, n, count, p.tw, false);
+        lds_fft<T, V>(buf, p.dims, n, count, s_tw, false);
     }
 
     // ---- store (+ epilogue) ----
@@ -214,6 +216,7 @@ __global__ void __launch_bounds__(256) fft_cols_kernel(const SplitParams<T> p) {
     c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
     constexpr int E = 256 * V;
     const int n = p.ny;
+    c2<T>* s_tw = buf + E;  // twiddles staged once per group
     const int cols_per = E / (n + 1);  // padded columns must fit the E-element buffer
     const int groups_per_plane = (p.nx + cols_per - 1) / cols_per;
     const int plane = blockIdx.x / groups_per_plane;
@@ -222,6 +225,7 @@ __global__ void __launch_bounds__(256) fft_cols_kernel(const SplitParams<T> p) {
     const int count = min(cols_per, p.nx - x0);
     const int64_t base = (int64_t)plane * n * p.nx;
     const int ld = n + 1;  // one-element pad per column: strided LDS stores hit distinct banks
+    for (int i = threadIdx.x; i < n; i += 256) s_tw[i] = p.tw[i];
     // load column segments: element (y, x0 + c) -> buf[c * ld + y]
     for (int i = threadIdx.x; i < count * n; i += 256) {
         const int y = i / count;
@@ -229,7 +233,7 @@ __global__ void __launch_bounds__(256) fft_cols_kernel(const SplitParams<T> p) {
         buf[c * ld + y] = p.in[base + (int64_t)y * p.nx + x0 + c];
     }
     __syncthreads();
-    lds_fft<T, V>(buf, p.dims, ld, count, p.tw, MODE == COL_INV);
+    lds_fft<T, V>(buf, p.dims, ld, count, s_tw, MODE == COL_INV);
     if constexpr (MODE == COL_PROP) {
         // x exp(-i hbar k^2 h / (2 m)), evaluated in double as the reference does
         const pgpe_member& mb = p.table[plane];
@@ -249,7 +253,7 @@ __global__ void __launch_bounds__(256) fft_cols_kernel(const SplitParams<T> p) {
             buf[c * ld + y] = make_c<T>((T)re, (T)im);
         }
         __syncthreads();
-        lds_fft<T, V>(buf, p.dims, ld, count, p.tw, true);
+        lds_fft<T, V>(buf, p.dims, ld, count, s_tw, true);
     }
     for (int i = threadIdx.x; i < count * n; i += 256) {
         const int y = i / count;
@@ -284,7 +288,13 @@ static int launch_rows_v(const pgpe_fft_plan* plan, const SplitParams<T>& p, hip
     const int rows_per = E / plan->nx;
     const int total_rows = p.planes * p.ny;
     const dim3 grid((total_rows + rows_per - 1) / rows_per);
-    hipLaunchKernelGGL((fft_rows_kernel<T, V, MODE, RES>), grid, dim3(256), (size_t)E * sizeof(c2<T>), s, p);
+    const size_t shm = (size_t)(E + plan->nx) * sizeof(c2<T>);
+    auto kern = fft_rows_kernel<T, V, MODE, RES>;
+    if (shm > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(fft_rows_kernel)");
+    }
+    hipLaunchKernelGGL(kern, grid, dim3(256), shm, s, p);
     PGPE_LAUNCHED("fft_rows_kernel");
     return PGPE_OK;
 }
@@ -307,8 +317,13 @@ static int launch_cols_v(const pgpe_fft_plan* plan, const SplitParams<T>& p, hip
     constexpr int E = 256 * V;
     const int cols_per = E / (plan->ny + 1);
     const int groups = (plan->nx + cols_per - 1) / cols_per;
-    hipLaunchKernelGGL((fft_cols_kernel<T, V, MODE>), dim3(groups * p.planes), dim3(256), (size_t)E * sizeof(c2<T>), s,
-                       p);
+    const size_t shm = (size_t)(E + plan->ny) * sizeof(c2<T>);
+    auto kern = fft_cols_kernel<T, V, MODE>;
+    if (shm > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(fft_cols_kernel)");
+    }
+    hipLaunchKernelGGL(kern, dim3(groups * p.planes), dim3(256), shm, s, p);
     PGPE_LAUNCHED("fft_cols_kernel");
     return PGPE_OK;
 }

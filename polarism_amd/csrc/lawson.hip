@@ -163,6 +163,7 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
     c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
     constexpr int E = 256 * V;
     const int n = p.ny;
+    c2<T>* s_tw = buf + E;  // twiddles staged once per group: every pass reads them from LDS
     const int ld = n + 1;
     const int kinds = 1 + (MODE == LC_COMBINE ? p.lap : (MODE == LC_FINAL ? p.nerr : 0));
     const int seqs_per_col = kinds * p.comps;
@@ -176,6 +177,7 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
     const int64_t planes_total = (int64_t)p.members * p.comps;
     const pgpe_member& mb = p.table[m];
     auto seq = [&](int a, int c, int i) { return ((a * p.comps + c) * cnt + i) * ld; };
+    for (int i = threadIdx.x; i < n; i += 256) s_tw[i] = p.tw[i];
 
     if constexpr (MODE == LC_FWD) {
         for (int c = 0; c < p.comps; ++c) {
@@ -186,7 +188,7 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
             }
         }
         __syncthreads();
-        lds_fft<T, V>(buf, p.dims, ld, cnt * p.comps, p.tw, false);
+        lds_fft<T, V>(buf, p.dims, ld, cnt * p.comps, s_tw, false);
     } else {
         // build the combination for every (k point, component) of the group
         for (int e = threadIdx.x; e < cnt * n; e += 256) {
@@ -222,7 +224,7 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
             }
         }
         __syncthreads();
-        lds_fft<T, V>(buf, p.dims, ld, cnt * seqs_per_col, p.tw, true);
+        lds_fft<T, V>(buf, p.dims, ld, cnt * seqs_per_col, s_tw, true);
     }
     // store: main -> out, aux kinds -> aux
     for (int a = 0; a < kinds; ++a) {
@@ -250,6 +252,7 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
     c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
     constexpr int NF = RES == 0 ? 0 : (RES == 1 ? 1 : 2);
     const int n = p.nx;
+    c2<T>* s_tw = buf + 256 * V;  // twiddles staged once per group
     const int kinds = 1 + (MODE == LR_RHS ? p.lap : (MODE == LR_FINAL ? p.nerr : 0));
     const int groups_per_member = p.ny / rows_per;
     const int m = blockIdx.x / groups_per_member;
@@ -261,6 +264,7 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
     const pgpe_member& mb = p.table[m];
     auto seq = [&](int a, int c, int r) { return ((a * C + c) * rows_per + r) * n; };
     const int elems = rows_per * n;
+    for (int i = threadIdx.x; i < n; i += 256) s_tw[i] = p.tw[i];
 
     // ---- load ----
     for (int a = 0; a < kinds; ++a)
@@ -270,7 +274,7 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
             for (int e = threadIdx.x; e < elems; e += 256) buf[seq(a, c, 0) + e] = src[e];
         }
     __syncthreads();
-    if (MODE != LR_RHS_REAL) lds_fft<T, V>(buf, p.dims, n, rows_per * C * kinds, p.tw, true);
+    if (MODE != LR_RHS_REAL) lds_fft<T, V>(buf, p.dims, n, rows_per * C * kinds, s_tw, true);
 
     // ---- pointwise physics ----
     const T gC = (T)mb.g_C, gR = (T)mb.g_R, gI = (T)mb.g_I, Rr = (T)mb.R, gammaC = (T)mb.gamma_C;
@@ -406,7 +410,7 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
         return;
     }
     __syncthreads();
-    lds_fft<T, V>(buf, p.dims, n, rows_per * C, p.tw, false);
+    lds_fft<T, V>(buf, p.dims, n, rows_per * C, s_tw, false);
     for (int c = 0; c < C; ++c) {
         c2<T>* dst = p.out + (m * C + c) * plane + (int64_t)y0 * n;
         for (int e = threadIdx.x; e < elems; e += 256) dst[e] = buf[seq(0, c, 0) + e];
@@ -441,7 +445,7 @@ static int rows_per_group(int E, int nx, int ny, int seqs) {
 
 template <typename T, int V, int MODE>
 static int launch_cols_v(LawsonParams<T> p, int cols_per, hipStream_t s) {
-    const size_t shm = (size_t)256 * V * sizeof(c2<T>);
+    const size_t shm = (size_t)(256 * V + p.ny) * sizeof(c2<T>);
     auto kern = lawson_cols_kernel<T, V, MODE>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -454,7 +458,8 @@ static int launch_cols_v(LawsonParams<T> p, int cols_per, hipStream_t s) {
 }
 
 // The column group must hold kinds x comps padded columns; tall spinor grids
-// with error fields take the doubled buffer (V = 32 or 24, up to 128 KB LDS).
+// with error fields take the doubled buffer (V = 32 or 24; with the staged twiddles
+// the group stays within the 160 KB of LDS).
 template <typename T, int MODE>
 static int launch_cols(const pgpe_fft_plan* plan, LawsonParams<T> p, hipStream_t s) {
     p.dims = plan->dy_dims;
@@ -464,7 +469,7 @@ static int launch_cols(const pgpe_fft_plan* plan, LawsonParams<T> p, hipStream_t
     const int V = plan->vy;
     if (256 * V >= need) return V == 16 ? launch_cols_v<T, 16, MODE>(p, 256 * 16 / need, s)
                                         : launch_cols_v<T, 12, MODE>(p, 256 * 12 / need, s);
-    if (512 * V >= need && 512 * V * (int)sizeof(c2<T>) <= 131072)
+    if (512 * V >= need && (512 * V + plan->ny) * (int)sizeof(c2<T>) <= 163840)
         return V == 16 ? launch_cols_v<T, 32, MODE>(p, 256 * 32 / need, s)
                        : launch_cols_v<T, 24, MODE>(p, 256 * 24 / need, s);
     return fail(PGPE_ERR_UNSUPPORTED, "lawson: grid too tall for one LDS column group");
@@ -472,7 +477,7 @@ static int launch_cols(const pgpe_fft_plan* plan, LawsonParams<T> p, hipStream_t
 
 template <typename T, int V, int MODE, int RES>
 static int launch_rows_v(LawsonParams<T> p, int rows_per, hipStream_t s) {
-    const size_t shm = (size_t)256 * V * sizeof(c2<T>);
+    const size_t shm = (size_t)(256 * V + p.nx) * sizeof(c2<T>);
     auto kern = lawson_rows_kernel<T, V, MODE, RES>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -501,7 +506,7 @@ static int launch_rows(const pgpe_fft_plan* plan, LawsonParams<T> p, hipStream_t
         return rc;
     }
     rows_per = rows_per_group(512 * V, plan->nx, plan->ny, kinds * p.comps);
-    if (rows_per < 1 || 512 * V * (int)sizeof(c2<T>) > 131072)
+    if (rows_per < 1 || (512 * V + plan->nx) * (int)sizeof(c2<T>) > 163840)
         return fail(PGPE_ERR_UNSUPPORTED, "lawson: rows too long for one LDS row group");
     if (groups_per_member) *groups_per_member = plan->ny / rows_per;
     return V == 16 ? launch_rows_v<T, 32, MODE, RES>(p, rows_per, s) : launch_rows_v<T, 24, MODE, RES>(p, rows_per, s);
