@@ -320,6 +320,35 @@ def cpu_baseline(point_steps, converged, steps=400):
             "cpu_model": cpu_model, "affinity_cores": len(os.sched_getaffinity(0))}
 
 
+def fft_hbm_leg(n=2048, planes=16, reps=10):
+    """Batched forward 2-D FFT of ``planes`` complex64 n x n planes (n = 2048, 16 planes: 537 MB,
+    beyond the Infinity Cache): FFT GB/s with 2 x 8 B per point per 2-D transform (one read and
+    one write, the single-pass ideal) and the HBM bytes of the two-pass (rows, columns) kernels."""
+    import ctypes as C
+    from polarism_amd import _native as N
+    from polarism_amd.device import DeviceArray, Event, Stream
+    from polarism_amd.solvers.split_step_hip import FFTPlan
+    s = Stream()
+    plan = FFTPlan(n, n, 0.5, 0.5, "single")
+    rng = np.random.default_rng(1)
+    x = DeviceArray.from_numpy((rng.standard_normal(n * n * planes) + 0j).astype(np.complex64))
+    y = DeviceArray.empty((n * n * planes,), np.complex64)
+    run = lambda: N.call("pgpe_fft2", plan.handle, x.ptr, y.ptr, planes, 0, s.handle)
+    run()
+    s.synchronize()
+    start, stop = Event(), Event()
+    start.record(s)
+    for _ in range(reps):
+        run()
+    stop.record(s)
+    s.synchronize()
+    ms = start.elapsed_ms(stop) / reps
+    pts = float(n) * n * planes
+    ideal = 16.0 * pts / (ms * 1e-3) / 1e9
+    return {"n": n, "planes": planes, "transform_us": round(ms * 1e3, 1), "fft_GBps_single_pass_model": round(ideal, 1),
+            "frac_of_hbm_peak": round(ideal / HBM_PEAK_GBS, 4), "two_pass_GBps": round(2 * ideal, 1)}
+
+
 def lawson_leg(ticks=5):
     """Spinor Lawson IFRK, DP5 adaptive (1 ps ticks), 1024^2, lx = ly = 300 um, TE-TM 0.05,
     Zeeman 0.02, alpha_2 = -0.1 g_C, elliptic ring pump (r 15 um, width 3 um), double reservoir,
@@ -393,7 +422,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--members", type=int, default=SWEEP["members"])
-    ap.add_argument("--legs", default="sweep,copy,roofline,uneven,fft,lawson,cpu")
+    ap.add_argument("--legs", default="sweep,copy,roofline,uneven,fft,ffthbm,lawson,cpu")
     ap.add_argument("--weights", default=WEIGHTS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -447,6 +476,8 @@ def main():
         result["uneven_grid_sweep"] = uneven_leg()
     if "fft" in legs:
         result["split_step_fft"] = fft_leg()
+    if "ffthbm" in legs:
+        result["fft_batched_2048"] = fft_hbm_leg()
     if "lawson" in legs:
         result["lawson_spinor_dp5"] = lawson_leg()
     result["config"] = {"workload": "steady-state pump sweep, 256 members of 256x256, rk4-fdm-hip complex64, "

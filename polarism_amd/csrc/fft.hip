@@ -328,16 +328,42 @@ static int launch_cols_v(const pgpe_fft_plan* plan, const SplitParams<T>& p, hip
     return PGPE_OK;
 }
 
+// Column groups: as many columns per group as make 64-byte row segments (8 complex64 or
+// 4 complex128) when the batch has enough columns for >= 256 groups, fewer on small batches
+// (more groups instead); the smallest elements-per-thread count that holds them, within the
+// 160 KB of LDS together with the staged twiddles.
+template <typename T>
+static int column_v(const pgpe_fft_plan* plan, int planes) {
+    const int n = plan->ny;
+    if (n % 3 == 0) return 12;
+    const int cols_total = plan->nx * planes;
+    int want = sizeof(T) == 4 ? 8 : 4;
+    while (want > 1 && cols_total / want < 256) want /= 2;
+    int best = 0;
+    for (int v = 4; v <= 64; v *= 2) {
+        const size_t shm = (size_t)(256 * v + n) * sizeof(c2<T>);
+        if (shm > 163840 || 256 * v < n + 1) continue;
+        best = v;
+        if (256 * v >= want * (n + 1)) break;
+    }
+    return best;
+}
+
 template <typename T, int MODE>
 static int launch_cols(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipStream_t s) {
     SplitParams<T> p = p0;
     p.dims = plan->dy_dims;
     p.tw = (const c2<T>*)plan->tw_y;
-    switch (plan->cv) {
+    switch (column_v<T>(plan, p.planes)) {
         case 4: return launch_cols_v<T, 4, MODE>(plan, p, s);
         case 8: return launch_cols_v<T, 8, MODE>(plan, p, s);
         case 12: return launch_cols_v<T, 12, MODE>(plan, p, s);
-        default: return launch_cols_v<T, 16, MODE>(plan, p, s);
+        case 16: return launch_cols_v<T, 16, MODE>(plan, p, s);
+        case 32: return launch_cols_v<T, 32, MODE>(plan, p, s);
+        case 64:
+            if constexpr (sizeof(T) == 4) return launch_cols_v<T, 64, MODE>(plan, p, s);
+            [[fallthrough]];
+        default: return fail(PGPE_ERR_UNSUPPORTED, "fft: column too tall for one LDS group");
     }
 }
 
@@ -438,8 +464,6 @@ int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pg
     plan->vx = vx;
     plan->vy = vy;
     plan->rv = small_v(nx, nx);
-    plan->cv = small_v(ny, 2 * (ny + 1));
-    if (256 * plan->cv < ny + 1) plan->cv = vy;          // tall columns: the full group
     std::vector<double> kx, ky;
     wavenumbers(nx, dx, kx);
     wavenumbers(ny, dy, ky);

@@ -13,9 +13,8 @@ struct pgpe_fft_plan {
     double* kx;   // 2 pi fftfreq(nx, dx)
     double* ky;
     int vx, vy;   // elements per thread of the Lawson row / column kernels (16, or 12 with a radix 3)
-    int rv, cv;   // elements per thread of the FFT / split-step row and column kernels, sized for
-                  // parallelism: rows take the fewest elements per thread that hold one row, columns
-                  // the fewest that hold two padded columns (more workgroups on small grids)
+    int rv;       // elements per thread of the FFT / split-step row kernels: the fewest that hold
+                  // one row (more workgroups on small grids); column groups are sized per launch
 };
 
 namespace pgpe {

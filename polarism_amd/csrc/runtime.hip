@@ -62,9 +62,8 @@ int pgpe_device_init(int device) {
 
 int pgpe_device_name(int device, char* buf, size_t len) {
     if (!buf || len == 0) return fail(PGPE_ERR_ARG, "pgpe_device_name: empty buffer");
-    hipDeviceProp_t prop;
-    PGPE_HIP(hipGetDeviceProperties(&prop, device));
-    std::snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+    PGPE_HIP(hipDeviceGetName(buf, (int)(len > 0x7fffffff ? 0x7fffffff : len), device));
+    buf[len - 1] = '\0';
     return PGPE_OK;
 }
 

@@ -61,3 +61,22 @@ def test_plan_rejects_unsupported_size():
     init()
     with pytest.raises(RuntimeError, match="sizes"):
         _plan(100, 64, "double")
+
+
+@pytest.mark.parametrize("ny,nx,B", [(2048, 16, 40), (2048, 64, 64), (1024, 32, 96)])
+@pytest.mark.parametrize("precision,tol", [("double", 1e-14), ("single", 2e-6)])
+def test_fft2_tall_columns_in_large_batches(ny, nx, B, precision, tol):
+    """Large batches take wide column groups (up to 64 elements per thread, 64-byte row segments)."""
+    from polarism_amd import _native as N
+    from polarism_amd.device import DeviceArray, init
+    init()
+    rng = np.random.default_rng(ny + nx + B)
+    cdt = np.complex64 if precision == "single" else np.complex128
+    x = (rng.standard_normal((B, ny, nx)) + 1j * rng.standard_normal((B, ny, nx))).astype(cdt)
+    plan = _plan(nx, ny, precision)
+    d_in = DeviceArray.from_numpy(x)
+    d_out = DeviceArray.empty(x.shape, cdt)
+    N.call("pgpe_fft2", plan.handle, d_in.ptr, d_out.ptr, B, 0, None)
+    ref = np.fft.fft2(x.astype(np.complex128), axes=(-2, -1))
+    err = np.linalg.norm(d_out.get() - ref) / np.linalg.norm(ref)
+    assert err < tol, err
