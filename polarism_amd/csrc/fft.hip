@@ -122,13 +122,19 @@ __global__ void __launch_bounds__(256) fft_rows_kernel(const SplitParams<T> p) {
     c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
     constexpr int E = 256 * V;
     const int n = p.nx;
-    c2<T>* s_tw = buf + E;  // twiddles staged once per group: every pass reads them from LDS
     const int rows_per = E / n;
+    const int ld = seq_stride<T>(n);
+    c2<T>* s_tw = buf + rows_per * ld;  // twiddles staged once per group: every pass reads them from LDS
+    const FastDiv by_n((uint32_t)n);
+    auto slot = [&](int i) {  // LDS slot of the group's i-th element (row i / n, column i % n)
+        const int r = by_n.div(i);
+        return r * ld + pad_at(i - r * n);
+    };
     const int total_rows = p.planes * p.ny;
     const int row0 = blockIdx.x * rows_per;
     const int count = min(rows_per, total_rows - row0);
     if (count <= 0) return;
-    for (int i = threadIdx.x; i < n; i += 256) s_tw[i] = p.tw[i];
+    stage_twiddles<T>(s_tw, p.tw, n);
     const int elems = count * n;
     const int64_t g0 = (int64_t)row0 * n;
     const bool res_here = RES != 0;
@@ -136,76 +142,121 @@ __global__ void __launch_bounds__(256) fft_rows_kernel(const SplitParams<T> p) {
     if (p.step_base) step += *p.step_base;
 
     // ---- load (+ prologue) ----
-    for (int i = threadIdx.x; i < elems; i += 256) {
-        c2<T> v = p.in[g0 + i];
-        if constexpr (MODE == ROW_SS_FIRST) {
-            const int row = row0 + i / n;
-            const int m = row / p.ny;
-            const int64_t cell = (int64_t)(row - m * p.ny) * n + (i % n);
-            const pgpe_member& mb = p.table[m];
-            T nn[2] = {0, 0};
-            load_res<T, RES>(p, g0 + i, nn);
-            const T P = p.pump[mb.pump_off + cell] * p.coef[m];
-            res_half_step<T, RES>(mb, v.x * v.x + v.y * v.y, P, p.dt / (T)2, nn);
-            if (res_here) store_res<T, RES>(p, g0 + i, nn);
-        }
-        buf[i] = v;
+    constexpr int CH = stage_chunk<T, V>();
+    if constexpr (MODE == ROW_SS_FIRST) {
+        struct Pt { c2<T> v; T n[2]; T P; };
+        staged<Pt, CH>(
+            elems,
+            [&](int i) {
+                Pt q;
+                q.v = p.in[g0 + i];
+                const int r = by_n.div(i);
+                const int row = row0 + r;
+                const int m = row / p.ny;
+                const int64_t cell = (int64_t)(row - m * p.ny) * n + (i - r * n);
+                q.n[0] = q.n[1] = (T)0;
+                load_res<T, RES>(p, g0 + i, q.n);
+                q.P = p.pump[p.table[m].pump_off + cell] * p.coef[m];
+                return q;
+            },
+            [&](int i, Pt q) {
+                const int m = (row0 + by_n.div(i)) / p.ny;
+                res_half_step<T, RES>(p.table[m], q.v.x * q.v.x + q.v.y * q.v.y, q.P, p.dt / (T)2, q.n);
+                if (res_here) store_res<T, RES>(p, g0 + i, q.n);
+                buf[slot(i)] = q.v;
+            });
+    } else {
+        staged<c2<T>, CH>(elems, [&](int i) { return p.in[g0 + i]; }, [&](int i, c2<T> v) { buf[slot(i)] = v; });
     }
     __syncthreads();
     const bool first_inverse = (MODE == ROW_INV || MODE == ROW_SS_MID || MODE == ROW_SS_LAST || MODE == ROW_KO_MID);
-    lds_fft<T, V>(buf, p.dims, n, count, s_tw, first_inverse);
+    lds_fft<T, V>(buf, p.dims, ld, count, s_tw, first_inverse);
 
     if constexpr (MODE == ROW_SS_MID || MODE == ROW_KO_MID) {
         // inverse done: scale, local physics, then forward again in place
-        for (int i = threadIdx.x; i < elems; i += 256) {
-            c2<T> v = buf[i];
-            v.x *= p.scale;
-            v.y *= p.scale;
-            const int row = row0 + i / n;
-            const int m = row / p.ny;
-            const int64_t cell = (int64_t)(row - m * p.ny) * n + (i % n);
-            const pgpe_member& mb = p.table[m];
-            T nn[2] = {0, 0};
-            load_res<T, RES>(p, g0 + i, nn);
-            if constexpr (MODE == ROW_KO_MID) {
-                const T P1 = p.pump[mb.pump_off + cell] * p.coef[m];
-                res_half_step<T, RES>(mb, v.x * v.x + v.y * v.y, P1, p.dt / (T)2, nn);
-                nonlinear_step<T>(mb, RES ? nn[0] : (T)0, RES >= 2 ? nn[1] : (T)0, p.dt, v);
-                const T P2 = p.pump[mb.pump_off + cell] * p.coef[p.members + m];
-                res_half_step<T, RES>(mb, v.x * v.x + v.y * v.y, P2, p.dt / (T)2, nn);
-                if (res_here) store_res<T, RES>(p, g0 + i, nn);
-            } else {
-                nonlinear_step<T>(mb, RES ? nn[0] : (T)0, RES >= 2 ? nn[1] : (T)0, p.dt, v);
-            }
-            if (p.noise) {
-                wigner_site<T>(v.x, v.y, mb.seed, (uint32_t)cell, step, 1u, (T)mb.noise_strength, (T)mb.noise_gamma,
-                               (T)mb.noise_R, RES ? nn[0] : (T)0, (T)mb.noise_dt_over_4v);
-            }
-            buf[i] = v;
-        }
+        struct Pt { c2<T> v; T n[2]; T P1, P2; };
+        staged<Pt, CH>(
+            elems,
+            [&](int i) {
+                Pt q;
+                q.v = buf[slot(i)];
+                q.n[0] = q.n[1] = (T)0;
+                load_res<T, RES>(p, g0 + i, q.n);
+                if constexpr (MODE == ROW_KO_MID) {
+                    const int r = by_n.div(i);
+                    const int row = row0 + r;
+                    const int m = row / p.ny;
+                    const int64_t cell = (int64_t)(row - m * p.ny) * n + (i - r * n);
+                    const T env = p.pump[p.table[m].pump_off + cell];
+                    q.P1 = env * p.coef[m];
+                    q.P2 = env * p.coef[p.members + m];
+                }
+                return q;
+            },
+            [&](int i, Pt q) {
+                c2<T> v = q.v;
+                v.x *= p.scale;
+                v.y *= p.scale;
+                const int r = by_n.div(i);
+                const int row = row0 + r;
+                const int m = row / p.ny;
+                const int64_t cell = (int64_t)(row - m * p.ny) * n + (i - r * n);
+                const pgpe_member& mb = p.table[m];
+                T* nn = q.n;
+                if constexpr (MODE == ROW_KO_MID) {
+                    res_half_step<T, RES>(mb, v.x * v.x + v.y * v.y, q.P1, p.dt / (T)2, nn);
+                    nonlinear_step<T>(mb, RES ? nn[0] : (T)0, RES >= 2 ? nn[1] : (T)0, p.dt, v);
+                    res_half_step<T, RES>(mb, v.x * v.x + v.y * v.y, q.P2, p.dt / (T)2, nn);
+                    if (res_here) store_res<T, RES>(p, g0 + i, nn);
+                } else {
+                    nonlinear_step<T>(mb, RES ? nn[0] : (T)0, RES >= 2 ? nn[1] : (T)0, p.dt, v);
+                }
+                if (p.noise) {
+                    wigner_site<T>(v.x, v.y, mb.seed, (uint32_t)cell, step, 1u, (T)mb.noise_strength,
+                                   (T)mb.noise_gamma, (T)mb.noise_R, RES ? nn[0] : (T)0, (T)mb.noise_dt_over_4v);
+                }
+                buf[slot(i)] = v;
+            });
         __syncthreads();
-        lds_fft<T, V>(buf, p.dims, n, count, s_tw, false);
+        lds_fft<T, V>(buf, p.dims, ld, count, s_tw, false);
     }
 
     // ---- store (+ epilogue) ----
-    for (int i = threadIdx.x; i < elems; i += 256) {
-        c2<T> v = buf[i];
-        if constexpr (MODE == ROW_INV || MODE == ROW_SS_LAST) {
-            v.x *= p.scale;
-            v.y *= p.scale;
-        }
-        if constexpr (MODE == ROW_SS_LAST) {
-            const int row = row0 + i / n;
-            const int m = row / p.ny;
-            const int64_t cell = (int64_t)(row - m * p.ny) * n + (i % n);
-            const pgpe_member& mb = p.table[m];
-            T nn[2] = {0, 0};
-            load_res<T, RES>(p, g0 + i, nn);
-            const T P = p.pump[mb.pump_off + cell] * p.coef[p.members + m];
-            res_half_step<T, RES>(mb, v.x * v.x + v.y * v.y, P, p.dt / (T)2, nn);
-            if (res_here) store_res<T, RES>(p, g0 + i, nn);
-        }
-        p.out[g0 + i] = v;
+    if constexpr (MODE == ROW_SS_LAST) {
+        struct Pt { c2<T> v; T n[2]; T P; };
+        staged<Pt, CH>(
+            elems,
+            [&](int i) {
+                Pt q;
+                q.v = buf[slot(i)];
+                const int r = by_n.div(i);
+                const int row = row0 + r;
+                const int m = row / p.ny;
+                const int64_t cell = (int64_t)(row - m * p.ny) * n + (i - r * n);
+                q.n[0] = q.n[1] = (T)0;
+                load_res<T, RES>(p, g0 + i, q.n);
+                q.P = p.pump[p.table[m].pump_off + cell] * p.coef[p.members + m];
+                return q;
+            },
+            [&](int i, Pt q) {
+                c2<T> v = q.v;
+                v.x *= p.scale;
+                v.y *= p.scale;
+                const int m = (row0 + by_n.div(i)) / p.ny;
+                res_half_step<T, RES>(p.table[m], v.x * v.x + v.y * v.y, q.P, p.dt / (T)2, q.n);
+                if (res_here) store_res<T, RES>(p, g0 + i, q.n);
+                p.out[g0 + i] = v;
+            });
+    } else {
+        staged<c2<T>, CH>(
+            elems, [&](int i) { return buf[slot(i)]; },
+            [&](int i, c2<T> v) {
+                if constexpr (MODE == ROW_INV) {
+                    v.x *= p.scale;
+                    v.y *= p.scale;
+                }
+                p.out[g0 + i] = v;
+            });
     }
 }
 
@@ -216,22 +267,31 @@ __global__ void __launch_bounds__(256) fft_cols_kernel(const SplitParams<T> p) {
     c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
     constexpr int E = 256 * V;
     const int n = p.ny;
-    c2<T>* s_tw = buf + E;  // twiddles staged once per group
-    const int cols_per = E / (n + 1);  // padded columns must fit the E-element buffer
+    const int cols_per = E / n;
+    const int ld = seq_stride<T>(n);
+    c2<T>* s_tw = buf + cols_per * ld;  // twiddles staged once per group
     const int groups_per_plane = (p.nx + cols_per - 1) / cols_per;
-    const int plane = blockIdx.x / groups_per_plane;
-    if (plane >= p.planes) return;
-    const int x0 = (blockIdx.x - plane * groups_per_plane) * cols_per;
+    // XCD-aware: the groups one XCD runs side by side are neighbours along x, so the row
+    // segments they share come through the same L2
+    const int g = xcd_tile(blockIdx.x, groups_per_plane * p.planes);
+    if (g < 0) return;
+    const int plane = g / groups_per_plane;
+    const int x0 = (g - plane * groups_per_plane) * cols_per;
     const int count = min(cols_per, p.nx - x0);
     const int64_t base = (int64_t)plane * n * p.nx;
-    const int ld = n + 1;  // one-element pad per column: strided LDS stores hit distinct banks
-    for (int i = threadIdx.x; i < n; i += 256) s_tw[i] = p.tw[i];
+    const FastDiv by_count((uint32_t)count), by_n((uint32_t)n);
+    stage_twiddles<T>(s_tw, p.tw, n);
     // load column segments: element (y, x0 + c) -> buf[c * ld + y]
-    for (int i = threadIdx.x; i < count * n; i += 256) {
-        const int y = i / count;
-        const int c = i - y * count;
-        buf[c * ld + y] = p.in[base + (int64_t)y * p.nx + x0 + c];
-    }
+    constexpr int CH = stage_chunk<T, V>();
+    struct Pt { c2<T> v; int at; };
+    staged<Pt, CH>(
+        count * n,
+        [&](int i) {
+            const int y = by_count.div(i);
+            const int c = i - y * count;
+            return Pt{p.in[base + (int64_t)y * p.nx + x0 + c], c * ld + pad_at(y)};
+        },
+        [&](int, Pt q) { buf[q.at] = q.v; });
     __syncthreads();
     lds_fft<T, V>(buf, p.dims, ld, count, s_tw, MODE == COL_INV);
     if constexpr (MODE == COL_PROP) {
@@ -239,27 +299,32 @@ __global__ void __launch_bounds__(256) fft_cols_kernel(const SplitParams<T> p) {
         const pgpe_member& mb = p.table[plane];
         const double hbar = mb.hbar;
         const double inv2m = 1.0 / (2 * mb.m_eff);
-        for (int i = threadIdx.x; i < count * n; i += 256) {
-            const int c = i / n;
-            const int y = i - c * n;
-            const double kxv = p.kx[x0 + c], kyv = p.ky[y];
-            const double k2 = kxv * kxv + kyv * kyv;
-            const double theta = ((-(hbar * k2)) * p.prop_h) * inv2m;
-            double s, co;
-            sincos(theta, &s, &co);
-            const c2<T> v = buf[c * ld + y];
-            const double re = (double)v.x * co - (double)v.y * s;
-            const double im = (double)v.x * s + (double)v.y * co;
-            buf[c * ld + y] = make_c<T>((T)re, (T)im);
-        }
+        struct K { double kx, ky; };
+        staged<K, CH>(
+            count * n, [&](int i) { const int c = by_n.div(i); return K{p.kx[x0 + c], p.ky[i - c * n]}; },
+            [&](int i, K q) {
+                const int c = by_n.div(i);
+                const int y = i - c * n;
+                const double k2 = q.kx * q.kx + q.ky * q.ky;
+                const double theta = ((-(hbar * k2)) * p.prop_h) * inv2m;
+                double s, co;
+                sincos(theta, &s, &co);
+                const c2<T> v = buf[c * ld + pad_at(y)];
+                const double re = (double)v.x * co - (double)v.y * s;
+                const double im = (double)v.x * s + (double)v.y * co;
+                buf[c * ld + pad_at(y)] = make_c<T>((T)re, (T)im);
+            });
         __syncthreads();
         lds_fft<T, V>(buf, p.dims, ld, count, s_tw, true);
     }
-    for (int i = threadIdx.x; i < count * n; i += 256) {
-        const int y = i / count;
-        const int c = i - y * count;
-        p.out[base + (int64_t)y * p.nx + x0 + c] = buf[c * ld + y];
-    }
+    staged<Pt, CH>(
+        count * n,
+        [&](int i) {
+            const int y = by_count.div(i);
+            const int c = i - y * count;
+            return Pt{buf[c * ld + pad_at(y)], y * p.nx + x0 + c};
+        },
+        [&](int, Pt q) { p.out[base + q.at] = q.v; });
 }
 
 template <typename T>
@@ -283,12 +348,15 @@ static void wavenumbers(int n, double d, std::vector<double>& k) {
 }
 
 template <typename T, int V, int MODE, int RES>
-static int launch_rows_v(const pgpe_fft_plan* plan, const SplitParams<T>& p, hipStream_t s) {
+static int launch_rows_v(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipStream_t s) {
     constexpr int E = 256 * V;
+    SplitParams<T> p = p0;
+    p.dims = plan->dims_x[radix_class<V>()];
+    p.tw = (const c2<T>*)plan->tw_x[radix_class<V>()];
     const int rows_per = E / plan->nx;
     const int total_rows = p.planes * p.ny;
     const dim3 grid((total_rows + rows_per - 1) / rows_per);
-    const size_t shm = (size_t)(E + plan->nx) * sizeof(c2<T>);
+    const size_t shm = (size_t)(rows_per * seq_stride<T>(plan->nx) + plan->nx) * sizeof(c2<T>);
     auto kern = fft_rows_kernel<T, V, MODE, RES>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -301,9 +369,7 @@ static int launch_rows_v(const pgpe_fft_plan* plan, const SplitParams<T>& p, hip
 
 template <typename T, int MODE, int RES>
 static int launch_rows(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipStream_t s) {
-    SplitParams<T> p = p0;
-    p.dims = plan->dx_dims;
-    p.tw = (const c2<T>*)plan->tw_x;
+    const SplitParams<T>& p = p0;
     switch (plan->rv) {
         case 4: return launch_rows_v<T, 4, MODE, RES>(plan, p, s);
         case 8: return launch_rows_v<T, 8, MODE, RES>(plan, p, s);
@@ -313,25 +379,28 @@ static int launch_rows(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipS
 }
 
 template <typename T, int V, int MODE>
-static int launch_cols_v(const pgpe_fft_plan* plan, const SplitParams<T>& p, hipStream_t s) {
+static int launch_cols_v(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipStream_t s) {
     constexpr int E = 256 * V;
-    const int cols_per = E / (plan->ny + 1);
+    SplitParams<T> p = p0;
+    p.dims = plan->dims_y[radix_class<V>()];
+    p.tw = (const c2<T>*)plan->tw_y[radix_class<V>()];
+    const int cols_per = E / plan->ny;
     const int groups = (plan->nx + cols_per - 1) / cols_per;
-    const size_t shm = (size_t)(E + plan->ny) * sizeof(c2<T>);
+    const size_t shm = (size_t)(cols_per * seq_stride<T>(plan->ny) + plan->ny) * sizeof(c2<T>);
     auto kern = fft_cols_kernel<T, V, MODE>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(fft_cols_kernel)");
     }
-    hipLaunchKernelGGL(kern, dim3(groups * p.planes), dim3(256), shm, s, p);
+    hipLaunchKernelGGL(kern, dim3(xcd_grid(groups * p.planes)), dim3(256), shm, s, p);
     PGPE_LAUNCHED("fft_cols_kernel");
     return PGPE_OK;
 }
 
 // Column groups: as many columns per group as make 64-byte row segments (8 complex64 or
 // 4 complex128) when the batch has enough columns for >= 256 groups, fewer on small batches
-// (more groups instead); the smallest elements-per-thread count that holds them, within the
-// 160 KB of LDS together with the staged twiddles.
+// (more groups instead); the smallest elements-per-thread count that holds them, with the
+// padded columns and the staged twiddles within the 160 KB of LDS.
 template <typename T>
 static int column_v(const pgpe_fft_plan* plan, int planes) {
     const int n = plan->ny;
@@ -341,19 +410,19 @@ static int column_v(const pgpe_fft_plan* plan, int planes) {
     while (want > 1 && cols_total / want < 256) want /= 2;
     int best = 0;
     for (int v = 4; v <= 64; v *= 2) {
-        const size_t shm = (size_t)(256 * v + n) * sizeof(c2<T>);
-        if (shm > 163840 || 256 * v < n + 1) continue;
+        const int cols = 256 * v / n;
+        if (cols < 1 || (v == 64 && n < 1024)) continue;
+        const size_t shm = (size_t)(cols * seq_stride<T>(n) + n) * sizeof(c2<T>);
+        if (shm > 163840) break;
         best = v;
-        if (256 * v >= want * (n + 1)) break;
+        if (cols >= want) break;
     }
     return best;
 }
 
 template <typename T, int MODE>
 static int launch_cols(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipStream_t s) {
-    SplitParams<T> p = p0;
-    p.dims = plan->dy_dims;
-    p.tw = (const c2<T>*)plan->tw_y;
+    const SplitParams<T>& p = p0;
     switch (column_v<T>(plan, p.planes)) {
         case 4: return launch_cols_v<T, 4, MODE>(plan, p, s);
         case 8: return launch_cols_v<T, 8, MODE>(plan, p, s);
@@ -448,9 +517,8 @@ int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pg
     if (!out) return fail(PGPE_ERR_ARG, "pgpe_fft_plan_create: null output");
     *out = nullptr;
     if (precision != 0 && precision != 1) return fail(PGPE_ERR_ARG, "pgpe_fft_plan_create: precision must be 0 or 1");
-    FftDims fx{}, fy{};
     int vx = 0, vy = 0;
-    if (!factor(nx, fx, vx) || !factor(ny, fy, vy))
+    if (!check_length(nx, vx) || !check_length(ny, vy))
         return fail(PGPE_ERR_UNSUPPORTED, "pgpe_fft_plan_create: sizes must be 2^a or 3*2^a with 2 <= n <= 3072");
     pgpe_fft_plan* plan = new (std::nothrow) pgpe_fft_plan{};
     if (!plan) return fail(PGPE_ERR_ARG, "pgpe_fft_plan_create: out of host memory");
@@ -459,42 +527,54 @@ int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pg
     plan->dx = dx;
     plan->dy = dy;
     plan->precision = precision;
-    plan->dx_dims = fx;
-    plan->dy_dims = fy;
     plan->vx = vx;
     plan->vy = vy;
     plan->rv = small_v(nx, nx);
+    // pass lists: x classes 0..2, then y classes 0..2, in one device block
+    std::vector<FftPass> lists[6], all;
+    for (int c = 0; c < 3; ++c) {
+        build_passes(nx, c, lists[c]);
+        build_passes(ny, c, lists[3 + c]);
+    }
+    int offs[6];
+    for (int k = 0; k < 6; ++k) {
+        offs[k] = (int)all.size();
+        all.insert(all.end(), lists[k].begin(), lists[k].end());
+    }
     std::vector<double> kx, ky;
     wavenumbers(nx, dx, kx);
     wavenumbers(ny, dy, ky);
     const size_t csz = precision ? sizeof(double2) : sizeof(float2);
-    hipError_t e = hipMalloc(&plan->tw_x, csz * nx);
-    if (e == hipSuccess) e = hipMalloc(&plan->tw_y, csz * ny);
-    if (e == hipSuccess) e = hipMalloc((void**)&plan->kx, sizeof(double) * nx);
-    if (e == hipSuccess) e = hipMalloc((void**)&plan->ky, sizeof(double) * ny);
-    if (e == hipSuccess) {
+    hipError_t e = hipMalloc((void**)&plan->passes, sizeof(FftPass) * all.size());
+    if (e == hipSuccess) e = hipMemcpy(plan->passes, all.data(), sizeof(FftPass) * all.size(), hipMemcpyHostToDevice);
+    for (int k = 0; k < 6 && e == hipSuccess; ++k) {
+        const int n = k < 3 ? nx : ny;
+        FftDims& d = k < 3 ? plan->dims_x[k] : plan->dims_y[k - 3];
+        d.n = n;
+        d.npass = (int)lists[k].size();
+        d.passes = plan->passes + offs[k];
+        void** tw = k < 3 ? &plan->tw_x[k] : &plan->tw_y[k - 3];
+        const size_t slots = n > 1 ? n - 1 : 1;
+        e = hipMalloc(tw, csz * slots);
+        if (e != hipSuccess) break;
         if (precision) {
-            std::vector<double2> tx, ty;
-            twiddles<double>(nx, tx);
-            twiddles<double>(ny, ty);
-            e = hipMemcpy(plan->tw_x, tx.data(), csz * nx, hipMemcpyHostToDevice);
-            if (e == hipSuccess) e = hipMemcpy(plan->tw_y, ty.data(), csz * ny, hipMemcpyHostToDevice);
+            std::vector<double2> root, tab;
+            twiddles<double>(n, root);
+            pass_twiddles(n, lists[k], root, tab);
+            e = hipMemcpy(*tw, tab.data(), csz * slots, hipMemcpyHostToDevice);
         } else {
-            std::vector<float2> tx, ty;
-            twiddles<float>(nx, tx);
-            twiddles<float>(ny, ty);
-            e = hipMemcpy(plan->tw_x, tx.data(), csz * nx, hipMemcpyHostToDevice);
-            if (e == hipSuccess) e = hipMemcpy(plan->tw_y, ty.data(), csz * ny, hipMemcpyHostToDevice);
+            std::vector<float2> root, tab;
+            twiddles<float>(n, root);
+            pass_twiddles(n, lists[k], root, tab);
+            e = hipMemcpy(*tw, tab.data(), csz * slots, hipMemcpyHostToDevice);
         }
     }
+    if (e == hipSuccess) e = hipMalloc((void**)&plan->kx, sizeof(double) * nx);
+    if (e == hipSuccess) e = hipMalloc((void**)&plan->ky, sizeof(double) * ny);
     if (e == hipSuccess) e = hipMemcpy(plan->kx, kx.data(), sizeof(double) * nx, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(plan->ky, ky.data(), sizeof(double) * ny, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        hipFree(plan->tw_x);
-        hipFree(plan->tw_y);
-        hipFree(plan->kx);
-        hipFree(plan->ky);
-        delete plan;
+        pgpe_fft_plan_destroy(plan);  // the first error is the one reported
         return hip_fail(e, "pgpe_fft_plan_create");
     }
     *out = plan;
@@ -503,12 +583,13 @@ int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pg
 
 int pgpe_fft_plan_destroy(pgpe_fft_plan* plan) {
     if (!plan) return PGPE_OK;
-    hipFree(plan->tw_x);
-    hipFree(plan->tw_y);
-    hipFree(plan->kx);
-    hipFree(plan->ky);
+    hipError_t e = hipSuccess, f;
+    void* ptrs[] = {plan->tw_x[0], plan->tw_x[1], plan->tw_x[2], plan->tw_y[0], plan->tw_y[1], plan->tw_y[2],
+                    (void*)plan->passes, (void*)plan->kx, (void*)plan->ky};
+    for (void* ptr : ptrs)
+        if (ptr && (f = hipFree(ptr)) != hipSuccess && e == hipSuccess) e = f;
     delete plan;
-    return PGPE_OK;
+    return e == hipSuccess ? PGPE_OK : hip_fail(e, "pgpe_fft_plan_destroy");
 }
 
 int pgpe_fft2(const pgpe_fft_plan* plan, const void* in, void* out, int planes, int inverse, void* stream) {

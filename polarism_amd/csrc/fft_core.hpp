@@ -1,14 +1,24 @@
-// In-LDS mixed-radix (2, 3, 4) Stockham FFT used by the row and column kernels.
+// In-LDS mixed-radix (2, 3, 4, 8, 16) Stockham FFT used by the row and column kernels.
 //
 // A workgroup of 256 threads transforms `count` sequences of length n that
-// sit contiguously in LDS (buf[s * n + k]).  Each pass is the Stockham
-// autosort iteration: butterfly j of radix r reads r inputs n/r apart,
-// multiplies input q by w^(q (j mod Ns)) with w = exp(-+2 pi i / (Ns r)), runs an
-// r-point DFT and writes the outputs Ns apart at (j / Ns) Ns r + (j mod Ns).
-// Twiddles come from a host-computed table of the n-th roots of unity
-// (float64 cos/sin rounded once), so the FP64 transform keeps ~1e-16
-// relative accuracy.  The LDS buffer is single: every thread gathers its
-// butterflies' inputs into registers, a barrier, then all scatter.
+// sit in LDS.  Each pass is the Stockham autosort iteration: butterfly j of
+// radix r reads r inputs n/r apart, multiplies input q by w^(q (j mod Ns)) with
+// w = exp(-+2 pi i / (Ns r)), runs an r-point DFT in registers and writes the
+// outputs Ns apart at (j / Ns) Ns r + (j mod Ns).  The pass list comes from the
+// plan (built on the host for three radix classes): the largest radices the
+// per-thread element count V allows (16 when V is a multiple of 16, else 8 or
+// 4), balanced so that n = 2^a takes ceil(a / 4) passes at V = 16 or 32 (a
+// 2048-point transform: 16, 16, 8), then one radix-3 pass when 3 divides n.
+// Twiddles come from host-computed n-th roots of unity (long double cos/sin
+// rounded once), so the FP64 transform keeps ~1e-16 relative accuracy; the
+// constants inside the radix-8/16 DFTs are cos(pi/8), sin(pi/8) and 1/sqrt(2).
+//
+// Layout: element k of a sequence sits at k + k/16 (one pad slot per 16
+// elements: a stride-16 scatter of the first radix-16 pass then spreads over
+// the LDS banks) and sequences are seq_stride(n) apart, chosen so that
+// consecutive sequences start 64 bytes apart modulo 256 (the column kernels'
+// transposing loads hit distinct banks).  The buffer is single: every thread
+// gathers its butterflies' inputs into registers, a barrier, then all scatter.
 #pragma once
 
 #include "common.hpp"
@@ -18,11 +28,49 @@ namespace pgpe {
 constexpr int kFftThreads = 256;
 constexpr int kFftMaxPasses = 8;
 
-struct FftDims {
-    int n;                          // transform length
-    int npass;
-    int radix[kFftMaxPasses];       // 2, 3 or 4
+// One Stockham pass of a plan: its radix, the product ns of the earlier radices and the
+// division magics of n / radix and ns (FastDiv; 0 when the divisor is 1).
+struct FftPass {
+    int radix;
+    int ns;
+    uint32_t nb_m;
+    uint32_t ns_m;
 };
+
+// A transform length with its pass list (device memory, read through the scalar cache).
+struct FftDims {
+    int n;
+    int npass;
+    const FftPass* passes;
+};
+
+// Loads then stores `elems` items strided over the group's threads, CH at a time: a thread
+// issues its CH loads before it waits on the first, so the group keeps CH x 256 loads in
+// flight (one at a time per thread when load and store alternate, which leaves the
+// few-workgroup column passes bound by memory latency).  load(i) -> P, store(i, P).
+template <typename P, int CH, typename Load, typename Store>
+__device__ __forceinline__ void staged(int elems, Load load, Store store) {
+    for (int i0 = 0; i0 < elems; i0 += kFftThreads * CH) {
+        P r[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            const int i = i0 + k * kFftThreads + (int)threadIdx.x;
+            if (i < elems) r[k] = load(i);
+        }
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            const int i = i0 + k * kFftThreads + (int)threadIdx.x;
+            if (i < elems) store(i, r[k]);
+        }
+    }
+}
+
+// Loads in flight per thread for V elements per thread: all of them up to 64 VGPRs of data.
+template <typename T, int V>
+constexpr int stage_chunk() {
+    constexpr int cap = sizeof(T) == 4 ? 32 : 16;
+    return V < cap ? V : cap;
+}
 
 template <typename T>
 __device__ __forceinline__ c2<T> cmul(c2<T> a, c2<T> b) {
@@ -64,66 +112,208 @@ __device__ __forceinline__ void dft_small(c2<T>* v, bool inverse) {
     }
 }
 
-// One Stockham pass over `count` sequences; V = elements per thread.
+// Unsigned division by a divisor fixed for a pass: q = floor(x m / 2^32) with
+// m = floor((2^32 - 1) / d) + 1 (one 32-bit division to set up, one mul_hi per use), exact
+// whenever x * d < 2^32 (here x < 2^16, d <= 4096); m = 0 marks d = 1.
+__host__ __device__ __forceinline__ uint32_t div_magic(uint32_t d) { return d == 1u ? 0u : 0xFFFFFFFFu / d + 1u; }
+
+struct FastDiv {
+    uint32_t m;
+    __device__ __forceinline__ explicit FastDiv(uint32_t d) : m(div_magic(d)) {}
+    __device__ __forceinline__ static FastDiv from_magic(uint32_t magic) {
+        FastDiv f(1u);
+        f.m = magic;
+        return f;
+    }
+    __device__ __forceinline__ int div(int x) const { return m ? (int)__umulhi((uint32_t)x, m) : x; }
+};
+
+// Padded position of element k inside its sequence.
+__host__ __device__ __forceinline__ constexpr int pad_at(int k) { return k + (k >> 4); }
+
+// Distance between consecutive sequences: holds pad_at(n - 1) + 1 slots and starts the next
+// sequence 64 bytes further modulo 256.
+template <typename T>
+__host__ __device__ __forceinline__ constexpr int seq_stride(int n) {
+    const int p = n + (n >> 4);
+    const int unit = 256 / (int)sizeof(c2<T>);
+    const int off = 64 / (int)sizeof(c2<T>);
+    return ((p - off + unit - 1) / unit) * unit + off;
+}
+
+// x * exp(-+2 pi i / 8): (1 -+ i) / sqrt(2).
+template <typename T>
+__device__ __forceinline__ c2<T> mul_w8(c2<T> a, bool inverse) {
+    const T h = (T)0.70710678118654752440084436210485;
+    return inverse ? make_c<T>(h * (a.x - a.y), h * (a.y + a.x)) : make_c<T>(h * (a.x + a.y), h * (a.y - a.x));
+}
+
+// x * exp(-+2 pi i m / 16) for the products m = n2 k1 of a 4 x 4 split.
+template <typename T>
+__device__ __forceinline__ c2<T> mul_w16(c2<T> a, int m, bool inverse) {
+    const T c = (T)0.92387953251128675612818318939679;  // cos(pi / 8)
+    const T s = (T)0.38268343236508977172845998403040;  // sin(pi / 8)
+    const T si = inverse ? s : -s;
+    const T ci = inverse ? c : -c;
+    switch (m) {
+        case 0: return a;
+        case 1: return make_c<T>(c * a.x - si * a.y, c * a.y + si * a.x);     // c -+ i s
+        case 2: return mul_w8<T>(a, inverse);
+        case 3: return make_c<T>(s * a.x - ci * a.y, s * a.y + ci * a.x);     // s -+ i c
+        case 4: return rot90<T>(a, inverse);
+        case 6: return rot90<T>(mul_w8<T>(a, inverse), inverse);
+        default: return make_c<T>(-(c * a.x - si * a.y), -(c * a.y + si * a.x));  // m = 9: -(c -+ i s)
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4_of(c2<T>& a, c2<T>& b, c2<T>& c, c2<T>& d, bool inverse) {
+    c2<T> v[4] = {a, b, c, d};
+    dft_small<T, 4>(v, inverse);
+    a = v[0];
+    b = v[1];
+    c = v[2];
+    d = v[3];
+}
+
+// 8-point DFT: one radix-2 split, the odd half rotated by w8^k, two 4-point DFTs.
+template <typename T>
+__device__ __forceinline__ void dft8(c2<T>* v, bool inverse) {
+    c2<T> u[4], w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        u[k] = make_c<T>(v[k].x + v[k + 4].x, v[k].y + v[k + 4].y);
+        w[k] = make_c<T>(v[k].x - v[k + 4].x, v[k].y - v[k + 4].y);
+    }
+    w[1] = mul_w8<T>(w[1], inverse);
+    w[2] = rot90<T>(w[2], inverse);
+    w[3] = rot90<T>(mul_w8<T>(w[3], inverse), inverse);
+    dft4_of<T>(u[0], u[1], u[2], u[3], inverse);
+    dft4_of<T>(w[0], w[1], w[2], w[3], inverse);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        v[2 * m] = u[m];
+        v[2 * m + 1] = w[m];
+    }
+}
+
+// 16-point DFT as 4 x 4: 4-point DFTs over n1 of x[4 n1 + n2], twiddles w16^(n2 k1),
+// 4-point DFTs over n2; output k1 + 4 k2.
+template <typename T>
+__device__ __forceinline__ void dft16(c2<T>* v, bool inverse) {
+    c2<T> y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        y[n2][0] = v[n2];
+        y[n2][1] = v[4 + n2];
+        y[n2][2] = v[8 + n2];
+        y[n2][3] = v[12 + n2];
+        dft4_of<T>(y[n2][0], y[n2][1], y[n2][2], y[n2][3], inverse);
+#pragma unroll
+        for (int k1 = 1; k1 < 4; ++k1) y[n2][k1] = mul_w16<T>(y[n2][k1], n2 * k1, inverse);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        dft4_of<T>(y[0][k1], y[1][k1], y[2][k1], y[3][k1], inverse);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = y[k2][k1];
+    }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void dft_radix(c2<T>* v, bool inverse) {
+    if constexpr (R == 16) dft16<T>(v, inverse);
+    else if constexpr (R == 8) dft8<T>(v, inverse);
+    else dft_small<T, R>(v, inverse);
+}
+
+// Stages a plan's twiddles into LDS (the n - 1 entries of the per-pass table, see the
+// plan builder): pass p (radix R, ns = product of the earlier radices) finds w^(q jm n / (ns R))
+// at ns - 1 + jm (R - 1) + q - 1 for jm < ns, 1 <= q < R, so a butterfly reads its R - 1
+// factors from consecutive slots and neighbouring butterflies from neighbouring runs.
+// Callers put a barrier between this and lds_fft.
+template <typename T>
+__device__ __forceinline__ void stage_twiddles(c2<T>* s_tw, const c2<T>* __restrict__ tw, int n) {
+    staged<c2<T>, (sizeof(T) == 4 ? 16 : 8)>(n - 1, [&](int i) { return tw[i]; }, [&](int i, c2<T> w) { s_tw[i] = w; });
+}
+
+// One Stockham pass over `count` sequences ld apart; V = elements per thread; tw = this
+// pass's staged twiddles.
 template <typename T, int R, int V>
-__device__ __forceinline__ void stockham_pass(c2<T>* buf, int n, int ld, int count, int Ns,
+__device__ __forceinline__ void stockham_pass(c2<T>* buf, int n, int ld, int count, const FftPass& ps,
                                               const c2<T>* __restrict__ tw, bool inverse) {
     constexpr int B = V / R;
     const int nb = n / R;
+    const int Ns = ps.ns;
     const int total = nb * count;
-    const int tstride = n / (Ns * R);  // twiddle index step in the n-th root table
+    const FastDiv by_nb = FastDiv::from_magic(ps.nb_m), by_ns = FastDiv::from_magic(ps.ns_m);
     c2<T> v[B][R];
-    int dst[B];
+    int base[B], k0[B];
 #pragma unroll
     for (int i = 0; i < B; ++i) {
         const int b = threadIdx.x + i * kFftThreads;
-        dst[i] = -1;
+        base[i] = -1;
         if (b < total) {
-            const int s = b / nb;
+            const int s = by_nb.div(b);
             const int j = b - s * nb;
-            const int base = s * ld;
-            const int jm = j % Ns;
+            const int jq = by_ns.div(j);
+            const int jm = j - jq * Ns;
+            base[i] = s * ld;
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                c2<T> x = buf[base + j + q * nb];
+                c2<T> x = buf[base[i] + pad_at(j + q * nb)];
                 if (q > 0 && jm > 0) {
-                    c2<T> w = tw[q * jm * tstride];
+                    c2<T> w = tw[jm * (R - 1) + q - 1];
                     if (inverse) w.y = -w.y;
                     x = cmul<T>(x, w);
                 }
                 v[i][q] = x;
             }
-            dft_small<T, R>(v[i], inverse);
-            dst[i] = base + (j / Ns) * Ns * R + jm;
+            dft_radix<T, R>(v[i], inverse);
+            k0[i] = jq * Ns * R + jm;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < B; ++i) {
-        if (dst[i] >= 0) {
+        if (base[i] >= 0) {
 #pragma unroll
-            for (int q = 0; q < R; ++q) buf[dst[i] + q * Ns] = v[i][q];
+            for (int q = 0; q < R; ++q) buf[base[i] + pad_at(k0[i] + q * Ns)] = v[i][q];
         }
     }
     __syncthreads();
 }
 
-// Sequence s occupies buf[s * ld + k], k < n (ld >= n; padding avoids LDS bank conflicts).
+// Sequence s occupies buf[s * ld + pad_at(k)], k < n, ld >= pad_at(n - 1) + 1; tw holds the
+// staged twiddles of the plan d, whose radices divide V (the launchers pick the plan by V).
 template <typename T, int V>
-__device__ void lds_fft(c2<T>* buf, const FftDims& d, int ld, int count, const c2<T>* __restrict__ tw,
-                        bool inverse) {
-    int Ns = 1;
+__device__ __forceinline__ void lds_fft(c2<T>* buf, const FftDims& d, int ld, int count,
+                                        const c2<T>* __restrict__ tw, bool inverse) {
     for (int p = 0; p < d.npass; ++p) {
-        const int r = d.radix[p];
-        if (r == 4) {
-            if constexpr (V % 4 == 0) stockham_pass<T, 4, V>(buf, d.n, ld, count, Ns, tw, inverse);
-        } else if (r == 2) {
-            stockham_pass<T, 2, V>(buf, d.n, ld, count, Ns, tw, inverse);
+        const FftPass ps = d.passes[p];
+        const c2<T>* t = tw + (ps.ns - 1);
+        if (ps.radix == 16) {
+            if constexpr (V % 16 == 0 && V < 64) stockham_pass<T, 16, V>(buf, d.n, ld, count, ps, t, inverse);
+        } else if (ps.radix == 8) {
+            if constexpr (V % 8 == 0) stockham_pass<T, 8, V>(buf, d.n, ld, count, ps, t, inverse);
+        } else if (ps.radix == 4) {
+            if constexpr (V % 4 == 0) stockham_pass<T, 4, V>(buf, d.n, ld, count, ps, t, inverse);
+        } else if (ps.radix == 2) {
+            // V = 64 serves only columns of 1024 or more (the launchers guarantee it), whose
+            // plans hold radices 8 and 4: 32 radix-2 butterflies per thread would spill registers
+            if constexpr (V < 64) stockham_pass<T, 2, V>(buf, d.n, ld, count, ps, t, inverse);
         } else {
-            if constexpr (V % 3 == 0) stockham_pass<T, 3, V>(buf, d.n, ld, count, Ns, tw, inverse);
+            if constexpr (V % 3 == 0) stockham_pass<T, 3, V>(buf, d.n, ld, count, ps, t, inverse);
         }
-        Ns *= r;
     }
+}
+
+// Plan class of a kernel with V elements per thread: 0 = radices up to 4, 1 = up to 8,
+// 2 = up to 16 (radix 16 at V = 64 would hold 4 x 16 inputs and their twiddles at once,
+// past the 512 VGPRs, so V = 64 takes class 1).
+template <int V>
+constexpr int radix_class() {
+    return (V % 16 == 0 && V < 64) ? 2 : (V % 8 == 0 ? 1 : 0);
 }
 
 }  // namespace pgpe

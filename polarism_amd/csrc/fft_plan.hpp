@@ -1,16 +1,21 @@
 // Plan object shared by the FFT, split-step and Lawson kernels.
 #pragma once
 
+#include <vector>
+
 #include "fft_core.hpp"
 
 struct pgpe_fft_plan {
     int nx, ny;
     int precision;
     double dx, dy;
-    pgpe::FftDims dx_dims, dy_dims;
-    void* tw_x;   // n-th roots of unity for nx (precision-typed c2)
-    void* tw_y;
-    double* kx;   // 2 pi fftfreq(nx, dx)
+    // per radix class (radix_class<V>(): radices up to 4, 8, 16): the pass list and the
+    // per-pass twiddle table (n - 1 entries, precision-typed c2) of each dimension
+    pgpe::FftDims dims_x[3], dims_y[3];
+    void* tw_x[3];
+    void* tw_y[3];
+    pgpe::FftPass* passes;  // device block holding the six pass lists
+    double* kx;             // 2 pi fftfreq(nx, dx)
     double* ky;
     int vx, vy;   // elements per thread of the Lawson row / column kernels (16, or 12 with a radix 3)
     int rv;       // elements per thread of the FFT / split-step row kernels: the fewest that hold
@@ -19,32 +24,57 @@ struct pgpe_fft_plan {
 
 namespace pgpe {
 
-// Radix plan of n = 2^a or 3 * 2^a: radix-4 passes, one radix-2 if a is odd,
-// one radix-3; v = elements per thread (16, or 12 when 3 divides n).
-inline bool factor(int n, FftDims& d, int& v) {
-    d.n = n;
-    d.npass = 0;
+// n = 2^a or 3 * 2^a with 2 <= n <= 256 v, v = elements per thread of the Lawson kernels
+// (16, or 12 when 3 divides n).
+inline bool check_length(int n, int& v) {
     int m = n, threes = 0;
     while (m % 3 == 0) { m /= 3; ++threes; }
-    if (threes > 1) return false;
-    int twos = 0;
-    while (m % 2 == 0) { m /= 2; ++twos; }
-    if (m != 1 || n < 2) return false;
-    for (int i = 0; i < twos / 2; ++i) d.radix[d.npass++] = 4;
-    if (twos % 2) d.radix[d.npass++] = 2;
-    if (threes) d.radix[d.npass++] = 3;
+    if (threes > 1 || n < 2) return false;
+    while (m % 2 == 0) m /= 2;
+    if (m != 1) return false;
     v = threes ? 12 : 16;
-    return n <= 256 * v && d.npass <= kFftMaxPasses;
+    return n <= 256 * v;
+}
+
+// Pass list of an n-point transform in radix class cls: the powers of two with exponents
+// up to cls + 2, balanced over ceil(a / (cls + 2)) passes, then radix 3 when 3 divides n.
+inline void build_passes(int n, int cls, std::vector<FftPass>& out) {
+    out.clear();
+    const int emax = cls + 2;
+    int a = 0;
+    while (((n >> a) & 1) == 0) ++a;
+    int left = (a + emax - 1) / emax;
+    int ns = 1;
+    while (a > 0) {
+        const int e = (a + left - 1) / left;
+        const int r = 1 << e;
+        out.push_back(FftPass{r, ns, div_magic((uint32_t)(n / r)), div_magic((uint32_t)ns)});
+        ns *= r;
+        a -= e;
+        --left;
+    }
+    if (n % 3 == 0) out.push_back(FftPass{3, ns, div_magic((uint32_t)(n / 3)), div_magic((uint32_t)ns)});
+}
+
+// Per-pass twiddle table from the n-th roots of unity root[k] = exp(-2 pi i k / n):
+// slot ns - 1 + jm (r - 1) + q - 1 of pass (r, ns) holds root[q jm n / (ns r)].
+template <typename C>
+inline void pass_twiddles(int n, const std::vector<FftPass>& passes, const std::vector<C>& root, std::vector<C>& out) {
+    out.assign(n > 1 ? n - 1 : 1, root[0]);
+    for (const FftPass& ps : passes) {
+        const int stride = n / (ps.ns * ps.radix);
+        for (int jm = 0; jm < ps.ns; ++jm)
+            for (int q = 1; q < ps.radix; ++q) out[ps.ns - 1 + jm * (ps.radix - 1) + q - 1] = root[q * jm * stride];
+    }
 }
 
 // Elements per thread of a 256-thread group holding `need` elements: 4, 8, 16 (power-of-two
-// lengths; every pass needs V divisible by its radix) or 12 (lengths with a radix 3).
+// lengths) or 12 (lengths with a radix 3).
 inline int small_v(int n, int need) {
     if (n % 3 == 0) return 12;
     for (int v = 4; v <= 16; v *= 2)
         if (need <= 256 * v) return v;
     return 16;
 }
-
 
 }  // namespace pgpe

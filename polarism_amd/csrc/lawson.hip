@@ -25,6 +25,12 @@ namespace pgpe {
 
 constexpr int kMaxStages = 12;
 
+// LDS of a group: `seqs` padded sequences of length n plus the n - 1 staged twiddles.
+template <typename T>
+static size_t lds_bytes(int seqs, int n) {
+    return (size_t)(seqs * seq_stride<T>(n) + n) * sizeof(c2<T>);
+}
+
 enum LawsonColMode { LC_FWD = 0, LC_COMBINE = 1, LC_FINAL = 2 };
 enum LawsonRowMode { LR_RHS = 0, LR_RHS_REAL = 1, LR_FINAL = 2 };
 
@@ -163,11 +169,11 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
     c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
     constexpr int E = 256 * V;
     const int n = p.ny;
-    c2<T>* s_tw = buf + E;  // twiddles staged once per group: every pass reads them from LDS
-    const int ld = n + 1;
+    const int ld = seq_stride<T>(n);
     const int kinds = 1 + (MODE == LC_COMBINE ? p.lap : (MODE == LC_FINAL ? p.nerr : 0));
     const int seqs_per_col = kinds * p.comps;
-    const int cols_per = E / (ld * seqs_per_col);
+    const int cols_per = E / (n * seqs_per_col);
+    c2<T>* s_tw = buf + cols_per * seqs_per_col * ld;  // twiddles staged once per group
     const int groups = (p.nx + cols_per - 1) / cols_per;
     const int m = blockIdx.x / groups;
     if (m >= p.members) return;
@@ -177,15 +183,20 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
     const int64_t planes_total = (int64_t)p.members * p.comps;
     const pgpe_member& mb = p.table[m];
     auto seq = [&](int a, int c, int i) { return ((a * p.comps + c) * cnt + i) * ld; };
-    for (int i = threadIdx.x; i < n; i += 256) s_tw[i] = p.tw[i];
+    stage_twiddles<T>(s_tw, p.tw, n);
+    constexpr int CH = stage_chunk<T, V>();
+    struct CPt { c2<T> v; int at; };
 
     if constexpr (MODE == LC_FWD) {
         for (int c = 0; c < p.comps; ++c) {
             const c2<T>* src = p.in + (m * p.comps + c) * plane;
-            for (int e = threadIdx.x; e < cnt * n; e += 256) {
-                const int y = e / cnt, i = e - y * cnt;
-                buf[seq(0, c, i) + y] = src[(int64_t)y * p.nx + x0 + i];
-            }
+            staged<CPt, CH>(
+                cnt * n,
+                [&](int e) {
+                    const int y = e / cnt, i = e - y * cnt;
+                    return CPt{src[(int64_t)y * p.nx + x0 + i], seq(0, c, i) + pad_at(y)};
+                },
+                [&](int, CPt q) { buf[q.at] = q.v; });
         }
         __syncthreads();
         lds_fft<T, V>(buf, p.dims, ld, cnt * p.comps, s_tw, false);
@@ -204,11 +215,11 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
                 accumulate_term<T>(mb, p.comps, kxv, kyv, p.fr[j] * p.h, (T)p.hc[j], v, acc);
             }
             for (int c = 0; c < p.comps; ++c) {
-                buf[seq(0, c, i) + y] = acc[c];
+                buf[seq(0, c, i) + pad_at(y)] = acc[c];
                 if (MODE == LC_FINAL && p.psik_out) p.psik_out[(m * p.comps + c) * plane + g] = acc[c];
                 if (MODE == LC_COMBINE && p.lap) {
                     const T mk2 = -(T)(kxv * kxv + kyv * kyv);
-                    buf[seq(1, c, i) + y] = make_c<T>(mk2 * acc[c].x, mk2 * acc[c].y);
+                    buf[seq(1, c, i) + pad_at(y)] = make_c<T>(mk2 * acc[c].x, mk2 * acc[c].y);
                 }
             }
             if (MODE == LC_FINAL) {
@@ -219,7 +230,7 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
                         for (int c = 0; c < p.comps; ++c) v[c] = p.f[j][(m * p.comps + c) * plane + g];
                         accumulate_term<T>(mb, p.comps, kxv, kyv, p.fr[j] * p.h, (T)p.he[q][j], v, er);
                     }
-                    for (int c = 0; c < p.comps; ++c) buf[seq(1 + q, c, i) + y] = er[c];
+                    for (int c = 0; c < p.comps; ++c) buf[seq(1 + q, c, i) + pad_at(y)] = er[c];
                 }
             }
         }
@@ -230,10 +241,13 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
     for (int a = 0; a < kinds; ++a) {
         for (int c = 0; c < p.comps; ++c) {
             c2<T>* dst = (a == 0 ? p.out : p.aux + (int64_t)(a - 1) * planes_total * plane) + (m * p.comps + c) * plane;
-            for (int e = threadIdx.x; e < cnt * n; e += 256) {
-                const int y = e / cnt, i = e - y * cnt;
-                dst[(int64_t)y * p.nx + x0 + i] = buf[seq(a, c, i) + y];
-            }
+            staged<CPt, CH>(
+                cnt * n,
+                [&](int e) {
+                    const int y = e / cnt, i = e - y * cnt;
+                    return CPt{buf[seq(a, c, i) + pad_at(y)], y * p.nx + x0 + i};
+                },
+                [&](int, CPt q) { dst[q.at] = q.v; });
         }
     }
 }
@@ -252,7 +266,12 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
     c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
     constexpr int NF = RES == 0 ? 0 : (RES == 1 ? 1 : 2);
     const int n = p.nx;
-    c2<T>* s_tw = buf + 256 * V;  // twiddles staged once per group
+    const int ld = seq_stride<T>(n);
+    const FastDiv by_n((uint32_t)n);
+    auto slot = [&](int e) {  // LDS slot of element e of a sequence group (row e / n, column e % n)
+        const int r = by_n.div(e);
+        return r * ld + pad_at(e - r * n);
+    };
     const int kinds = 1 + (MODE == LR_RHS ? p.lap : (MODE == LR_FINAL ? p.nerr : 0));
     const int groups_per_member = p.ny / rows_per;
     const int m = blockIdx.x / groups_per_member;
@@ -262,19 +281,21 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
     const int64_t planes_total = (int64_t)p.members * p.comps;
     const int C = p.comps;
     const pgpe_member& mb = p.table[m];
-    auto seq = [&](int a, int c, int r) { return ((a * C + c) * rows_per + r) * n; };
+    auto seq = [&](int a, int c, int r) { return ((a * C + c) * rows_per + r) * ld; };
+    c2<T>* s_tw = buf + kinds * C * rows_per * ld;  // twiddles staged once per group
     const int elems = rows_per * n;
-    for (int i = threadIdx.x; i < n; i += 256) s_tw[i] = p.tw[i];
+    stage_twiddles<T>(s_tw, p.tw, n);
 
     // ---- load ----
     for (int a = 0; a < kinds; ++a)
         for (int c = 0; c < C; ++c) {
             const c2<T>* src = MODE == LR_RHS_REAL ? p.psi_real : (a == 0 ? p.in : p.aux + (int64_t)(a - 1) * planes_total * plane);
             src += (m * C + c) * plane + (int64_t)y0 * n;
-            for (int e = threadIdx.x; e < elems; e += 256) buf[seq(a, c, 0) + e] = src[e];
+            c2<T>* dst = buf + seq(a, c, 0);
+            staged<c2<T>, stage_chunk<T, V>()>(elems, [&](int e) { return src[e]; }, [&](int e, c2<T> v) { dst[slot(e)] = v; });
         }
     __syncthreads();
-    if (MODE != LR_RHS_REAL) lds_fft<T, V>(buf, p.dims, n, rows_per * C * kinds, s_tw, true);
+    if (MODE != LR_RHS_REAL) lds_fft<T, V>(buf, p.dims, ld, rows_per * C * kinds, s_tw, true);
 
     // ---- pointwise physics ----
     const T gC = (T)mb.g_C, gR = (T)mb.g_R, gI = (T)mb.g_I, Rr = (T)mb.R, gammaC = (T)mb.gamma_C;
@@ -293,7 +314,7 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
         c2<T> psi[2], lap[2] = {make_c<T>(0, 0), make_c<T>(0, 0)};
         T rho[2] = {0, 0};
         for (int c = 0; c < C; ++c) {
-            c2<T> v = buf[seq(0, c, 0) + e];
+            c2<T> v = buf[seq(0, c, 0) + slot(e)];
             if (MODE != LR_RHS_REAL) {
                 v.x *= p.scale;
                 v.y *= p.scale;
@@ -301,7 +322,7 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
             psi[c] = v;
             rho[c] = v.x * v.x + v.y * v.y;
             if (MODE == LR_RHS && p.lap) {
-                const c2<T> l = buf[seq(1, c, 0) + e];
+                const c2<T> l = buf[seq(1, c, 0) + slot(e)];
                 lap[c] = make_c<T>(l.x * p.scale, l.y * p.scale);
             }
         }
@@ -347,7 +368,7 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
                     const double sc = p.atol + p.rtol * fmax(hypot((double)st.x, (double)st.y),
                                                              hypot((double)v.x, (double)v.y));
                     for (int q = 0; q < p.nerr; ++q) {
-                        const c2<T> er = buf[seq(1 + q, c, 0) + e];
+                        const c2<T> er = buf[seq(1 + q, c, 0) + slot(e)];
                         const double mag = hypot((double)er.x * (double)p.scale, (double)er.y * (double)p.scale);
                         const double s = sc > 0 ? mag / sc : (mag == 0 ? 0.0 : INFINITY);
                         if (q == 0) acc5 += s * s; else acc3 += s * s;
@@ -383,7 +404,7 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
                     d.x = d.x + s * lap[c].x;
                     d.y = d.y + s * lap[c].y;
                 }
-                buf[seq(0, c, 0) + e] = d;
+                buf[seq(0, c, 0) + slot(e)] = d;
             }
             for (int rcix = 0; rcix < nres_c; ++rcix) {
                 T r = nres_c == 2 ? rho[rcix] : (C == 2 ? rho[0] + rho[1] : rho[0]);
@@ -410,10 +431,11 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
         return;
     }
     __syncthreads();
-    lds_fft<T, V>(buf, p.dims, n, rows_per * C, s_tw, false);
+    lds_fft<T, V>(buf, p.dims, ld, rows_per * C, s_tw, false);
     for (int c = 0; c < C; ++c) {
         c2<T>* dst = p.out + (m * C + c) * plane + (int64_t)y0 * n;
-        for (int e = threadIdx.x; e < elems; e += 256) dst[e] = buf[seq(0, c, 0) + e];
+        const c2<T>* src = buf + seq(0, c, 0);
+        staged<c2<T>, stage_chunk<T, V>()>(elems, [&](int e) { return src[slot(e)]; }, [&](int e, c2<T> v) { dst[e] = v; });
     }
 }
 
@@ -435,17 +457,24 @@ __global__ void lawson_member_error(const double* __restrict__ partials, int gro
     }
 }
 
-static int rows_per_group(int E, int nx, int ny, int seqs) {
-    int r = E / (nx * seqs);
+// Rows per group: the largest power of two dividing ny whose seqs x rows sequences fit the
+// group's 256 x V elements (one per thread slot), with their padded LDS copy and the staged
+// twiddles within the 160 KB of LDS; 0 when none does.
+template <typename T>
+static int rows_per_group(int V, int nx, int ny, int seqs) {
+    const int r = 256 * V / (nx * seqs);
     int best = 0;
     for (int d = 1; d <= r; d *= 2)
-        if (ny % d == 0) best = d;
+        if (ny % d == 0 && lds_bytes<T>(d * seqs, nx) <= 163840) best = d;
     return best;
 }
 
 template <typename T, int V, int MODE>
-static int launch_cols_v(LawsonParams<T> p, int cols_per, hipStream_t s) {
-    const size_t shm = (size_t)(256 * V + p.ny) * sizeof(c2<T>);
+static int launch_cols_v(const pgpe_fft_plan* plan, LawsonParams<T> p, int seqs, hipStream_t s) {
+    p.dims = plan->dims_y[radix_class<V>()];
+    p.tw = (const c2<T>*)plan->tw_y[radix_class<V>()];
+    const int cols_per = 256 * V / (p.ny * seqs);
+    const size_t shm = lds_bytes<T>(cols_per * seqs, p.ny);
     auto kern = lawson_cols_kernel<T, V, MODE>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -457,27 +486,28 @@ static int launch_cols_v(LawsonParams<T> p, int cols_per, hipStream_t s) {
     return PGPE_OK;
 }
 
-// The column group must hold kinds x comps padded columns; tall spinor grids
-// with error fields take the doubled buffer (V = 32 or 24; with the staged twiddles
-// the group stays within the 160 KB of LDS).
+// A column group holds kinds x comps sequences per column; tall spinor grids with error
+// fields take twice the elements per thread (V = 32 or 24) when one column's sequences
+// exceed 256 x V.
 template <typename T, int MODE>
 static int launch_cols(const pgpe_fft_plan* plan, LawsonParams<T> p, hipStream_t s) {
-    p.dims = plan->dy_dims;
-    p.tw = (const c2<T>*)plan->tw_y;
     const int kinds = 1 + (MODE == LC_COMBINE ? p.lap : (MODE == LC_FINAL ? p.nerr : 0));
-    const int need = (plan->ny + 1) * kinds * p.comps;
+    const int seqs = kinds * p.comps;
+    const int need = plan->ny * seqs;
     const int V = plan->vy;
-    if (256 * V >= need) return V == 16 ? launch_cols_v<T, 16, MODE>(p, 256 * 16 / need, s)
-                                        : launch_cols_v<T, 12, MODE>(p, 256 * 12 / need, s);
-    if (512 * V >= need && (512 * V + plan->ny) * (int)sizeof(c2<T>) <= 163840)
-        return V == 16 ? launch_cols_v<T, 32, MODE>(p, 256 * 32 / need, s)
-                       : launch_cols_v<T, 24, MODE>(p, 256 * 24 / need, s);
+    auto fits = [&](int v) { return 256 * v >= need && lds_bytes<T>(256 * v / need * seqs, plan->ny) <= 163840; };
+    if (fits(V))
+        return V == 16 ? launch_cols_v<T, 16, MODE>(plan, p, seqs, s) : launch_cols_v<T, 12, MODE>(plan, p, seqs, s);
+    if (fits(2 * V))
+        return V == 16 ? launch_cols_v<T, 32, MODE>(plan, p, seqs, s) : launch_cols_v<T, 24, MODE>(plan, p, seqs, s);
     return fail(PGPE_ERR_UNSUPPORTED, "lawson: grid too tall for one LDS column group");
 }
 
 template <typename T, int V, int MODE, int RES>
-static int launch_rows_v(LawsonParams<T> p, int rows_per, hipStream_t s) {
-    const size_t shm = (size_t)(256 * V + p.nx) * sizeof(c2<T>);
+static int launch_rows_v(const pgpe_fft_plan* plan, LawsonParams<T> p, int rows_per, int seqs, hipStream_t s) {
+    p.dims = plan->dims_x[radix_class<V>()];
+    p.tw = (const c2<T>*)plan->tw_x[radix_class<V>()];
+    const size_t shm = lds_bytes<T>(rows_per * seqs, p.nx);
     auto kern = lawson_rows_kernel<T, V, MODE, RES>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -491,25 +521,23 @@ static int launch_rows_v(LawsonParams<T> p, int rows_per, hipStream_t s) {
 
 // Rows of a group never straddle two members (rows_per divides ny), so the
 // error partials of a group belong to one member.  Long rows with several
-// sequences per row take the doubled buffer (V = 32 or 24).
+// sequences per row take twice the elements per thread (V = 32 or 24).
 template <typename T, int MODE, int RES>
 static int launch_rows(const pgpe_fft_plan* plan, LawsonParams<T> p, hipStream_t s, int* groups_per_member) {
-    p.dims = plan->dx_dims;
-    p.tw = (const c2<T>*)plan->tw_x;
     const int kinds = 1 + (MODE == LR_RHS ? p.lap : (MODE == LR_FINAL ? p.nerr : 0));
+    const int seqs = kinds * p.comps;
     const int V = plan->vx;
-    int rows_per = rows_per_group(256 * V, plan->nx, plan->ny, kinds * p.comps);
-    int rc;
+    int rows_per = rows_per_group<T>(V, plan->nx, plan->ny, seqs);
     if (rows_per >= 1) {
         if (groups_per_member) *groups_per_member = plan->ny / rows_per;
-        rc = V == 16 ? launch_rows_v<T, 16, MODE, RES>(p, rows_per, s) : launch_rows_v<T, 12, MODE, RES>(p, rows_per, s);
-        return rc;
+        return V == 16 ? launch_rows_v<T, 16, MODE, RES>(plan, p, rows_per, seqs, s)
+                       : launch_rows_v<T, 12, MODE, RES>(plan, p, rows_per, seqs, s);
     }
-    rows_per = rows_per_group(512 * V, plan->nx, plan->ny, kinds * p.comps);
-    if (rows_per < 1 || (512 * V + plan->nx) * (int)sizeof(c2<T>) > 163840)
-        return fail(PGPE_ERR_UNSUPPORTED, "lawson: rows too long for one LDS row group");
+    rows_per = rows_per_group<T>(2 * V, plan->nx, plan->ny, seqs);
+    if (rows_per < 1) return fail(PGPE_ERR_UNSUPPORTED, "lawson: rows too long for one LDS row group");
     if (groups_per_member) *groups_per_member = plan->ny / rows_per;
-    return V == 16 ? launch_rows_v<T, 32, MODE, RES>(p, rows_per, s) : launch_rows_v<T, 24, MODE, RES>(p, rows_per, s);
+    return V == 16 ? launch_rows_v<T, 32, MODE, RES>(plan, p, rows_per, seqs, s)
+                   : launch_rows_v<T, 24, MODE, RES>(plan, p, rows_per, seqs, s);
 }
 
 template <typename T>
