@@ -116,11 +116,11 @@ __device__ __forceinline__ void store_res(const SplitParams<T>& p, int64_t g, co
 }
 
 // ---------------------------------------------------------------- rows
-template <typename T, int V, int MODE, int RES>
-__global__ void __launch_bounds__(256) fft_rows_kernel(const SplitParams<T> p) {
+template <typename T, int V, int MODE, int RES, int NT>
+__global__ void __launch_bounds__(NT) fft_rows_kernel(const SplitParams<T> p) {
     extern __shared__ unsigned char smem_raw[];
     c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
-    constexpr int E = 256 * V;
+    constexpr int E = NT * V;
     const int n = p.nx;
     const int rows_per = E / n;
     const int ld = seq_stride<T>(n);
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(256) fft_rows_kernel(const SplitParams<T> p) {
     const int row0 = blockIdx.x * rows_per;
     const int count = min(rows_per, total_rows - row0);
     if (count <= 0) return;
-    stage_twiddles<T>(s_tw, p.tw, n);
+    stage_twiddles<T, NT>(s_tw, p.tw, n);
     const int elems = count * n;
     const int64_t g0 = (int64_t)row0 * n;
     const bool res_here = RES != 0;
@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(256) fft_rows_kernel(const SplitParams<T> p) {
     constexpr int CH = stage_chunk<T, V>();
     if constexpr (MODE == ROW_SS_FIRST) {
         struct Pt { c2<T> v; T n[2]; T P; };
-        staged<Pt, CH>(
+        staged<Pt, CH, NT>(
             elems,
             [&](int i) {
                 Pt q;
@@ -166,16 +166,16 @@ __global__ void __launch_bounds__(256) fft_rows_kernel(const SplitParams<T> p) {
                 buf[slot(i)] = q.v;
             });
     } else {
-        staged<c2<T>, CH>(elems, [&](int i) { return p.in[g0 + i]; }, [&](int i, c2<T> v) { buf[slot(i)] = v; });
+        staged<c2<T>, CH, NT>(elems, [&](int i) { return p.in[g0 + i]; }, [&](int i, c2<T> v) { buf[slot(i)] = v; });
     }
     __syncthreads();
     const bool first_inverse = (MODE == ROW_INV || MODE == ROW_SS_MID || MODE == ROW_SS_LAST || MODE == ROW_KO_MID);
-    lds_fft<T, V>(buf, p.dims, ld, count, s_tw, first_inverse);
+    lds_fft<T, V, NT>(buf, p.dims, ld, count, s_tw, first_inverse);
 
     if constexpr (MODE == ROW_SS_MID || MODE == ROW_KO_MID) {
         // inverse done: scale, local physics, then forward again in place
         struct Pt { c2<T> v; T n[2]; T P1, P2; };
-        staged<Pt, CH>(
+        staged<Pt, CH, NT>(
             elems,
             [&](int i) {
                 Pt q;
@@ -218,13 +218,13 @@ __global__ void __launch_bounds__(256) fft_rows_kernel(const SplitParams<T> p) {
                 buf[slot(i)] = v;
             });
         __syncthreads();
-        lds_fft<T, V>(buf, p.dims, ld, count, s_tw, false);
+        lds_fft<T, V, NT>(buf, p.dims, ld, count, s_tw, false);
     }
 
     // ---- store (+ epilogue) ----
     if constexpr (MODE == ROW_SS_LAST) {
         struct Pt { c2<T> v; T n[2]; T P; };
-        staged<Pt, CH>(
+        staged<Pt, CH, NT>(
             elems,
             [&](int i) {
                 Pt q;
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(256) fft_rows_kernel(const SplitParams<T> p) {
                 p.out[g0 + i] = v;
             });
     } else {
-        staged<c2<T>, CH>(
+        staged<c2<T>, CH, NT>(
             elems, [&](int i) { return buf[slot(i)]; },
             [&](int i, c2<T> v) {
                 if constexpr (MODE == ROW_INV) {
@@ -261,11 +261,11 @@ __global__ void __launch_bounds__(256) fft_rows_kernel(const SplitParams<T> p) {
 }
 
 // ---------------------------------------------------------------- columns
-template <typename T, int V, int MODE>
-__global__ void __launch_bounds__(256) fft_cols_kernel(const SplitParams<T> p) {
+template <typename T, int V, int MODE, int NT>
+__global__ void __launch_bounds__(NT) fft_cols_kernel(const SplitParams<T> p) {
     extern __shared__ unsigned char smem_raw[];
     c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
-    constexpr int E = 256 * V;
+    constexpr int E = NT * V;
     const int n = p.ny;
     const int cols_per = E / n;
     const int ld = seq_stride<T>(n);
@@ -280,11 +280,11 @@ __global__ void __launch_bounds__(256) fft_cols_kernel(const SplitParams<T> p) {
     const int count = min(cols_per, p.nx - x0);
     const int64_t base = (int64_t)plane * n * p.nx;
     const FastDiv by_count((uint32_t)count), by_n((uint32_t)n);
-    stage_twiddles<T>(s_tw, p.tw, n);
+    stage_twiddles<T, NT>(s_tw, p.tw, n);
     // load column segments: element (y, x0 + c) -> buf[c * ld + y]
     constexpr int CH = stage_chunk<T, V>();
     struct Pt { c2<T> v; int at; };
-    staged<Pt, CH>(
+    staged<Pt, CH, NT>(
         count * n,
         [&](int i) {
             const int y = by_count.div(i);
@@ -293,14 +293,14 @@ __global__ void __launch_bounds__(256) fft_cols_kernel(const SplitParams<T> p) {
         },
         [&](int, Pt q) { buf[q.at] = q.v; });
     __syncthreads();
-    lds_fft<T, V>(buf, p.dims, ld, count, s_tw, MODE == COL_INV);
+    lds_fft<T, V, NT>(buf, p.dims, ld, count, s_tw, MODE == COL_INV);
     if constexpr (MODE == COL_PROP) {
         // x exp(-i hbar k^2 h / (2 m)), evaluated in double as the reference does
         const pgpe_member& mb = p.table[plane];
         const double hbar = mb.hbar;
         const double inv2m = 1.0 / (2 * mb.m_eff);
         struct K { double kx, ky; };
-        staged<K, CH>(
+        staged<K, CH, NT>(
             count * n, [&](int i) { const int c = by_n.div(i); return K{p.kx[x0 + c], p.ky[i - c * n]}; },
             [&](int i, K q) {
                 const int c = by_n.div(i);
@@ -315,9 +315,9 @@ __global__ void __launch_bounds__(256) fft_cols_kernel(const SplitParams<T> p) {
                 buf[c * ld + pad_at(y)] = make_c<T>((T)re, (T)im);
             });
         __syncthreads();
-        lds_fft<T, V>(buf, p.dims, ld, count, s_tw, true);
+        lds_fft<T, V, NT>(buf, p.dims, ld, count, s_tw, true);
     }
-    staged<Pt, CH>(
+    staged<Pt, CH, NT>(
         count * n,
         [&](int i) {
             const int y = by_count.div(i);
@@ -347,9 +347,9 @@ static void wavenumbers(int n, double d, std::vector<double>& k) {
     }
 }
 
-template <typename T, int V, int MODE, int RES>
+template <typename T, int V, int MODE, int RES, int NT>
 static int launch_rows_v(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipStream_t s) {
-    constexpr int E = 256 * V;
+    constexpr int E = NT * V;
     SplitParams<T> p = p0;
     p.dims = plan->dims_x[radix_class<V>()];
     p.tw = (const c2<T>*)plan->tw_x[radix_class<V>()];
@@ -357,81 +357,85 @@ static int launch_rows_v(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hi
     const int total_rows = p.planes * p.ny;
     const dim3 grid((total_rows + rows_per - 1) / rows_per);
     const size_t shm = (size_t)(rows_per * seq_stride<T>(plan->nx) + plan->nx) * sizeof(c2<T>);
-    auto kern = fft_rows_kernel<T, V, MODE, RES>;
+    auto kern = fft_rows_kernel<T, V, MODE, RES, NT>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(fft_rows_kernel)");
     }
-    hipLaunchKernelGGL(kern, grid, dim3(256), shm, s, p);
+    hipLaunchKernelGGL(kern, grid, dim3(NT), shm, s, p);
     PGPE_LAUNCHED("fft_rows_kernel");
     return PGPE_OK;
 }
 
 template <typename T, int MODE, int RES>
-static int launch_rows(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipStream_t s) {
-    const SplitParams<T>& p = p0;
-    switch (plan->rv) {
-        case 4: return launch_rows_v<T, 4, MODE, RES>(plan, p, s);
-        case 8: return launch_rows_v<T, 8, MODE, RES>(plan, p, s);
-        case 12: return launch_rows_v<T, 12, MODE, RES>(plan, p, s);
-        default: return launch_rows_v<T, 16, MODE, RES>(plan, p, s);
+static int launch_rows(const pgpe_fft_plan* plan, const SplitParams<T>& p, hipStream_t s) {
+    switch (plan->row_shape) {
+        case ROWS_64x4: return launch_rows_v<T, 4, MODE, RES, 64>(plan, p, s);
+        case ROWS_128x4: return launch_rows_v<T, 4, MODE, RES, 128>(plan, p, s);
+        case ROWS_256x4: return launch_rows_v<T, 4, MODE, RES, 256>(plan, p, s);
+        case ROWS_256x8: return launch_rows_v<T, 8, MODE, RES, 256>(plan, p, s);
+        case ROWS_256x12: return launch_rows_v<T, 12, MODE, RES, 256>(plan, p, s);
+        default: return launch_rows_v<T, 16, MODE, RES, 256>(plan, p, s);
     }
 }
 
-template <typename T, int V, int MODE>
+template <typename T, int V, int MODE, int NT>
 static int launch_cols_v(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipStream_t s) {
-    constexpr int E = 256 * V;
+    constexpr int E = NT * V;
     SplitParams<T> p = p0;
     p.dims = plan->dims_y[radix_class<V>()];
     p.tw = (const c2<T>*)plan->tw_y[radix_class<V>()];
     const int cols_per = E / plan->ny;
     const int groups = (plan->nx + cols_per - 1) / cols_per;
     const size_t shm = (size_t)(cols_per * seq_stride<T>(plan->ny) + plan->ny) * sizeof(c2<T>);
-    auto kern = fft_cols_kernel<T, V, MODE>;
+    auto kern = fft_cols_kernel<T, V, MODE, NT>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(fft_cols_kernel)");
     }
-    hipLaunchKernelGGL(kern, dim3(xcd_grid(groups * p.planes)), dim3(256), shm, s, p);
+    hipLaunchKernelGGL(kern, dim3(xcd_grid(groups * p.planes)), dim3(NT), shm, s, p);
     PGPE_LAUNCHED("fft_cols_kernel");
     return PGPE_OK;
 }
 
+// Column group shapes (threads x elements per thread), in the order they are tried.
+constexpr int kColShapes[][2] = {{64, 4}, {128, 4}, {256, 4}, {256, 8}, {256, 16}, {256, 32}, {512, 32}};
+
 // Column groups: as many columns per group as make 64-byte row segments (8 complex64 or
-// 4 complex128) when the batch has enough columns for >= 256 groups, fewer on small batches
-// (more groups instead); the smallest elements-per-thread count that holds them, with the
-// padded columns and the staged twiddles within the 160 KB of LDS.
+// 4 complex128) when the batch has enough columns for >= 512 groups (two per CU), fewer on
+// small batches (more groups instead); the first shape that holds them, with the padded
+// columns and the staged twiddles within the 160 KB of LDS.  Lengths with a radix 3 take
+// 256 x 12.  Returns an index into kColShapes, 7 for 256 x 12, or -1.
 template <typename T>
-static int column_v(const pgpe_fft_plan* plan, int planes) {
+static int column_shape(const pgpe_fft_plan* plan, int planes) {
     const int n = plan->ny;
-    if (n % 3 == 0) return 12;
+    auto lds = [&](int cols) { return (size_t)(cols * seq_stride<T>(n) + n) * sizeof(c2<T>); };
+    if (n % 3 == 0) return 256 * 12 >= n && lds(256 * 12 / n) <= 163840 ? 7 : -1;
     const int cols_total = plan->nx * planes;
     int want = sizeof(T) == 4 ? 8 : 4;
-    while (want > 1 && cols_total / want < 256) want /= 2;
-    int best = 0;
-    for (int v = 4; v <= 64; v *= 2) {
-        const int cols = 256 * v / n;
-        if (cols < 1 || (v == 64 && n < 1024)) continue;
-        const size_t shm = (size_t)(cols * seq_stride<T>(n) + n) * sizeof(c2<T>);
-        if (shm > 163840) break;
-        best = v;
+    while (want > 1 && cols_total / want < 512) want /= 2;
+    int best = -1;
+    for (int k = 0; k < 7; ++k) {
+        const int cols = kColShapes[k][0] * kColShapes[k][1] / n;
+        if (cols < 1) continue;
+        if (lds(cols) > 163840) break;
+        best = k;
         if (cols >= want) break;
     }
     return best;
 }
 
 template <typename T, int MODE>
-static int launch_cols(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipStream_t s) {
-    const SplitParams<T>& p = p0;
-    switch (column_v<T>(plan, p.planes)) {
-        case 4: return launch_cols_v<T, 4, MODE>(plan, p, s);
-        case 8: return launch_cols_v<T, 8, MODE>(plan, p, s);
-        case 12: return launch_cols_v<T, 12, MODE>(plan, p, s);
-        case 16: return launch_cols_v<T, 16, MODE>(plan, p, s);
-        case 32: return launch_cols_v<T, 32, MODE>(plan, p, s);
-        case 64:
-            if constexpr (sizeof(T) == 4) return launch_cols_v<T, 64, MODE>(plan, p, s);
-            [[fallthrough]];
+static int launch_cols(const pgpe_fft_plan* plan, const SplitParams<T>& p, hipStream_t s) {
+    switch (column_shape<T>(plan, p.planes)) {
+        case 0: return launch_cols_v<T, 4, MODE, 64>(plan, p, s);
+        case 1: return launch_cols_v<T, 4, MODE, 128>(plan, p, s);
+        case 2: return launch_cols_v<T, 4, MODE, 256>(plan, p, s);
+        case 3: return launch_cols_v<T, 8, MODE, 256>(plan, p, s);
+        case 4: return launch_cols_v<T, 16, MODE, 256>(plan, p, s);
+        case 5: return launch_cols_v<T, 32, MODE, 256>(plan, p, s);
+        case 6: return launch_cols_v<T, 32, MODE, 512>(plan, p, s);
+        case 7: return launch_cols_v<T, 12, MODE, 256>(plan, p, s);
         default: return fail(PGPE_ERR_UNSUPPORTED, "fft: column too tall for one LDS group");
     }
 }
@@ -529,7 +533,7 @@ int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pg
     plan->precision = precision;
     plan->vx = vx;
     plan->vy = vy;
-    plan->rv = small_v(nx, nx);
+    plan->row_shape = row_shape(nx);
     // pass lists: x classes 0..2, then y classes 0..2, in one device block
     std::vector<FftPass> lists[6], all;
     for (int c = 0; c < 3; ++c) {

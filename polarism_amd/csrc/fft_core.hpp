@@ -1,7 +1,7 @@
 // In-LDS mixed-radix (2, 3, 4, 8, 16) Stockham FFT used by the row and column kernels.
 //
-// A workgroup of 256 threads transforms `count` sequences of length n that
-// sit in LDS.  Each pass is the Stockham autosort iteration: butterfly j of
+// A workgroup of NT threads (256 unless a launcher picks 64, 128 or 512) transforms
+// `count` sequences of length n that sit in LDS.  Each pass is the Stockham autosort iteration: butterfly j of
 // radix r reads r inputs n/r apart, multiplies input q by w^(q (j mod Ns)) with
 // w = exp(-+2 pi i / (Ns r)), runs an r-point DFT in registers and writes the
 // outputs Ns apart at (j / Ns) Ns r + (j mod Ns).  The pass list comes from the
@@ -48,18 +48,18 @@ struct FftDims {
 // issues its CH loads before it waits on the first, so the group keeps CH x 256 loads in
 // flight (one at a time per thread when load and store alternate, which leaves the
 // few-workgroup column passes bound by memory latency).  load(i) -> P, store(i, P).
-template <typename P, int CH, typename Load, typename Store>
+template <typename P, int CH, int NT = kFftThreads, typename Load, typename Store>
 __device__ __forceinline__ void staged(int elems, Load load, Store store) {
-    for (int i0 = 0; i0 < elems; i0 += kFftThreads * CH) {
+    for (int i0 = 0; i0 < elems; i0 += NT * CH) {
         P r[CH];
 #pragma unroll
         for (int k = 0; k < CH; ++k) {
-            const int i = i0 + k * kFftThreads + (int)threadIdx.x;
+            const int i = i0 + k * NT + (int)threadIdx.x;
             if (i < elems) r[k] = load(i);
         }
 #pragma unroll
         for (int k = 0; k < CH; ++k) {
-            const int i = i0 + k * kFftThreads + (int)threadIdx.x;
+            const int i = i0 + k * NT + (int)threadIdx.x;
             if (i < elems) store(i, r[k]);
         }
     }
@@ -232,14 +232,15 @@ __device__ __forceinline__ void dft_radix(c2<T>* v, bool inverse) {
 // at ns - 1 + jm (R - 1) + q - 1 for jm < ns, 1 <= q < R, so a butterfly reads its R - 1
 // factors from consecutive slots and neighbouring butterflies from neighbouring runs.
 // Callers put a barrier between this and lds_fft.
-template <typename T>
+template <typename T, int NT = kFftThreads>
 __device__ __forceinline__ void stage_twiddles(c2<T>* s_tw, const c2<T>* __restrict__ tw, int n) {
-    staged<c2<T>, (sizeof(T) == 4 ? 16 : 8)>(n - 1, [&](int i) { return tw[i]; }, [&](int i, c2<T> w) { s_tw[i] = w; });
+    staged<c2<T>, (sizeof(T) == 4 ? 16 : 8), NT>(n - 1, [&](int i) { return tw[i]; },
+                                                 [&](int i, c2<T> w) { s_tw[i] = w; });
 }
 
-// One Stockham pass over `count` sequences ld apart; V = elements per thread; tw = this
-// pass's staged twiddles.
-template <typename T, int R, int V>
+// One Stockham pass over `count` sequences ld apart by a group of NT threads; V = elements
+// per thread; tw = this pass's staged twiddles.
+template <typename T, int R, int V, int NT = kFftThreads>
 __device__ __forceinline__ void stockham_pass(c2<T>* buf, int n, int ld, int count, const FftPass& ps,
                                               const c2<T>* __restrict__ tw, bool inverse) {
     constexpr int B = V / R;
@@ -251,7 +252,7 @@ __device__ __forceinline__ void stockham_pass(c2<T>* buf, int n, int ld, int cou
     int base[B], k0[B];
 #pragma unroll
     for (int i = 0; i < B; ++i) {
-        const int b = threadIdx.x + i * kFftThreads;
+        const int b = threadIdx.x + i * NT;
         base[i] = -1;
         if (b < total) {
             const int s = by_nb.div(b);
@@ -286,24 +287,24 @@ __device__ __forceinline__ void stockham_pass(c2<T>* buf, int n, int ld, int cou
 
 // Sequence s occupies buf[s * ld + pad_at(k)], k < n, ld >= pad_at(n - 1) + 1; tw holds the
 // staged twiddles of the plan d, whose radices divide V (the launchers pick the plan by V).
-template <typename T, int V>
+template <typename T, int V, int NT = kFftThreads>
 __device__ __forceinline__ void lds_fft(c2<T>* buf, const FftDims& d, int ld, int count,
                                         const c2<T>* __restrict__ tw, bool inverse) {
     for (int p = 0; p < d.npass; ++p) {
         const FftPass ps = d.passes[p];
         const c2<T>* t = tw + (ps.ns - 1);
         if (ps.radix == 16) {
-            if constexpr (V % 16 == 0 && V < 64) stockham_pass<T, 16, V>(buf, d.n, ld, count, ps, t, inverse);
+            if constexpr (V % 16 == 0 && V < 64) stockham_pass<T, 16, V, NT>(buf, d.n, ld, count, ps, t, inverse);
         } else if (ps.radix == 8) {
-            if constexpr (V % 8 == 0) stockham_pass<T, 8, V>(buf, d.n, ld, count, ps, t, inverse);
+            if constexpr (V % 8 == 0) stockham_pass<T, 8, V, NT>(buf, d.n, ld, count, ps, t, inverse);
         } else if (ps.radix == 4) {
-            if constexpr (V % 4 == 0) stockham_pass<T, 4, V>(buf, d.n, ld, count, ps, t, inverse);
+            if constexpr (V % 4 == 0) stockham_pass<T, 4, V, NT>(buf, d.n, ld, count, ps, t, inverse);
         } else if (ps.radix == 2) {
             // V = 64 serves only columns of 1024 or more (the launchers guarantee it), whose
             // plans hold radices 8 and 4: 32 radix-2 butterflies per thread would spill registers
-            if constexpr (V < 64) stockham_pass<T, 2, V>(buf, d.n, ld, count, ps, t, inverse);
+            if constexpr (V < 64) stockham_pass<T, 2, V, NT>(buf, d.n, ld, count, ps, t, inverse);
         } else {
-            if constexpr (V % 3 == 0) stockham_pass<T, 3, V>(buf, d.n, ld, count, ps, t, inverse);
+            if constexpr (V % 3 == 0) stockham_pass<T, 3, V, NT>(buf, d.n, ld, count, ps, t, inverse);
         }
     }
 }

@@ -18,8 +18,8 @@ struct pgpe_fft_plan {
     double* kx;             // 2 pi fftfreq(nx, dx)
     double* ky;
     int vx, vy;   // elements per thread of the Lawson row / column kernels (16, or 12 with a radix 3)
-    int rv;       // elements per thread of the FFT / split-step row kernels: the fewest that hold
-                  // one row (more workgroups on small grids); column groups are sized per launch
+    int row_shape;  // threads x elements per thread of the FFT / split-step row kernels
+                    // (pgpe::RowShape); column groups are sized per launch
 };
 
 namespace pgpe {
@@ -68,13 +68,19 @@ inline void pass_twiddles(int n, const std::vector<FftPass>& passes, const std::
     }
 }
 
-// Elements per thread of a 256-thread group holding `need` elements: 4, 8, 16 (power-of-two
-// lengths) or 12 (lengths with a radix 3).
-inline int small_v(int n, int need) {
-    if (n % 3 == 0) return 12;
-    for (int v = 4; v <= 16; v *= 2)
-        if (need <= 256 * v) return v;
-    return 16;
+// Row group shapes (threads x elements per thread).
+enum RowShape { ROWS_64x4, ROWS_128x4, ROWS_256x4, ROWS_256x8, ROWS_256x12, ROWS_256x16 };
+
+// One row per group at 4 elements per thread on 64, 128 or 256 threads (n <= 1024: more,
+// smaller workgroups on small grids, several rows per group below 256), then 8 or 16
+// elements per thread; lengths with a radix 3 take 256 x 12.
+inline int row_shape(int n) {
+    if (n % 3 == 0) return ROWS_256x12;
+    if (n <= 256) return ROWS_64x4;
+    if (n <= 512) return ROWS_128x4;
+    if (n <= 1024) return ROWS_256x4;
+    if (n <= 2048) return ROWS_256x8;
+    return ROWS_256x16;
 }
 
 }  // namespace pgpe
