@@ -56,11 +56,11 @@ SWEEP = dict(members=256, n=256, L=128.0, dt=0.05, check_every=1.0, rtol=1e-6, m
 WEIGHTS = os.path.join(ROOT, "polarism_amd", "surrogate", "weights", "fno_steady_256.npz")
 
 
-def run_sweep(members, tile_h="auto"):
+def run_sweep(members, tile_h="auto", graphs=True):
     from polarism_amd.sweep import PackedSweep
     sw = PackedSweep(members, reservoir="single", precision="single", dt=SWEEP["dt"],
                      check_every=SWEEP["check_every"], rtol=SWEEP["rtol"], max_time=SWEEP["max_time"],
-                     tile_h=tile_h, progress=lambda m: print(m, file=sys.stderr, flush=True))
+                     tile_h=tile_h, graphs=graphs, progress=lambda m: print(m, file=sys.stderr, flush=True))
     return sw.run()
 
 
@@ -424,6 +424,9 @@ def main():
     ap.add_argument("--members", type=int, default=SWEEP["members"])
     ap.add_argument("--legs", default="sweep,copy,roofline,uneven,fft,ffthbm,lawson,cpu")
     ap.add_argument("--weights", default=WEIGHTS)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the sweep's steps one by one instead of as captured graph chunks (same "
+                         "kernels; for kernel-trace profilers that cannot follow graph launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (the sweep would shard "
@@ -442,18 +445,18 @@ def main():
     have_weights = os.path.exists(args.weights)
     if "sweep" in legs:
         for _ in range(args.warmup):
-            run_sweep(ws.make_members(8, SWEEP["seed"] + 1))
+            run_sweep(ws.make_members(8, SWEEP["seed"] + 1), graphs=not args.no_graphs)
         plain_runs, warm_runs, infer_s = [], [], []
         t_all = time.perf_counter()
         for _ in range(args.steps):
-            plain_runs.append(run_sweep(members))
+            plain_runs.append(run_sweep(members, graphs=not args.no_graphs))
             if have_weights:
                 params, cfg, norm = ws.load_weights(args.weights)
                 t0 = time.perf_counter()
                 y = ws.predict(params, ws.features(members), cfg)
                 wm = ws.warm_start(members, y, norm)
                 infer_s.append(time.perf_counter() - t0)
-                warm_runs.append(run_sweep(wm))
+                warm_runs.append(run_sweep(wm, graphs=not args.no_graphs))
         wall = time.perf_counter() - t_all
         plain = summarise(plain_runs[-1])
         sweep = {"plain": plain, "warm": None}
