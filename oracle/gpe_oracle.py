@@ -19,6 +19,7 @@ It restates, in plain NumPy, the reference's CPU algorithms for the hot path
 * split-step FFT, CPU order -- solver/split_step_fft_solver.py:54-159
 * split-step FFT, K-outer order -- tests/crosscheck/test_split_step_fft_cuda.py:46-70
 * Lawson IFRK generic step + embedded error -- solver/ifrk_fft_cuda_solver.py:96-293
+* interaction-picture RK4 -- solver/ip_rk4_solver.py:122-213
 * PI step controller -- solver/step_controller.py:48-81
 * spinor dispersion and 2x2 exponential -- dispersion.py:34-89
 * steady-state test -- simulation_controller.py:710-734
@@ -32,6 +33,9 @@ Pinning (what checks each function; tests/ file in brackets):
   eta 0 and 0.04; complex potential; pulsed pump; f32/f64): the reference's
   golden trajectories tests/data/ifrk4_reference/reference.npz
   [test_oracle_golden.py]; 4th order on the uniform closed form [test_oracle_pins.py].
+* ip_rk4_step: equals LawsonProblem.step with the rk4 tableau (the same method written
+  with exp(+-L h/2) factors) to rounding, scalar and spinor, every reservoir kind
+  [test_oracle_pins.py]; through it, the golden trajectories above.
 * LawsonProblem adaptive + error + StepController + lawson_advance_adaptive:
   closed-form uniform decay within 50 rtol (tests/unit/test_adaptive_dt.py:34-38),
   rejection / tick landing, factor bounds, underflow guard [test_oracle_pins.py].
@@ -696,6 +700,50 @@ class LawsonProblem:
         else:
             member = np.sqrt(err5 / count)
         return float(np.max(member))
+
+
+def ip_rk4_step(prob: LawsonProblem, psi0, res0, pump_at, t, dt):
+    """One interaction-picture RK4 step (solver/ip_rk4_solver.py:122-213): the state is carried
+    as u = exp(-L s) psi^k; stages at the midpoint use exp(+-L dt/2), the end exp(+-L dt); the
+    propagators are formed in complex128 and rounded to the field precision
+    (ip_rk4_solver.py:45-71).  Reservoir stages are coupled as in the Lawson step."""
+    p, g, cplx = prob.p, prob.grid, prob.cplx
+    fft = lambda a: np.fft.fft2(a, axes=(-2, -1)).astype(cplx, copy=False)
+    ifft = lambda a: np.fft.ifft2(a, axes=(-2, -1)).astype(cplx, copy=False)
+    spinor = psi0.shape[1] == 2
+    if spinor:
+        E = {s: spinor_propagator(g, p, s * dt).astype(cplx) for s in (0.5, 1.0, -0.5, -1.0)}
+        lin = lambda s, x: apply_2x2(E[s], x).astype(cplx, copy=False)
+    else:
+        L = -1j * p.hbar * g.k_squared / (2 * p.m_eff)
+        E = {s: np.exp(L * (s * dt)).astype(cplx) for s in (0.5, 1.0, -0.5, -1.0)}
+        lin = lambda s, x: (E[s] * x).astype(cplx, copy=False)
+    rho_of = lambda a: a.real * a.real + a.imag * a.imag
+    p0, p_half, p1 = pump_at(t), pump_at(t + 0.5 * dt), pump_at(t + dt)
+
+    def stage(psi, psi_k, res, pump):
+        rho = rho_of(psi)
+        return fft(prob.rhs_psi(psi, res, rho, psi_k)), prob.res_derivs(psi, pump, res, rho)
+
+    shift = lambda res, ks, h: tuple((r + h * k).astype(prob.real, copy=False) for r, k in zip(res, ks))
+    psi0_k = fft(psi0)
+    k1, kr1 = stage(psi0, psi0_k, res0, p0)
+    mid_k = lin(0.5, psi0_k + (dt / 2) * k1)
+    res_mid = shift(res0, kr1, 0.5 * dt)
+    n2, kr2 = stage(ifft(mid_k), mid_k, res_mid, p_half)
+    k2 = lin(-0.5, n2)
+    mid2_k = lin(0.5, psi0_k + (dt / 2) * k2)
+    res_mid2 = shift(res0, kr2, 0.5 * dt)
+    n3, kr3 = stage(ifft(mid2_k), mid2_k, res_mid2, p_half)
+    k3 = lin(-0.5, n3)
+    end_k = lin(1.0, psi0_k + dt * k3)
+    res_end = shift(res0, kr3, dt)
+    n4, kr4 = stage(ifft(end_k), end_k, res_end, p1)
+    k4 = lin(-1.0, n4)
+    psi_new = ifft(lin(1.0, psi0_k + (dt / 6.0) * (k1 + 2 * k2 + 2 * k3 + k4)))
+    res_new = tuple(np.maximum(r + (dt / 6.0) * (a + 2 * b + 2 * c + d), 0).astype(prob.real, copy=False)
+                    for r, a, b, c, d in zip(res0, kr1, kr2, kr3, kr4))
+    return psi_new, res_new
 
 
 class StepController:

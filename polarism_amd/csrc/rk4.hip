@@ -232,11 +232,17 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
     // Wavefront w loads the interior rows w, w + 4, ... it later computes (one point per
     // lane, one coalesced 64-point segment per row) and keeps psi0 and k_prev of those rows
     // in registers for the final combination; waves 0 / 1 load the halo rows above / below
-    // (lanes 0 and 1 also the corners), waves 2 / 3 the halo columns left / right.
+    // (lanes 0 and 1 also the corners), waves 2 / 3 the halo columns left / right.  The
+    // reservoir and pump values of the wave's rows are loaded here too, so every stream's
+    // loads are in flight before the first wait (rows past the member's edge load the
+    // wrapped or mirrored point, in bounds and unused).
     constexpr int RPW = TH / 4;
+    constexpr int NFA = NF > 0 ? NF : 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int bc = mb.boundary;
+    const T* __restrict__ env = p.pump + mb.pump_off;
     c2<T> z[RPW], kp[RPW];
+    T r0a[RPW][NFA], b3a[RPW][NFA], envq[RPW];
     {
         const int gxi = map_index(x0 + lane, nx, bc);
 #pragma unroll
@@ -250,6 +256,12 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
                 v.x = v.x + p.a * kp[q].x;
                 v.y = v.y + p.a * kp[q].y;
             }
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                r0a[q][f] = ld_stream(&p.res0[f][off + g]);
+                if constexpr (KIND != 0) b3a[q][f] = ld_stream(&p.kres_prev[f][off + g]);
+            }
+            envq[q] = ld_stream(&env[g]);
             s_psi[(r + 1) * W + lane + 1] = v;
         }
         auto halo = [&](int ly, int lx) {
@@ -278,7 +290,6 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
     const bool nine = mb.laplacian == PGPE_LAP_NINE;
     const T inv_6h2 = kc.inv_dx2 / (T)6;
     const T coef = p.coef[m];
-    const T* __restrict__ env = p.pump + mb.pump_off;
     uint64_t step = p.step;
     if (KIND == 2 && p.noise && p.step_base) step += *p.step_base;
 
@@ -292,15 +303,15 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
         T n[2] = {(T)0, (T)0}, r0[2] = {(T)0, (T)0}, b3[2] = {(T)0, (T)0};
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            r0[f] = ld_stream(&p.res0[f][off + idx]);
+            r0[f] = r0a[q][f];
             T v = r0[f];
             if constexpr (KIND != 0) {
-                b3[f] = ld_stream(&p.kres_prev[f][off + idx]);
+                b3[f] = b3a[q][f];
                 v = v + p.a * b3[f];
             }
             n[f] = v;
         }
-        const T P = ld_stream(&env[idx]) * coef;
+        const T P = envq[q] * coef;
         const c2<T> ps = s_psi[c];
         const c2<T> lap = nine ? lap9<T>(ps, s_psi[c + W], s_psi[c - W], s_psi[c + 1], s_psi[c - 1], s_psi[c - W - 1],
                                          s_psi[c - W + 1], s_psi[c + W - 1], s_psi[c + W + 1], inv_6h2)

@@ -1,4 +1,4 @@
-"""``ifrk-fft-hip`` / ``ifrk4-fft-hip``: Lawson integrating-factor RK on gfx950.
+"""``ifrk-fft-hip`` / ``ifrk4-fft-hip`` / ``ip-rk4-hip``: Lawson integrating-factor RK on gfx950.
 
 Drop-in for the reference's ``ifrk-fft-cuda`` / ``ifrk4-fft-cuda``
 (solver/ifrk_fft_cuda_solver.py:24-336): stage-coupled reservoir, RK4, DP5 or
@@ -262,3 +262,17 @@ class IFRK4FFTHipSolver(IFRKFFTHipSolver):
     """The RK4 tableau regardless of ``solver.tableau`` (ifrk_fft_cuda_solver.py:332-336)."""
     capabilities = SolverCapabilities(**{**IFRKFFTHipSolver.capabilities.__dict__, "adaptive": False})
     tableau_name = "rk4"
+
+
+@register_solver("ip-rk4-hip")
+class IPRK4HipSolver(IFRK4FFTHipSolver):
+    """Drop-in for ``ip-rk4`` (solver/ip_rk4_solver.py:23-213).  The interaction-picture RK4
+    carries u = exp(-L s) psi^k and applies exp(+-L dt/2), exp(+-L dt) around each stage; that is
+    Lawson's RK4 (stage i: exp(L c_i h) psi0^k + h sum_j a_ij exp(L (c_i - c_j) h) f_j) with the
+    products of the factors taken in closed form, so it runs on the Lawson RK4 kernels: same
+    stage-coupled reservoir, spectral kinetic relaxation, scalar or TE-TM / Zeeman spinor.
+    The reference's coherent drive, extra terms and time-dependent potentials are not taken."""
+    capabilities = SolverCapabilities(**{
+        **IFRK4FFTHipSolver.capabilities.__dict__,
+        "description": "Interaction-picture RK4 (ip-rk4) on the Lawson RK4 kernels: stage-coupled reservoir, "
+                       "FFT-only, periodic grid, scalar or TE-TM/Zeeman spinor."})

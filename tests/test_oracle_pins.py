@@ -268,6 +268,43 @@ def test_lawson_fixed_rk4_is_fourth_order_on_uniform_decay():
     assert 13 < errs[0] / errs[1] < 19
 
 
+@pytest.mark.parametrize("kind", ["none", "single", "double", "quadratic-double"])
+@pytest.mark.parametrize("spinor", [False, True])
+def test_ip_rk4_equals_lawson_rk4(kind, spinor):
+    """The interaction-picture RK4 is Lawson's RK4 written with exp(+-L h/2) factors: on a
+    structured field with a Gaussian pump, kinetic relaxation and (spinor) TE-TM / Zeeman /
+    cross-spin terms, five steps agree to rounding, while a wrong stage propagator (exp(L h)
+    at the midpoint) is caught by the same comparison."""
+    g = O.Grid(32, 24, 24.0, 18.0)
+    p = O.Physics(kinetic_relaxation_eta=0.03, te_tm=0.05 if spinor else 0.0, zeeman=0.02 if spinor else 0.0,
+                  alpha_2=-1e-4 if spinor else 0.0)
+    C = 2 if spinor else 1
+    rng = np.random.default_rng(4)
+    psi = (0.3 * (rng.standard_normal((1, C) + g.shape) + 1j * rng.standard_normal((1, C) + g.shape)))
+    nf = {"none": 0, "single": 1, "double": 2, "quadratic-double": 2}[kind]
+    res = tuple(rng.uniform(0.0, 2.0, (1, C) + g.shape) for _ in range(nf))
+    env = 0.05 * O.gaussian_profile(g, 5.0)
+    pump_at = lambda t: env * (1.0 + 0.5 * t)
+    w = np.asarray([0.65, 0.35]).reshape(1, 2, 1, 1) if spinor else None
+    prob = O.LawsonProblem(kind, p, g, tableau="rk4", pump_weights=w, reservoir_components=C)
+    a_psi, a_res = psi, res
+    b_psi, b_res = psi, res
+    dt = 0.02
+    for s in range(5):
+        _, _, a_psi, a_res = prob.step(a_psi, a_res, pump_at, s * dt, dt)
+        b_psi, b_res = O.ip_rk4_step(prob, b_psi, b_res, pump_at, s * dt, dt)
+    assert O.rel_l2(b_psi, a_psi) <= 1e-13
+    for x, y in zip(b_res, a_res):
+        assert O.rel_l2(x, y) <= 1e-13
+    # a planted mistake: the midpoint stages propagated by a full step
+    bad = O.LawsonProblem(kind, p, g, tableau="rk4", pump_weights=w, reservoir_components=C)
+    lin = bad.linear
+    bad.linear = lambda fraction, h, fk: lin(1.0 if fraction == 0.5 else fraction, h, fk)
+    _, _, c_psi, _ = bad.step(psi, res, pump_at, 0.0, dt)
+    d_psi, _ = O.ip_rk4_step(prob, psi, res, pump_at, 0.0, dt)
+    assert O.rel_l2(d_psi, c_psi) > 1e-6
+
+
 def test_controller_rejects_then_lands_on_the_tick():
     g = O.Grid(4, 4, 4.0, 4.0)
     p = O.Physics(g_C=0.5, gamma_C=0.4)
