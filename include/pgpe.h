@@ -199,7 +199,7 @@ int pgpe_density_delta(const void* psi_dev, void* old_density_dev, int precision
                        double* partials_dev, double* sums_dev, int update, void* stream);
 
 /* ------------------------------------------------------------------ */
-/* Batched 2-D FFT (LDS Stockham, radix 2/3/4) and the fused split-step */
+/* Batched 2-D FFT (LDS Stockham, radix 2-16) and the fused split-step  */
 /* step (replaces cuFFT plans split_step_fft_cuda_solver.py:729-735,    */
 /* numpy.fft in split_step_fft_solver.py:70-79, and the local kernels   */
 /* split_step_fft_cuda_solver.py:106-184).                              */
@@ -219,18 +219,22 @@ typedef struct pgpe_split_args {
     int32_t reservoir;      /* pgpe_reservoir_kind */
     int32_t ordering;       /* 0: R K N noise K R (CPU solver order); 1: K [R N R noise] K */
     int32_t noise;          /* add the Wigner increment */
-    int32_t members;        /* planes of equal grids, one table row each */
-    int32_t pad0;
+    int32_t members;        /* members of equal grids, one table row each */
+    int32_t components;     /* 1 (scalar) or 2 (spinor psi+, psi-; 0 reads as 1) */
     double dt;
     const pgpe_member* table;
     const pgpe_fft_plan* plan;
-    void* psi;              /* in/out, members x (ny, nx) complex */
+    void* psi;              /* in/out, members x components x (ny, nx) complex */
     void* tmp;              /* scratch of the same size */
-    void* res[2];           /* in/out reservoir fields */
+    void* res[2];           /* in/out reservoir fields, members x res_components x (ny, nx) */
     const void* pump;       /* envelopes (real) */
     const void* coef;       /* 2 * members reals: pump coefficient at t+dt/4, then at t+3dt/4 */
     uint64_t step;          /* noise counter step = (*step_base if set) + step */
     const uint64_t* step_base;
+    int32_t res_components; /* spinor: 2 = one reservoir per component driven by its density and
+                               its weighted pump, 1 = one shared reservoir driven by the total
+                               density (split_step_fft_solver.py:81-94); 0 reads as 1 */
+    int32_t pad1;
 } pgpe_split_args;
 
 int pgpe_split_step(const pgpe_split_args* args, void* stream);

@@ -18,6 +18,8 @@ It restates, in plain NumPy, the reference's CPU algorithms for the hot path
 * RK4-FDM step -- solver/rk4_fdm_solver.py:36-77
 * split-step FFT, CPU order -- solver/split_step_fft_solver.py:54-159
 * split-step FFT, K-outer order -- tests/crosscheck/test_split_step_fft_cuda.py:46-70
+* spinor split-step -- solver/split_step_fft_solver.py:46-53, 81-94, 124-145;
+  K-outer spinor order -- solver/split_step_fft_cuda_solver.py:229-330
 * Lawson IFRK generic step + embedded error -- solver/ifrk_fft_cuda_solver.py:96-293
 * interaction-picture RK4 -- solver/ip_rk4_solver.py:122-213
 * PI step controller -- solver/step_controller.py:48-81
@@ -503,6 +505,72 @@ def split_step_kouter(kind, p, grid, psi, res, pump_at, t, dt, step, noise=None)
         active = res[0] if res else 0
         wigner_add(psi, active, dt=dt, step=step, **noise)
     psi = np.fft.ifft2(np.fft.fft2(psi) * half).astype(cdt)
+    return psi, res
+
+
+def _spinor_reservoir_half(kind, p, dt, psi, pump, res, weights, res_components):
+    """Spinor reservoir half-step (split_step_fft_solver.py:81-94): one reservoir per component
+    driven by |psi_c|^2 and P w_c, or one shared reservoir driven by |psi+|^2 + |psi-|^2 and P;
+    midpoint rule with psi frozen, then clamp >= 0."""
+    if kind == "none":
+        return ()
+    if res_components == 2:
+        return reservoir_rk2(kind, p, dt, psi, pump * weights, res)
+    rho = (psi.real * psi.real + psi.imag * psi.imag).sum(axis=1, keepdims=True)
+    k1 = reservoir_derivatives(kind, p, rho, pump, res)
+    mid = tuple(s + 0.5 * dt * k for s, k in zip(res, k1))
+    k2 = reservoir_derivatives(kind, p, rho, pump, mid)
+    return tuple(np.maximum(s + dt * k, 0).astype(s.dtype, copy=False) for s, k in zip(res, k2))
+
+
+def _spinor_nonlinear(p, psi, res, kind, dt):
+    """Exact local step of both components (split_step_fft_solver.py:124-143): the scalar step
+    per component plus the cross-spin phase alpha_2 |psi_o|^2 tau_o / hbar of the other one."""
+    nA, nI = active_inactive(kind, res)
+    rho = np.abs(psi) ** 2
+    gain = (p.R * nA - p.gamma_C) / 2.0
+    x = 2 * gain * dt
+    x = np.asarray(x, dtype=np.float64) if np.ndim(x) == 0 else x
+    small = np.abs(x) < 1e-6
+    phase_time = dt * np.where(small, 1 + x / 2 + x * x / 6, np.expm1(x) / np.where(small, 1, x))
+    phase = ((p.g_R * nA + p.g_I * nI) * dt + p.g_C * rho * phase_time) / p.hbar
+    other = np.broadcast_to(phase_time, rho.shape)[:, ::-1]
+    phase = phase + p.alpha_2 * rho[:, ::-1] * other / p.hbar
+    return psi * np.exp(gain * dt - 1j * phase)
+
+
+def _spinor_kinetic_half(grid, p, psi, dt):
+    """FFT, exp(-i H(k) dt / (2 hbar)) as a complex matrix of the field precision, IFFT
+    (split_step_fft_solver.py:46-53, 71-79)."""
+    cdt = psi.dtype
+    M = spinor_propagator(grid, p, dt / 2).astype(cdt)
+    psi_k = np.fft.fft2(psi, axes=(-2, -1)).astype(cdt)
+    return np.fft.ifft2(apply_2x2(M, psi_k), axes=(-2, -1)).astype(cdt)
+
+
+def split_step_spinor(kind, p, grid, psi, res, pump_at, t, dt, step, weights, res_components, noise=None,
+                      kouter=False):
+    """Spinor split step, (1, 2, ny, nx) psi.  CPU order R K N noise K R
+    (split_step_fft_solver.py:97-159); ``kouter``: K [R N R noise] K, the order of the
+    reference's spinor fast path (split_step_fft_cuda_solver.py:229-330)."""
+    cdt = psi.dtype
+    half = lambda r, pump: _spinor_reservoir_half(kind, p, dt / 2, psi, pump, r, weights, res_components)
+    if not kouter:
+        res = half(res, pump_at(t + dt / 4))
+        psi = _spinor_kinetic_half(grid, p, psi, dt)
+        psi = _spinor_nonlinear(p, psi, res, kind, dt).astype(cdt)
+        if noise is not None:
+            wigner_add(psi, res[0] if res else 0, dt=dt, step=step, **noise)
+        psi = _spinor_kinetic_half(grid, p, psi, dt)
+        res = half(res, pump_at(t + 3 * dt / 4))
+        return psi, res
+    psi = _spinor_kinetic_half(grid, p, psi, dt)
+    res = half(res, pump_at(t + dt / 4))
+    psi = _spinor_nonlinear(p, psi, res, kind, dt).astype(cdt)
+    res = _spinor_reservoir_half(kind, p, dt / 2, psi, pump_at(t + 3 * dt / 4), res, weights, res_components)
+    if noise is not None:
+        wigner_add(psi, res[0] if res else 0, dt=dt, step=step, **noise)
+    psi = _spinor_kinetic_half(grid, p, psi, dt)
     return psi, res
 
 

@@ -70,10 +70,11 @@ class SplitArgs(C.Structure):
     """Mirror of ``pgpe_split_args``."""
     _fields_ = [
         ("precision", C.c_int32), ("reservoir", C.c_int32), ("ordering", C.c_int32), ("noise", C.c_int32),
-        ("members", C.c_int32), ("pad0", C.c_int32),
+        ("members", C.c_int32), ("components", C.c_int32),
         ("dt", C.c_double),
         ("table", vp), ("plan", vp), ("psi", vp), ("tmp", vp), ("res", vp * 2),
         ("pump", vp), ("coef", vp), ("step", C.c_uint64), ("step_base", vp),
+        ("res_components", C.c_int32), ("pad1", C.c_int32),
     ]
 
 

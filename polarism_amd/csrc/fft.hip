@@ -22,98 +22,10 @@
 #include <new>
 #include <vector>
 
-#include "fft_core.hpp"
-#include "fft_plan.hpp"
+#include "split_common.hpp"
 
 namespace pgpe {
 
-enum RowMode { ROW_FWD = 0, ROW_INV = 1, ROW_SS_FIRST = 2, ROW_SS_MID = 3, ROW_SS_LAST = 4, ROW_KO_MID = 5 };
-enum ColMode { COL_FWD = 0, COL_INV = 1, COL_PROP = 2 };
-
-template <typename T>
-struct SplitParams {
-    const pgpe_member* table;
-    FftDims dims;
-    const c2<T>* tw;
-    const double* kx;
-    const double* ky;
-    c2<T>* in;
-    c2<T>* out;
-    int nx, ny, planes;
-    int reservoir;        // pgpe_reservoir_kind
-    int noise;
-    T dt;
-    double prop_h;        // propagator step (dt/2 for a half step)
-    T scale;              // inverse-transform normalisation
-    T* res[2];
-    const T* pump;
-    const T* coef;        // coef[0*members + m]: first half-step, coef[1*members + m]: second
-    int members;
-    uint64_t step;
-    const uint64_t* step_base;
-};
-
-template <typename T, int RES>
-__device__ __forceinline__ void res_rhs(const pgpe_member& mb, T rho, T P, const T n[2], T d[2]) {
-    if constexpr (RES == PGPE_RES_SINGLE) {
-        d[0] = P - ((T)mb.res_a + (T)mb.R * rho) * n[0];
-    } else if constexpr (RES == PGPE_RES_DOUBLE) {
-        d[1] = (P - (T)mb.res_a * n[1]) + (T)mb.res_b * n[0];
-        d[0] = (T)mb.res_c * n[1] - ((T)mb.res_d + (T)mb.R * rho) * n[0];
-    } else if constexpr (RES == PGPE_RES_QDOUBLE) {
-        const T transfer = (T)mb.res_a * n[1] * n[1];
-        d[0] = (transfer - (T)mb.res_b * n[0]) - (T)mb.R * n[0] * rho;
-        d[1] = (P - transfer) - (T)mb.res_c * n[1];
-    }
-}
-
-// Midpoint RK2 with psi frozen, then clamp at zero (single_reservoir.py:100-111).
-template <typename T, int RES>
-__device__ __forceinline__ void res_half_step(const pgpe_member& mb, T rho, T P, T h, T n[2]) {
-    constexpr int NF = RES == 0 ? 0 : (RES == 1 ? 1 : 2);
-    if constexpr (NF == 0) return;
-    T k1[2] = {0, 0}, k2[2] = {0, 0}, mid[2] = {0, 0};
-    res_rhs<T, RES>(mb, rho, P, n, k1);
-    const T hh = (T)0.5 * h;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) mid[f] = n[f] + hh * k1[f];
-    res_rhs<T, RES>(mb, rho, P, mid, k2);
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-        const T v = n[f] + h * k2[f];
-        n[f] = v > (T)0 ? v : (T)0;
-    }
-}
-
-// Exact local gain/phase step (split_step_fft_solver.py:124-140), zero potential.
-template <typename T>
-__device__ __forceinline__ void nonlinear_step(const pgpe_member& mb, T nA, T nI, T dt, c2<T>& v) {
-    const T rho = v.x * v.x + v.y * v.y;
-    const T gain = ((T)mb.R * nA - (T)mb.gamma_C) / (T)2;
-    const T x = (T)2 * gain * dt;
-    const T phase_time = fabs(x) < (T)1e-6 ? dt * (((T)1 + x / (T)2) + x * x / (T)6) : dt * (expm1(x) / x);
-    const T phase = (((T)mb.g_R * nA + (T)mb.g_I * nI) * dt + (T)mb.g_C * rho * phase_time) / (T)mb.hbar;
-    const T amp = exp(gain * dt);
-    T s, c;
-    sin_cos(phase, &s, &c);
-    // exp(gain dt - i phase) = amp (cos phase - i sin phase)
-    const c2<T> f = make_c<T>(amp * c, -(amp * s));
-    v = cmul<T>(v, f);
-}
-
-template <typename T, int RES>
-__device__ __forceinline__ void load_res(const SplitParams<T>& p, int64_t g, T n[2]) {
-    constexpr int NF = RES == 0 ? 0 : (RES == 1 ? 1 : 2);
-#pragma unroll
-    for (int f = 0; f < NF; ++f) n[f] = p.res[f][g];
-}
-
-template <typename T, int RES>
-__device__ __forceinline__ void store_res(const SplitParams<T>& p, int64_t g, const T n[2]) {
-    constexpr int NF = RES == 0 ? 0 : (RES == 1 ? 1 : 2);
-#pragma unroll
-    for (int f = 0; f < NF; ++f) p.res[f][g] = n[f];
-}
 
 // ---------------------------------------------------------------- rows
 template <typename T, int V, int MODE, int RES, int NT>
@@ -327,6 +239,7 @@ __global__ void __launch_bounds__(NT) fft_cols_kernel(const SplitParams<T> p) {
         [&](int, Pt q) { p.out[base + q.at] = q.v; });
 }
 
+
 template <typename T>
 static void twiddles(int n, std::vector<c2<T>>& out) {
     out.resize(n);
@@ -398,8 +311,6 @@ static int launch_cols_v(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hi
     return PGPE_OK;
 }
 
-// Column group shapes (threads x elements per thread), in the order they are tried.
-constexpr int kColShapes[][2] = {{64, 4}, {128, 4}, {256, 4}, {256, 8}, {256, 16}, {256, 32}, {512, 32}};
 
 // Column groups: as many columns per group as make 64-byte row segments (8 complex64 or
 // 4 complex128) when the batch has enough columns for >= 512 groups (two per CU), fewer on
@@ -464,28 +375,11 @@ static int fft2_typed(const pgpe_fft_plan* plan, const void* in, void* out, int 
     return launch_rows<T, ROW_INV, 0>(plan, p, s);
 }
 
+
 template <typename T, int RES>
 static int split_typed(const pgpe_split_args* a, hipStream_t s) {
     const pgpe_fft_plan* plan = (const pgpe_fft_plan*)a->plan;
-    SplitParams<T> p{};
-    p.table = a->table;
-    p.kx = plan->kx;
-    p.ky = plan->ky;
-    p.nx = plan->nx;
-    p.ny = plan->ny;
-    p.planes = a->members;
-    p.members = a->members;
-    p.reservoir = a->reservoir;
-    p.noise = a->noise;
-    p.dt = (T)a->dt;
-    p.prop_h = a->dt / 2.0;
-    p.scale = (T)(1.0 / ((double)plan->nx * (double)plan->ny));
-    p.res[0] = (T*)a->res[0];
-    p.res[1] = (T*)a->res[1];
-    p.pump = (const T*)a->pump;
-    p.coef = (const T*)a->coef;
-    p.step = a->step;
-    p.step_base = a->step_base;
+    SplitParams<T> p = split_params<T>(a);
     c2<T>* psi = (c2<T>*)a->psi;
     c2<T>* tmp = (c2<T>*)a->tmp;
     int rc;
@@ -608,12 +502,15 @@ int pgpe_split_step(const pgpe_split_args* a, void* stream) {
         return fail(PGPE_ERR_ARG, "pgpe_split_step: null argument");
     if (a->members <= 0) return fail(PGPE_ERR_ARG, "pgpe_split_step: members must be positive");
     if (a->ordering != 0 && a->ordering != 1) return fail(PGPE_ERR_ARG, "pgpe_split_step: ordering must be 0 or 1");
+    if (a->components < 0 || a->components > 2 || a->res_components < 0 || a->res_components > 2)
+        return fail(PGPE_ERR_ARG, "pgpe_split_step: components and res_components must be 1 or 2");
     const pgpe_fft_plan* plan = (const pgpe_fft_plan*)a->plan;
     if (plan->precision != a->precision) return fail(PGPE_ERR_ARG, "pgpe_split_step: plan precision mismatch");
     const int nf = a->reservoir == PGPE_RES_NONE ? 0 : (a->reservoir == PGPE_RES_SINGLE ? 1 : 2);
     for (int f = 0; f < nf; ++f)
         if (!a->res[f]) return fail(PGPE_ERR_ARG, "pgpe_split_step: null reservoir field");
     hipStream_t s = (hipStream_t)stream;
+    if (a->components == 2) return split_spinor_step(a, s);
     const bool dbl = a->precision == 1;
     switch (a->reservoir) {
         case PGPE_RES_NONE: return dbl ? split_typed<double, 0>(a, s) : split_typed<float, 0>(a, s);

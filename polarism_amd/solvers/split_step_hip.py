@@ -5,6 +5,11 @@
 R(dt/2) K(dt/2) N(dt) noise K(dt/2) R(dt/2), four transforms per step, noise
 drawn with the reservoir after the first half-step.
 
+A spinor field (psi+, psi-) takes the 2x2 TE-TM / Zeeman propagator exp(-i H(k) dt / 2 hbar)
+(dispersion.py:66-83) in the column passes, per-component or shared reservoirs and the
+cross-spin phase alpha_2 |psi_o|^2 in the local step (split_step_fft_solver.py:46-53, 81-94,
+141-143).
+
 ``split-step-fft-kouter-hip`` follows the order of the reference's
 ``split-step-fft-cuda`` (solver/split_step_fft_cuda_solver.py:106-184,
 1144-1325): K(dt/2) [R(dt/2) N(dt) R(dt/2) noise] K(dt/2), noise with the
@@ -50,8 +55,7 @@ class _SplitStepBase(HipSolverBase):
 
     def __init__(self, config, grid, model):
         super().__init__(config, grid, model)
-        if model.components != 1:
-            raise ValueError(f"{config.solver.method} is scalar")
+        self.res_components = getattr(model.reservoir, "components", 1)
         self.plan = FFTPlan(grid.nx, grid.ny, grid.dx, grid.dy, self.precision)
         self._psi = None
         self._tmp = None
@@ -63,11 +67,12 @@ class _SplitStepBase(HipSolverBase):
 
     def _ensure(self, model, batch):
         tab = self.ensure_table(model, batch)
-        n = tab.total_points
+        n = tab.total_points * self.components
         if self._psi is None or self._psi.size != n:
             self._psi = DeviceArray.empty((n,), self.cdtype)
             self._tmp = DeviceArray.empty((n,), self.cdtype)
-            self._res = [DeviceArray.empty((n,), self.rdtype) for _ in range(self.nres)]
+            nr = tab.total_points * self.res_components
+            self._res = [DeviceArray.empty((nr,), self.rdtype) for _ in range(self.nres)]
             self._graphs.clear()
             self._const = None
         return tab
@@ -96,6 +101,8 @@ class _SplitStepBase(HipSolverBase):
         a.ordering = self.ordering
         a.noise = 1 if noise else 0
         a.members = tab.members
+        a.components = self.components
+        a.res_components = self.res_components
         a.dt = float(self.config.solver.dt)
         a.table = tab.table_ptr
         a.plan = self.plan.handle
@@ -164,7 +171,7 @@ class _SplitStepBase(HipSolverBase):
             self._run(model, state, n_steps)
 
 
-_CAPS = dict(components=frozenset({1}), ndim=frozenset({2}), grid_types=frozenset({"periodic"}),
+_CAPS = dict(components=frozenset({1, 2}), ndim=frozenset({2}), grid_types=frozenset({"periodic"}),
              reservoirs=_ALL_RES, custom_reservoirs="unsupported", potential=False, kinetic_relaxation=False,
              dispersion=frozenset({"parabolic"}), terms=False, noise=frozenset({"none", "wigner"}),
              time_dependent_potential=False, quantitative_reservoirs=_ALL_RES)

@@ -268,6 +268,104 @@ def test_lawson_fixed_rk4_is_fourth_order_on_uniform_decay():
     assert 13 < errs[0] / errs[1] < 19
 
 
+# ---------------------------------------------------------------- spinor split-step
+def _structured(g, C, seed=5):
+    rng = np.random.default_rng(seed)
+    return 0.3 * (rng.standard_normal((1, C) + g.shape) + 1j * rng.standard_normal((1, C) + g.shape))
+
+
+@pytest.mark.parametrize("kouter", [False, True])
+@pytest.mark.parametrize("kind", ["single", "double", "quadratic-double"])
+def test_uncoupled_spinor_split_step_is_two_scalar_steps(kind, kouter):
+    """No TE-TM, Zeeman or cross-spin term and one reservoir per component: each component
+    advances as the scalar split step with its own reservoir and pump P w_c."""
+    g = O.Grid(32, 24, 24.0, 18.0)
+    p = O.Physics()
+    psi = _structured(g, 2)
+    nf = 1 if kind == "single" else 2
+    res = tuple(np.full((1, 2) + g.shape, 0.7) for _ in range(nf))
+    env = 0.05 * O.gaussian_profile(g, 5.0)
+    pump_at = lambda t: env * (1.0 + t)
+    w = np.asarray([0.7, 0.3]).reshape(1, 2, 1, 1)
+    a_psi, a_res = psi, res
+    scal = [(psi[:, c:c + 1], tuple(r[:, c:c + 1] for r in res)) for c in range(2)]
+    fn = O.split_step_kouter if kouter else O.split_step_cpu
+    for s in range(4):
+        a_psi, a_res = O.split_step_spinor(kind, p, g, a_psi, a_res, pump_at, s * 0.01, 0.01, s, w, 2, kouter=kouter)
+        scal = [fn(kind, p, g, ps, rs, (lambda t, c=c: pump_at(t) * w[:, c:c + 1]), s * 0.01, 0.01, s)
+                for c, (ps, rs) in enumerate(scal)]
+    for c in range(2):
+        np.testing.assert_allclose(a_psi[:, c:c + 1], scal[c][0], rtol=1e-12, atol=1e-14)
+        for x, y in zip(a_res, scal[c][1]):
+            np.testing.assert_allclose(x[:, c:c + 1], y, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("kouter", [False, True])
+def test_uniform_spinor_split_step_matches_closed_form(kouter):
+    """Uniform psi+ / psi- without reservoir: |psi_c|^2 = |psi_c0|^2 e^{-gamma t} and the phase
+    -(g rho_c0 + alpha_2 rho_o0)(1 - e^{-gamma t}) / (gamma hbar) -+ Z t / (2 hbar) solve the local
+    ODE exactly, and the k = 0 propagator adds only the Zeeman phase: the step is exact."""
+    g = O.Grid(8, 8, 8.0, 8.0)
+    p = O.Physics(g_C=2e-3, alpha_2=-8e-4, zeeman=0.03, te_tm=0.05, gamma_C=0.3)
+    a0, b0 = 0.8 + 0.3j, 0.4 - 0.5j
+    psi = np.empty((1, 2) + g.shape, dtype=complex)
+    psi[:, 0], psi[:, 1] = a0, b0
+    dt, n = 0.05, 12
+    for s in range(n):
+        psi, _ = O.split_step_spinor("none", p, g, psi, (), lambda t: 0.0, s * dt, dt, s, None, 2, kouter=kouter)
+    t = n * dt
+    decay = (1 - np.exp(-p.gamma_C * t)) / p.gamma_C
+    for c, (z0, o0, sgn) in enumerate(((a0, b0, -1), (b0, a0, 1))):
+        phase = -(p.g_C * abs(z0) ** 2 + p.alpha_2 * abs(o0) ** 2) * decay / p.hbar + sgn * p.zeeman * t / (2 * p.hbar)
+        exact = z0 * np.exp(-p.gamma_C * t / 2) * np.exp(1j * phase)
+        np.testing.assert_allclose(psi[0, c], exact, rtol=1e-12)
+
+
+def test_linear_spinor_split_step_is_the_te_tm_exponential():
+    """Without nonlinearity, gain or reservoir the step is exp(-i H(k) dt / hbar) per k: a
+    plane wave of psi- over five steps against scipy's expm of the 2 x 2 Hamiltonian."""
+    g = O.Grid(16, 16, 16.0, 16.0)
+    p = O.Physics(g_C=0.0, alpha_2=0.0, gamma_C=0.0, zeeman=0.02, te_tm=0.05)
+    kx, ky = 2 * np.pi * 3 / g.lx, 2 * np.pi * 1 / g.ly
+    wave = np.exp(1j * (kx * g.X + ky * g.Y))
+    psi = np.zeros((1, 2) + g.shape, dtype=complex)
+    psi[0, 1] = wave
+    dt, n = 0.1, 5
+    for s in range(n):
+        psi, _ = O.split_step_spinor("none", p, g, psi, (), lambda t: 0.0, s * dt, dt, s, None, 2)
+    E = p.hbar ** 2 * (kx * kx + ky * ky) / (2 * p.m_eff)
+    H = np.array([[E + p.zeeman / 2, p.te_tm * (kx - 1j * ky) ** 2],
+                  [p.te_tm * (kx + 1j * ky) ** 2, E - p.zeeman / 2]])
+    amp = expm(-1j * H * n * dt / p.hbar) @ np.array([0.0, 1.0])
+    np.testing.assert_allclose(psi[0, 0], amp[0] * wave, atol=1e-12)
+    np.testing.assert_allclose(psi[0, 1], amp[1] * wave, atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["single", "double"])
+def test_shared_spinor_reservoir_sees_the_total_density(kind):
+    """One shared reservoir and psi- = 0 (no coupling): psi+ and the reservoir advance as the
+    scalar split step; with psi- = psi+ the reservoir half-step sees twice the density."""
+    g = O.Grid(16, 16, 16.0, 16.0)
+    p = O.Physics()
+    one = _structured(g, 1)
+    nf = 1 if kind == "single" else 2
+    res = tuple(np.full((1, 1) + g.shape, 0.5) for _ in range(nf))
+    env = 0.05 * O.gaussian_profile(g, 4.0)
+    pump_at = lambda t: env
+    psi = np.concatenate((one, np.zeros_like(one)), axis=1)
+    a_psi, a_res = O.split_step_spinor(kind, p, g, psi, res, pump_at, 0.0, 0.01, 0, None, 1)
+    b_psi, b_res = O.split_step_cpu(kind, p, g, one, res, pump_at, 0.0, 0.01, 0)
+    np.testing.assert_allclose(a_psi[:, :1], b_psi, rtol=1e-12, atol=1e-14)
+    assert np.all(a_psi[:, 1] == 0)
+    for x, y in zip(a_res, b_res):
+        np.testing.assert_allclose(x, y, rtol=1e-12, atol=1e-14)
+    both = np.concatenate((one, one), axis=1)
+    c_res = O._spinor_reservoir_half(kind, p, 0.005, both, env, res, None, 1)
+    d_res = O.reservoir_rk2(kind, p, 0.005, np.sqrt(2.0) * one, env, res)
+    for x, y in zip(c_res, d_res):
+        np.testing.assert_allclose(x, y, rtol=1e-13)
+
+
 @pytest.mark.parametrize("kind", ["none", "single", "double", "quadratic-double"])
 @pytest.mark.parametrize("spinor", [False, True])
 def test_ip_rk4_equals_lawson_rk4(kind, spinor):

@@ -93,15 +93,10 @@ def test_adaptive_needs_an_embedded_tableau():
 
 
 @pytest.mark.parametrize("method", HIP_METHODS)
-def test_spinor_support_of_the_hip_methods(method):
-    """Spinor fields run on rk4-fdm-hip and the Lawson family; the split-step methods here are
-    scalar only (the reference's CPU split-step also takes spinors)."""
+def test_every_hip_method_takes_spinor_fields(method):
+    """Like the reference methods they replace, every HIP method accepts psi+ / psi- fields."""
     cfg = quickstart_config(solver__method=method, physics__spin="spinor")
-    if method.startswith("split-step"):
-        with pytest.raises(SolverCompatibilityError, match="2 components"):
-            check_solver_compatibility(cfg)
-    else:
-        check_solver_compatibility(cfg)
+    check_solver_compatibility(cfg)
 
 
 def test_kinetic_relaxation_needs_a_method_that_evaluates_it():
