@@ -199,6 +199,18 @@ int pgpe_density_delta(const void* psi_dev, void* old_density_dev, int precision
                        double* partials_dev, double* sums_dev, int update, void* stream);
 
 /* ------------------------------------------------------------------ */
+/* In-run accumulators (replace the device sums of                     */
+/* results/accumulators.py:94-188).  n points of a complex field; the   */
+/* sums are double.  pgpe_accumulate_power: sum[i] += |f[i]|^2 (the      */
+/* momentum distribution, f = FFT of the field).                        */
+/* pgpe_accumulate_moments: rho = |f[i]|^2; sum_rho[i] += rho,           */
+/* sum_rho2[i] += rho^2 (g2).  Components are widened to double first.  */
+/* ------------------------------------------------------------------ */
+int pgpe_accumulate_power(const void* field_dev, double* sum_dev, int64_t n, int precision, void* stream);
+int pgpe_accumulate_moments(const void* field_dev, double* sum_rho_dev, double* sum_rho2_dev, int64_t n,
+                            int precision, void* stream);
+
+/* ------------------------------------------------------------------ */
 /* Batched 2-D FFT (LDS Stockham, radix 2-16) and the fused split-step  */
 /* step (replaces cuFFT plans split_step_fft_cuda_solver.py:729-735,    */
 /* numpy.fft in split_step_fft_solver.py:70-79, and the local kernels   */

@@ -155,6 +155,8 @@ _SIGNATURES = {
     "pgpe_mode_scale_grad": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
     "pgpe_colsum": (C.c_int, [vp, vp, C.c_int64, C.c_int64, C.c_int64, C.c_int64, vp]),
     "pgpe_density_delta": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp]),
+    "pgpe_accumulate_power": (C.c_int, [vp, vp, C.c_int64, C.c_int, vp]),
+    "pgpe_accumulate_moments": (C.c_int, [vp, vp, vp, C.c_int64, C.c_int, vp]),
 }
 
 _lib = None

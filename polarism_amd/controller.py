@@ -49,6 +49,12 @@ def apply_overrides(cfg: Config, overrides: dict) -> Config:
     return out
 
 
+def _next_sample(step, start, every, stop):
+    """First sampling step after ``step`` (simulation_controller.py:1024-1026)."""
+    due = start if step < start else start + ((step - start) // every + 1) * every
+    return due if due <= stop else None
+
+
 def _steps(value, dt, what):
     n = round(value / dt)
     if n <= 0 or not math.isclose(n * dt, value, rel_tol=1e-9, abs_tol=1e-12):
@@ -122,6 +128,10 @@ class SimulationController:
                            pump_weights, getattr(self.reservoir, "components", 1), components,
                            self._member_cfgs)
         self.solver = create_solver(cfg, self.grid, self.model)
+        from .accumulators import available_accumulators
+        self._accumulators = {name: available_accumulators.lookup(name)(self, **params)
+                              for name, params in cfg.result.accumulators.items()}
+        self.accumulated: dict = {}
         self.stop_times = np.full(batch, np.nan)
         self._steady_old = None
         self._steady_norm = None
@@ -187,18 +197,43 @@ class SimulationController:
             for signum in (signal.SIGTERM, signal.SIGUSR1):
                 previous[signum] = signal.getsignal(signum)
                 signal.signal(signum, request_stop)
+        schedules = self._accumulator_schedules(dt, n_steps)
         try:
             step = self.state.step
+            if step == 0:
+                for acc, start, _, stop in schedules:
+                    if start == 0 and stop >= 0 and acc.samples == 0:
+                        acc.update(self.state, self.state.t)
             while step < n_steps and not self._stop_requested and not (steady and np.isfinite(self.stop_times).all()):
                 end = n_steps
                 if steady:
                     end = min(end, (step // steady + 1) * steady)
                 if previous:
                     end = min(end, step + 1000)
+                for _, start, every, stop in schedules:
+                    due = _next_sample(step, start, every, stop)
+                    if due is not None:
+                        end = min(end, due)
                 self.solver.advance(self.model, self.state, end - step)
                 step = self.state.step
+                for acc, start, every, stop in schedules:
+                    if start <= step <= stop and (step - start) % every == 0:
+                        acc.update(self.state, self.state.t)
                 if steady and step % steady == 0 and self._steady_last_step != step:
                     self._check_steady_state()
         finally:
             for signum, handler in previous.items():
                 signal.signal(signum, handler)
+            if self._accumulators:
+                self.accumulated = {result.name: result for acc in self._accumulators.values() if acc.samples
+                                    for result in acc.results()}
+
+    def _accumulator_schedules(self, dt, n_steps):
+        """(accumulator, start, every, stop) in steps (simulation_controller.py:1015-1022)."""
+        out = []
+        for acc in self._accumulators.values():
+            start = 0 if acc.start == 0 else _steps(acc.start, dt, "accumulator start")
+            every = _steps(acc.every or self.cfg.result.save_every or dt, dt, "accumulator every")
+            stop = n_steps if acc.stop is None else (0 if acc.stop == 0 else _steps(acc.stop, dt, "accumulator stop"))
+            out.append((acc, start, every, min(stop, n_steps)))
+        return out

@@ -33,6 +33,7 @@ def validate_config(cfg) -> None:
         raise ConfigValidationError("noise.stochastic must be 'none' or 'wigner'")
     if cfg.stop.steady_state and cfg.stop.check_every <= 0:
         raise ConfigValidationError("stop.check_every must be > 0")
+    _validate_accumulators(cfg.result.accumulators)
     if s.adaptive:
         from .solver_api import solver_capabilities
         from .tableaus import TABLEAUS
@@ -51,3 +52,22 @@ def validate_config(cfg) -> None:
                 raise ConfigValidationError(f"solver.{name} must be {'> 0' if positive else '>= 0'} and finite")
         if s.precision == "single" and s.rtol < 1e-5:
             raise ConfigValidationError("solver.rtol must be >= 1e-5 in single precision")
+
+
+def _validate_accumulators(accumulators) -> None:
+    """result.accumulators maps registered names to parameter mappings (config/validation.py:274-287)."""
+    from .accumulators import available_accumulators
+    if not isinstance(accumulators, dict):
+        raise ConfigValidationError("result.accumulators must map names to parameter mappings")
+    for name, params in accumulators.items():
+        try:
+            accumulator = available_accumulators.lookup(name)
+        except ValueError as exc:
+            raise ConfigValidationError(f"result.accumulators: {exc}") from exc
+        if not isinstance(params, dict):
+            raise ConfigValidationError(f"result.accumulators.{name} must be a parameter mapping")
+        unknown = set(params) - {"start", "stop", "every"} - accumulator.parameter_keys
+        if unknown:
+            raise ConfigValidationError(
+                f"result.accumulators.{name}: unknown keys {sorted(unknown)}; "
+                f"allowed: {sorted({'start', 'stop', 'every'} | accumulator.parameter_keys)}")
