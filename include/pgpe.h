@@ -209,6 +209,17 @@ int pgpe_density_delta(const void* psi_dev, void* old_density_dev, int precision
 int pgpe_accumulate_power(const void* field_dev, double* sum_dev, int64_t n, int precision, void* stream);
 int pgpe_accumulate_moments(const void* field_dev, double* sum_rho_dev, double* sum_rho2_dev, int64_t n,
                             int precision, void* stream);
+/* Time spectra (accumulators.py:173-270).  pgpe_spectrum_line: planes x nx line cut into a   */
+/* ring slot; mode 0 = row `row` of the (planes, ny, nx) field, mode 1 = row ky = 0 of its     */
+/* 2-D transform times scale, in fftshift order along x.  pgpe_spectrum_block: the ring of     */
+/* segment slots of width values, oldest (slot `position`) first, times the FP64 window, into */
+/* a complex128 block.  pgpe_spectrum_power: psd[(line segment + f) nx + x] += |B[f][line nx   */
+/* + x]|^2 for the transformed block B (width = lines x nx).                                  */
+int pgpe_spectrum_line(const void* field_dev, void* slot_dev, int planes, int nx, int ny, int row, int mode,
+                       double scale, int precision, void* stream);
+int pgpe_spectrum_block(const void* ring_dev, void* block_dev, const double* window_dev, int segment, int64_t width,
+                        int position, int precision, void* stream);
+int pgpe_spectrum_power(const void* block_k_dev, double* psd_dev, int segment, int64_t width, int nx, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Batched 2-D FFT (LDS Stockham, radix 2-16) and the fused split-step  */
@@ -225,6 +236,10 @@ int pgpe_fft_plan_destroy(pgpe_fft_plan* plan);
  * 1/(nx ny)), the numpy.fft.fft2 / ifft2 conventions.  in may equal out. */
 int pgpe_fft2(const pgpe_fft_plan* plan, const void* in_dev, void* out_dev, int planes, int inverse,
               void* stream);
+/* Forward transform along y only of planes x (ny, width) complex (unnormalised, any width;
+ * the plan's ny and precision are used, its nx is not): numpy.fft.fft(a, axis=-2). */
+int pgpe_fft_columns(const pgpe_fft_plan* plan, const void* in_dev, void* out_dev, int width, int planes,
+                     void* stream);
 
 typedef struct pgpe_split_args {
     int32_t precision;

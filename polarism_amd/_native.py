@@ -145,6 +145,7 @@ _SIGNATURES = {
     "pgpe_fft_plan_create": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(vp)]),
     "pgpe_fft_plan_destroy": (C.c_int, [vp]),
     "pgpe_fft2": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),
+    "pgpe_fft_columns": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),
     "pgpe_split_step": (C.c_int, [C.POINTER(SplitArgs), vp]),
     "pgpe_lawson": (C.c_int, [C.POINTER(LawsonArgs), vp]),
     "pgpe_gemm_bf16": (C.c_int, [C.POINTER(GemmArgs), vp]),
@@ -157,6 +158,9 @@ _SIGNATURES = {
     "pgpe_density_delta": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp]),
     "pgpe_accumulate_power": (C.c_int, [vp, vp, C.c_int64, C.c_int, vp]),
     "pgpe_accumulate_moments": (C.c_int, [vp, vp, vp, C.c_int64, C.c_int, vp]),
+    "pgpe_spectrum_line": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, vp]),
+    "pgpe_spectrum_block": (C.c_int, [vp, vp, vp, C.c_int, C.c_int64, C.c_int, C.c_int, vp]),
+    "pgpe_spectrum_power": (C.c_int, [vp, vp, C.c_int, C.c_int64, C.c_int, vp]),
 }
 
 _lib = None

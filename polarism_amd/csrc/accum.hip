@@ -1,8 +1,8 @@
 // In-run observables accumulated on the device in double precision
-// (results/accumulators.py:94-188): the per-mode power |psi_k|^2 of the momentum
-// distribution and the density moments rho, rho^2 of g2.  Elementwise and
-// HBM-bound: each launch reads the field (c bytes per point) and reads and writes
-// its FP64 sums (16 or 32 bytes per point); 4 points per thread, grid-stride.
+// (results/accumulators.py:94-270): the per-mode power |psi_k|^2 of the momentum
+// distribution, the density moments rho, rho^2 of g2, and the pieces of the time
+// spectra (line cuts into a ring, the windowed block in time order, the power of its
+// transform).  Elementwise and HBM-bound; grid-stride loops.
 #include "common.hpp"
 
 namespace pgpe {
@@ -30,6 +30,60 @@ __global__ void __launch_bounds__(256) accumulate_moments_kernel(const c2<T>* __
         const double rho = re * re + im * im;
         s1[i] += rho;
         s2[i] += rho * rho;
+    }
+}
+
+// Line cut of every plane into one ring slot: mode 0 copies row `row` of the field
+// (spectrum-x); mode 1 takes row ky = 0 of the field's 2-D transform, times `scale`, in
+// fftshift order along x (spectrum-k: FFT_x of the y-sum is that row, accumulators.py:206-215).
+template <typename T>
+__global__ void spectrum_line_kernel(const c2<T>* __restrict__ f, c2<T>* __restrict__ slot, int planes, int nx, int ny,
+                                     int row, int mode, T scale) {
+    const int64_t n = (int64_t)planes * nx;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int plane = (int)(i / nx), x = (int)(i - (int64_t)plane * nx);
+        if (mode == 0) {
+            slot[i] = f[((int64_t)plane * ny + row) * nx + x];
+        } else {
+            c2<T> v = f[(int64_t)plane * ny * nx + x];
+            v.x *= scale;
+            v.y *= scale;
+            slot[(int64_t)plane * nx + (x + nx / 2) % nx] = v;
+        }
+    }
+}
+
+// block[t][j] = ring[(position + t) mod segment][j] * window[t] in double (time order, oldest
+// first; the reference multiplies the complex ring by a float64 window, accumulators.py:217-219).
+template <typename T>
+__global__ void spectrum_block_kernel(const c2<T>* __restrict__ ring, double2* __restrict__ block,
+                                      const double* __restrict__ window, int segment, int64_t width, int position) {
+    const int64_t n = (int64_t)segment * width;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int t = (int)(i / width);
+        const int64_t j = i - (int64_t)t * width;
+        int src = position + t;
+        if (src >= segment) src -= segment;
+        const c2<T> v = ring[(int64_t)src * width + j];
+        const double w = window[t];
+        block[i] = make_double2((double)v.x * w, (double)v.y * w);
+    }
+}
+
+// psd[(line * segment + f) * nx + x] += |B[f][line * nx + x]|^2 (the power moved to the
+// (..., E, x) layout of accumulators.py:220-222).
+__global__ void spectrum_power_kernel(const double2* __restrict__ bk, double* __restrict__ psd, int segment,
+                                      int64_t width, int nx) {
+    const int64_t n = (int64_t)segment * width;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int f = (int)(i / width);
+        const int64_t j = i - (int64_t)f * width;
+        const int64_t line = j / nx, x = j - line * nx;
+        const double2 v = bk[i];
+        psd[(line * segment + f) * nx + x] += v.x * v.x + v.y * v.y;
     }
 }
 
@@ -71,5 +125,52 @@ extern "C" int pgpe_accumulate_moments(const void* field, double* sum_rho, doubl
     else
         return fail(PGPE_ERR_ARG, "pgpe_accumulate_moments: precision must be 0 or 1");
     PGPE_LAUNCHED("accumulate_moments_kernel");
+    return PGPE_OK;
+}
+
+extern "C" int pgpe_spectrum_line(const void* field, void* slot, int planes, int nx, int ny, int row, int mode,
+                                  double scale, int precision, void* stream) {
+    if (!field || !slot) return fail(PGPE_ERR_ARG, "pgpe_spectrum_line: null pointer");
+    if (planes <= 0 || nx <= 0 || ny <= 0 || (mode != 0 && mode != 1)) return fail(PGPE_ERR_ARG, "pgpe_spectrum_line: bad size or mode");
+    if (mode == 0 && (row < 0 || row >= ny)) return fail(PGPE_ERR_ARG, "pgpe_spectrum_line: row out of range");
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid = accum_grid((int64_t)planes * nx);
+    if (precision == 0)
+        hipLaunchKernelGGL(spectrum_line_kernel<float>, grid, dim3(256), 0, s, (const float2*)field, (float2*)slot,
+                           planes, nx, ny, row, mode, (float)scale);
+    else if (precision == 1)
+        hipLaunchKernelGGL(spectrum_line_kernel<double>, grid, dim3(256), 0, s, (const double2*)field, (double2*)slot,
+                           planes, nx, ny, row, mode, scale);
+    else
+        return fail(PGPE_ERR_ARG, "pgpe_spectrum_line: precision must be 0 or 1");
+    PGPE_LAUNCHED("spectrum_line_kernel");
+    return PGPE_OK;
+}
+
+extern "C" int pgpe_spectrum_block(const void* ring, void* block, const double* window, int segment, int64_t width,
+                                   int position, int precision, void* stream) {
+    if (!ring || !block || !window) return fail(PGPE_ERR_ARG, "pgpe_spectrum_block: null pointer");
+    if (segment <= 0 || width <= 0 || position < 0 || position >= segment)
+        return fail(PGPE_ERR_ARG, "pgpe_spectrum_block: bad size or position");
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid = accum_grid((int64_t)segment * width);
+    if (precision == 0)
+        hipLaunchKernelGGL(spectrum_block_kernel<float>, grid, dim3(256), 0, s, (const float2*)ring, (double2*)block,
+                           window, segment, width, position);
+    else if (precision == 1)
+        hipLaunchKernelGGL(spectrum_block_kernel<double>, grid, dim3(256), 0, s, (const double2*)ring,
+                           (double2*)block, window, segment, width, position);
+    else
+        return fail(PGPE_ERR_ARG, "pgpe_spectrum_block: precision must be 0 or 1");
+    PGPE_LAUNCHED("spectrum_block_kernel");
+    return PGPE_OK;
+}
+
+extern "C" int pgpe_spectrum_power(const void* block_k, double* psd, int segment, int64_t width, int nx, void* stream) {
+    if (!block_k || !psd) return fail(PGPE_ERR_ARG, "pgpe_spectrum_power: null pointer");
+    if (segment <= 0 || width <= 0 || nx <= 0 || width % nx) return fail(PGPE_ERR_ARG, "pgpe_spectrum_power: bad size");
+    hipLaunchKernelGGL(spectrum_power_kernel, accum_grid((int64_t)segment * width), dim3(256), 0, (hipStream_t)stream,
+                       (const double2*)block_k, psd, segment, width, nx);
+    PGPE_LAUNCHED("spectrum_power_kernel");
     return PGPE_OK;
 }

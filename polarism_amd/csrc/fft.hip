@@ -497,6 +497,29 @@ int pgpe_fft2(const pgpe_fft_plan* plan, const void* in, void* out, int planes, 
     return fft2_typed<double>(plan, in, out, planes, inverse, (hipStream_t)stream);
 }
 
+int pgpe_fft_columns(const pgpe_fft_plan* plan, const void* in, void* out, int width, int planes, void* stream) {
+    if (!plan || !in || !out) return fail(PGPE_ERR_ARG, "pgpe_fft_columns: null argument");
+    if (planes <= 0 || width <= 0) return fail(PGPE_ERR_ARG, "pgpe_fft_columns: width and planes must be positive");
+    pgpe_fft_plan q = *plan;  // the column kernels read only the y pass list and the width
+    q.nx = width;
+    if (plan->precision == 0) {
+        SplitParams<float> p{};
+        p.nx = width;
+        p.ny = plan->ny;
+        p.planes = planes;
+        p.in = (float2*)in;
+        p.out = (float2*)out;
+        return launch_cols<float, COL_FWD>(&q, p, (hipStream_t)stream);
+    }
+    SplitParams<double> p{};
+    p.nx = width;
+    p.ny = plan->ny;
+    p.planes = planes;
+    p.in = (double2*)in;
+    p.out = (double2*)out;
+    return launch_cols<double, COL_FWD>(&q, p, (hipStream_t)stream);
+}
+
 int pgpe_split_step(const pgpe_split_args* a, void* stream) {
     if (!a || !a->plan || !a->table || !a->psi || !a->tmp || !a->pump || !a->coef)
         return fail(PGPE_ERR_ARG, "pgpe_split_step: null argument");

@@ -82,3 +82,81 @@ def test_plane_wave_occupies_one_mode_with_unit_g2():
     assert np.sum(nk) == pytest.approx(total, rel=1e-10)
     np.testing.assert_allclose(ctl.accumulated["g2"].value, 1.0, rtol=1e-10)
     np.testing.assert_allclose(np.abs(ctl.accumulated["g1"].value), 1.0, atol=1e-8)
+
+
+def _host_spectrum(lines, segment, window):
+    """The reference's segment schedule (accumulators.py:216-222) on host lines (time, ..., nx)."""
+    psd, segments, since = 0.0, 0, 0
+    for i in range(1, len(lines) + 1):
+        since += 1
+        if i >= segment and (segments == 0 or since >= segment // 2):
+            block = np.asarray(lines[i - segment:i])
+            t = np.fft.fft(block * window.reshape((segment,) + (1,) * (block.ndim - 1)), axis=0)
+            psd = psd + np.moveaxis(np.abs(t) ** 2, 0, -2)
+            segments += 1
+            since = 0
+    return psd, segments
+
+
+@pytest.mark.parametrize("spin", ["scalar", "spinor"])
+def test_spectra_equal_the_reference_schedule_on_host_lines(spin):
+    """spectrum-k (ky = 0 row of the 2-D transform, fftshifted) and spectrum-x (the row nearest
+    y = 3) sampled every step with 16-sample segments: 41 samples, four segments; the device
+    power sums equal the host's on the same lines (FP64)."""
+    from polarism_amd import SimulationController
+    kw = dict(solver__total_time=0.2, solver__dt=5e-3, physics__spin=spin)
+    if spin == "spinor":
+        kw.update(physics__te_tm=0.05, physics__zeeman=0.02)
+    cfg = _cfg(**kw)
+    cfg.result.accumulators = {"spectrum-k": {"segment": 16, "every": 5e-3},
+                               "spectrum-x": {"segment": 16, "every": 5e-3, "y": 3.0}}
+    ctl = SimulationController(cfg)
+    ctl.run()
+    acc = ctl.accumulated
+    ref = SimulationController(_cfg(**kw))
+    g = ref.grid
+    row = int(np.argmin(np.abs(g.Y[:, 0] - 3.0)))
+    shape = ref.state._public_shape
+    kl, xl = [], []
+    for i in range(41):
+        if i:
+            ref.solver.advance(ref.model, ref.state, 1)
+        f = ref.state.field.get().reshape(shape).astype(np.complex128)
+        k = np.fft.fft(f.sum(axis=-2), axis=-1) * (g.dx * g.dy / (g.nx * g.ny)) ** 0.5
+        kl.append(np.fft.fftshift(k, axes=-1))
+        xl.append(f[..., row, :])
+    window = 0.5 - 0.5 * np.cos(2 * np.pi * np.arange(16) / 16)
+    freqs = np.fft.fftfreq(16, d=5e-3)
+    order = np.argsort(-freqs)
+    dE = 2 * np.pi * cfg.physics.hbar / (16 * 5e-3)
+    for name, lines in (("spectrum_k", kl), ("spectrum_x", xl)):
+        psd, segments = _host_spectrum(lines, 16, window)
+        assert segments == 4
+        want = np.take(psd, order, axis=-2) / (segments * 16 * np.sum(window ** 2) * dE)
+        got = acc[name].value
+        np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12 * np.max(np.abs(want)))
+    np.testing.assert_allclose(acc["spectrum_k_energy"].value, -2 * np.pi * cfg.physics.hbar * freqs[order])
+
+
+def test_uniform_condensate_line_sits_at_its_interaction_energy():
+    """A uniform field with interaction g and no gain, loss, reservoir or pump rotates as
+    exp(-i g a^2 t / hbar) (the split step is exact for it): spectrum-k has its whole weight in
+    the kx = 0 column, peaked at the energy bin E = g a^2."""
+    from polarism_amd import SimulationController
+    segment, dt = 32, 5e-3
+    cfg = quickstart_config(solver__method="split-step-fft-hip", grid__nx=16, grid__ny=16, grid__lx=16.0,
+                            grid__ly=16.0, solver__dt=dt, solver__total_time=segment * dt, laser__P0=0.0,
+                            reservoir__reservoir_type="none", physics__gamma_C=0.0)
+    a = 0.5
+    dE = 2 * np.pi * cfg.physics.hbar / (segment * dt)
+    cfg.physics.g_C = 5 * dE / (a * a)        # E0 on the fifth bin
+    cfg.noise.enabled = False
+    cfg.result.accumulators = {"spectrum-k": {"segment": segment, "every": dt}}
+    ctl = SimulationController(cfg)
+    ctl.state.psi = np.full((16, 16), a, dtype=complex)
+    ctl.run()
+    spec = ctl.accumulated["spectrum_k"].value
+    energy = ctl.accumulated["spectrum_k_energy"].value
+    col = spec.shape[-1] // 2
+    assert np.sum(spec[:, col]) > 0.999 * np.sum(spec)
+    assert energy[np.argmax(spec[:, col])] == pytest.approx(cfg.physics.g_C * a * a)
