@@ -453,8 +453,7 @@ def main():
             if have_weights:
                 params, cfg, norm = ws.load_weights(args.weights)
                 t0 = time.perf_counter()
-                y = ws.predict(params, ws.features(members), cfg)
-                wm = ws.warm_start(members, y, norm)
+                wm = ws.warm_start(members, None, norm, states=ws.predict_states(params, members, cfg, norm))
                 infer_s.append(time.perf_counter() - t0)
                 warm_runs.append(run_sweep(wm, graphs=not args.no_graphs))
         wall = time.perf_counter() - t_all

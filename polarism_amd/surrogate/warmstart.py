@@ -203,24 +203,77 @@ def predict(params, x, cfg=None, batch=16):
         return torch.cat(outs, dim=1).cpu().numpy()
 
 
-def warm_start(members, y_pred, norm: Normaliser | None = None):
-    """Members whose initial state is the predicted (psi, n_R).
+def predict_states(params, members, cfg=None, norm: Normaliser | None = None, batch=16):
+    """The warm-start pipeline on the device: features (as ``features``) built from the
+    envelopes on the GPU, the FNO forward, and the denormalised fields with each member's peak
+    density (as ``Normaliser.fields``, single precision).  Returns host arrays
+    (psi (B, H, W) complex64, n (B, H, W) float32, peak (B,) float32) for ``warm_start``."""
+    import torch
+    from .fno import FNOConfig, FNOModule
+    norm = norm or default_normaliser()
+    H, W = members[0].ny, members[0].nx
+    cfg = cfg or FNOConfig(H=H, W=W)
+    ratio = np.asarray([m.P0 / threshold(m.physics, "single") for m in members])
+    gam = np.asarray([m.physics.gamma_C / 0.083 for m in members], dtype=np.float32)
+    gc = np.asarray([m.physics.g_C / 0.001 for m in members], dtype=np.float32)
+    env = np.stack([np.asarray(m.envelope) for m in members])
+    mod = FNOModule(cfg, device="cuda")
+    with torch.no_grad():
+        for k, v in params.items():
+            mod.params[k].copy_(torch.tensor(v))
+        dev = torch.device("cuda")
+        E = torch.tensor(env, device=dev)
+        R = torch.tensor(ratio, device=dev)
+        B = len(members)
+        X = torch.empty((IN_FEATURES, B, H, W), dtype=torch.float32, device=dev)
+        X[0] = (E * R[:, None, None]).to(torch.float32)
+        X[1] = R.to(torch.float32)[:, None, None]
+        X[2] = torch.tensor(gam, device=dev)[:, None, None]
+        X[3] = torch.tensor(gc, device=dev)[:, None, None]
+        outs = []
+        for s0 in range(0, B, batch):
+            outs.append(mod.net.forward(X[:, s0:s0 + batch].contiguous(), {k: mod.params[k] for k in mod.names}))
+        Y = torch.cat(outs, dim=1)
+        re = Y[0] * np.float32(norm.psi_scale)
+        im = Y[1] * np.float32(norm.psi_scale)
+        n = torch.clamp(Y[2] * np.float32(norm.n_scale), min=0.0)
+        peak = (re * re + im * im).reshape(B, -1).amax(dim=1)
+        psi = torch.complex(re, im)
+        torch.cuda.synchronize()
+        return psi.cpu().numpy(), n.cpu().numpy(), peak.cpu().numpy()
+
+
+def warm_start(members, y_pred, norm: Normaliser | None = None, states=None):
+    """Members whose initial state is the predicted (psi, n_R), from the network output
+    ``y_pred`` or from the ``states`` of ``predict_states``.
 
     The cold start's noise field (RMS 1e-3) is not added: the prediction's own error is larger,
     so it seeds any symmetry breaking the dynamics needs.  A member predicted to hold no
     condensate (peak density below ``ZERO_STATE_PEAK``) starts from psi = 0 exactly."""
     norm = norm or default_normaliser()
+    B = len(members)
+    if states is not None:
+        psi_all, n_all, peak = states
+    else:
+        # all members at once, with the arithmetic of Normaliser.fields (single precision)
+        re = y_pred[0, :B] * np.float32(norm.psi_scale)
+        im = y_pred[1, :B] * np.float32(norm.psi_scale)
+        n_all = np.maximum(y_pred[2, :B] * np.float32(norm.n_scale), np.float32(0.0))
+        peak = (re * re + im * im).reshape(B, -1).max(axis=1)
+        psi_all = np.empty(re.shape, np.complex64)
+        psi_all.real = re
+        psi_all.imag = im
     out = []
     for b, m in enumerate(members):
-        psi, n = norm.fields(y_pred, b)
         label = dict(m.label)
-        if float(np.max(psi.real ** 2 + psi.imag ** 2)) < ZERO_STATE_PEAK:
+        if float(peak[b]) < ZERO_STATE_PEAK:
             psi = np.zeros((m.ny, m.nx), np.complex64)
             label["warm"] = "zero"
         else:
+            psi = psi_all[b]
             label["warm"] = "predicted"
         out.append(SweepMember(m.nx, m.ny, m.lx, m.ly, m.physics, m.envelope, m.P0, seed=m.seed,
-                               psi0=psi, res0=(n,), label=label))
+                               psi0=psi, res0=(n_all[b],), label=label))
     return out
 
 

@@ -44,3 +44,17 @@ def test_warm_start_reaches_the_cold_steady_state_sooner():
     print("density rel-L2 warm vs cold:", np.round(agree, 6))
     assert both.sum() >= max(4, int(0.8 * np.isfinite(cold.stop_times).sum()))
     assert np.median(agree) <= 1e-3
+
+
+def test_device_pipeline_equals_host_pipeline():
+    """predict_states (features, forward and denormalisation on the GPU) gives the warm starts
+    of predict + warm_start on the host: the same zero-state choices and fields to rounding."""
+    from polarism_amd.surrogate import warmstart as ws
+    members = ws.make_members(16, seed=99)
+    params, cfg, norm = ws.load_weights(WEIGHTS)
+    host = ws.warm_start(members, ws.predict(params, ws.features(members), cfg), norm)
+    dev = ws.warm_start(members, None, norm, states=ws.predict_states(params, members, cfg, norm))
+    for a, b in zip(host, dev):
+        assert a.label["warm"] == b.label["warm"]
+        np.testing.assert_allclose(b.psi0, a.psi0, rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(b.res0[0], a.res0[0], rtol=1e-5, atol=1e-6)
