@@ -4,16 +4,27 @@
 noise, state and model, creates ``cfg.solver.method`` through the registry and
 runs ``advance`` in chunks.  Supported run-loop features: total time,
 steady-state stop per member (simulation_controller.py:710-734, evaluated on
-the device by ``pgpe_density_delta``), and SIGTERM/SIGUSR1 stop requests that
-end the run after the current chunk (:1045-1051).  Storage, visualisation and
-checkpoints are outside this package.
+the device by ``pgpe_density_delta``), in-run accumulators, SIGTERM/SIGUSR1 stop
+requests that end the run after the current chunk (:1045-1051), and checkpoints:
+periodic (``checkpoint.every``), at a stop request and at the end of a
+checkpointed run, and ``checkpoint.resume_from`` (:736-923).  A checkpoint holds
+the version-2 content of the reference's (field, field_k, reservoir fields, time
+and step, configuration, stop times, steady-state reference, adaptive controller,
+accumulator sums) in a NumPy ``.npz`` container (h5py is not a dependency here),
+written atomically; resuming continues the run bit for bit.  Storage and
+visualisation are outside this package.
 """
 from __future__ import annotations
 
 import copy
+import json
 import math
+import os
 import signal
+import tempfile
 import threading
+from dataclasses import asdict
+from pathlib import Path
 
 import numpy as np
 
@@ -49,6 +60,14 @@ def apply_overrides(cfg: Config, overrides: dict) -> Config:
     return out
 
 
+def _read_checkpoint(path):
+    """(attrs, arrays) of a checkpoint written by save_checkpoint."""
+    with np.load(path, allow_pickle=False) as z:
+        arrays = {k: z[k] for k in z.files if k != "attrs"}
+        attrs = json.loads(str(z["attrs"]))
+    return attrs, arrays
+
+
 def _next_sample(step, start, every, stop):
     """First sampling step after ``step`` (simulation_controller.py:1024-1026)."""
     due = start if step < start else start + ((step - start) // every + 1) * every
@@ -68,8 +87,12 @@ class SimulationController:
         if members is not None and len(members) == 0:
             raise ValueError("members must contain at least one override mapping")
         cfg = copy.deepcopy(cfg)
+        self._seed_drawn = cfg.noise.seed is None and cfg.checkpoint.resume_from is None
         if cfg.noise.seed is None:
-            cfg.noise.seed = draw_seed()
+            if cfg.checkpoint.resume_from is not None:
+                cfg.noise.seed = int(_read_checkpoint(cfg.checkpoint.resume_from)[0]["config"]["noise"]["seed"])
+            else:
+                cfg.noise.seed = draw_seed()
         member_cfgs = [apply_overrides(cfg, o) for o in members] if members is not None else None
         for m in member_cfgs or []:
             if m.noise.seed is None:
@@ -138,6 +161,9 @@ class SimulationController:
         self._steady_last_step = -1
         self._stop_requested = False
         self.steady_checks = 0
+        self._resumed = False
+        if cfg.checkpoint.resume_from is not None:
+            self.load_checkpoint(cfg.checkpoint.resume_from)
 
     # -- steady state ------------------------------------------------------------
     def _steady_buffers(self):
@@ -187,6 +213,12 @@ class SimulationController:
     def run(self) -> None:
         dt = self.cfg.solver.dt
         n_steps = round(self.cfg.solver.total_time / dt)
+        if self.state.step > n_steps:
+            raise ValueError("checkpoint step exceeds configured total_time")
+        ck = self.cfg.checkpoint
+        interval = _steps(ck.every, dt, "checkpoint.every") if ck.every else 0
+        checkpointing = interval > 0 or ck.path is not None or ck.resume_from is not None
+        last_checkpoint = -1
         steady = _steps(self.cfg.stop.check_every, dt, "stop.check_every") if self.cfg.stop.steady_state else 0
         previous = {}
 
@@ -200,7 +232,7 @@ class SimulationController:
         schedules = self._accumulator_schedules(dt, n_steps)
         try:
             step = self.state.step
-            if step == 0:
+            if step == 0 and not self._resumed:
                 for acc, start, _, stop in schedules:
                     if start == 0 and stop >= 0 and acc.samples == 0:
                         acc.update(self.state, self.state.t)
@@ -210,6 +242,8 @@ class SimulationController:
                     end = min(end, (step // steady + 1) * steady)
                 if previous:
                     end = min(end, step + 1000)
+                if interval:
+                    end = min(end, (step // interval + 1) * interval)
                 for _, start, every, stop in schedules:
                     due = _next_sample(step, start, every, stop)
                     if due is not None:
@@ -221,12 +255,160 @@ class SimulationController:
                         acc.update(self.state, self.state.t)
                 if steady and step % steady == 0 and self._steady_last_step != step:
                     self._check_steady_state()
+                if interval and step % interval == 0:
+                    self.save_checkpoint(self._checkpoint_path())
+                    last_checkpoint = step
+            if (checkpointing or self._stop_requested) and last_checkpoint != self.state.step:
+                self.save_checkpoint(self._checkpoint_path())
         finally:
             for signum, handler in previous.items():
                 signal.signal(signum, handler)
             if self._accumulators:
                 self.accumulated = {result.name: result for acc in self._accumulators.values() if acc.samples
                                     for result in acc.results()}
+
+    # -- checkpoints ---------------------------------------------------------------
+    def _checkpoint_path(self) -> Path:
+        """checkpoint.path, else result.output_directory/checkpoint.npz."""
+        if self.cfg.checkpoint.path is not None:
+            return Path(self.cfg.checkpoint.path)
+        return Path(self.cfg.result.output_directory) / "checkpoint.npz"
+
+    def _member_json(self):
+        if self._member_cfgs is None:
+            return None
+        return json.dumps([{name: asdict(m)[name] for name in ("physics", "laser", "potential", "drive", "noise")}
+                           for m in self._member_cfgs], sort_keys=True, default=str)
+
+    def save_checkpoint(self, path) -> None:
+        """Atomically write the run state (simulation_controller.py:736-798, npz container)."""
+        target = Path(path)
+        target.parent.mkdir(parents=True, exist_ok=True)
+        st = self.state
+        arrays = {"field": st.field.get()}
+        if st.field_k is not None:
+            arrays["field_k"] = st.field_k.get()
+        for name, value in zip(self.reservoir.state_names, self.reservoir.get_state()):
+            arrays[f"reservoir/{name}"] = value.get()
+        attrs = dict(checkpoint_version=2, step=st.step, t=st.t, t0=st.t0, dt=self.cfg.solver.dt, ndim=2,
+                     shape=list(self.grid.shape), components=st.components, batched=st.batched,
+                     reservoir_type=self.cfg.reservoir.reservoir_type,
+                     config=json.loads(json.dumps(asdict(self.cfg), default=str)),
+                     stop_time=[None if not np.isfinite(v) else float(v) for v in self.stop_times],
+                     steady_last_step=self._steady_last_step, member_configs_json=self._member_json(),
+                     accumulators={})
+        if self.cfg.solver.adaptive:
+            ctl = getattr(self.solver, "step_controller", None)
+            attrs["step_controller"] = ctl.state_dict() if ctl is not None else None
+        if self._steady_norm is not None:
+            attrs["steady_norm"] = [float(v) for v in self._steady_norm]
+            arrays["steady_density"] = self._steady_old.get().reshape(st.batch_size, *self.grid.shape)
+        for name, acc in self._accumulators.items():
+            values = acc.state_dict()
+            attrs["accumulators"][name] = {k: values[k] for k in ("samples", "t_first", "t_last")}
+            for k, v in values.items():
+                if k not in ("samples", "t_first", "t_last"):
+                    arrays[f"accumulators/{name}/{k}"] = np.asarray(v)
+        arrays["attrs"] = np.asarray(json.dumps(attrs))
+        fd, tmp = tempfile.mkstemp(prefix=f".{target.name}.", suffix=".tmp", dir=target.parent)
+        try:
+            with os.fdopen(fd, "wb") as fh:
+                np.savez(fh, **arrays)
+            os.replace(tmp, target)
+        finally:
+            if os.path.exists(tmp):
+                os.unlink(tmp)
+
+    def load_checkpoint(self, path) -> None:
+        """Validate and restore a checkpoint for this configuration (simulation_controller.py:800-923)."""
+        attrs, arrays = _read_checkpoint(path)
+        cfg = self.cfg
+        if attrs.get("checkpoint_version") != 2:
+            raise ValueError(f"checkpoint format v{attrs.get('checkpoint_version')} is not supported; supported: 2")
+        if attrs["reservoir_type"] != cfg.reservoir.reservoir_type:
+            raise ValueError("reservoir type mismatch")
+        if attrs["dt"] != cfg.solver.dt:
+            raise ValueError("checkpoint dt mismatch")
+        saved = attrs["config"]
+        if bool(saved["solver"].get("adaptive", False)) != cfg.solver.adaptive or \
+                ("step_controller" in attrs) != cfg.solver.adaptive:
+            raise ValueError("checkpoint adaptive mode mismatch")
+        if saved.get("stop") != json.loads(json.dumps(asdict(cfg.stop), default=str)):
+            raise ValueError("checkpoint stop config mismatch")
+        if saved.get("result", {}).get("accumulators", {}) != json.loads(json.dumps(cfg.result.accumulators)):
+            raise ValueError("checkpoint accumulator config mismatch")
+        if set(attrs.get("accumulators", {})) != set(self._accumulators):
+            raise ValueError("checkpoint accumulators mismatch")
+        if self._seed_drawn:
+            drawn = cfg.noise.seed
+            cfg.noise.seed = saved["noise"]["seed"]
+            for m in self._member_cfgs or []:
+                if m.noise.seed == drawn:
+                    m.noise.seed = saved["noise"]["seed"]
+            self._seed_drawn = False
+        if self._member_cfgs is None and any(saved["noise"][k] != getattr(cfg.noise, k)
+                                             for k in ("enabled", "seed", "stochastic", "stochastic_strength")):
+            raise ValueError("checkpoint noise config mismatch")
+        if self._member_cfgs is not None and attrs.get("member_configs_json") != self._member_json():
+            raise ValueError("checkpoint member physics, pump, potential or drive mismatch")
+        field = arrays["field"]
+        st = self.state
+        if int(attrs["ndim"]) != 2 or tuple(attrs["shape"]) != tuple(self.grid.shape):
+            raise ValueError("checkpoint grid shape mismatch")
+        if int(attrs["components"]) != st.components or field.shape[1] != st.components:
+            raise ValueError("checkpoint components mismatch")
+        if field.shape != st._canonical_shape or bool(attrs["batched"]) != st.batched:
+            raise ValueError("checkpoint batch shape mismatch")
+        names = self.reservoir.state_names
+        res = [arrays.get(f"reservoir/{n}") for n in names]
+        if any(r is None or r.shape != self.reservoir.shape for r in res) or \
+                {k for k in arrays if k.startswith("reservoir/")} != {f"reservoir/{n}" for n in names}:
+            raise ValueError("checkpoint reservoir names or shapes mismatch")
+        step, t, t0 = int(attrs["step"]), float(attrs["t"]), float(attrs["t0"])
+        if step < 0 or not np.isfinite(t) or not np.isfinite(t0) or t != t0 + step * cfg.solver.dt:
+            raise ValueError("checkpoint time and step mismatch")
+        stop_times = np.asarray([np.nan if v is None else v for v in attrs["stop_time"]], dtype=float)
+        if stop_times.shape != (st.batch_size,):
+            raise ValueError("checkpoint stop_time shape mismatch")
+        steady_last = int(attrs["steady_last_step"])
+        norm = attrs.get("steady_norm")
+        density = arrays.get("steady_density")
+        if (norm is None) != (density is None):
+            raise ValueError("checkpoint steady-state reference is incomplete")
+        if steady_last > step or steady_last < -1:
+            raise ValueError("checkpoint steady-state check step mismatch")
+        field_k = arrays.get("field_k")
+        if field_k is not None and field_k.shape != field.shape:
+            raise ValueError("checkpoint spectral batch shape mismatch")
+        saved_precision = saved["solver"]["precision"]
+        cdt = np.complex64 if cfg.solver.precision == "single" else np.complex128
+        if saved_precision != cfg.solver.precision:
+            import warnings
+            warnings.warn("checkpoint precision differs; casting state", UserWarning, stacklevel=2)
+        st.field = DeviceArray.from_numpy(field.astype(cdt))
+        if field_k is not None:
+            st.field_k = DeviceArray.from_numpy(field_k.astype(cdt))
+        st.t, st.t0, st.step = t, t0, step
+        self.reservoir.set_state(tuple(r for r in res))
+        self.stop_times = stop_times
+        self._steady_last_step = steady_last
+        if norm is not None:
+            self._steady_norm = np.asarray(norm, dtype=float)
+            r = np.float32 if cfg.solver.precision == "single" else np.float64
+            self._steady_buffers().upload(np.asarray(density, dtype=r).reshape(-1))
+        else:
+            self._steady_norm = None
+        for name, acc in self._accumulators.items():
+            values = dict(attrs["accumulators"][name])
+            prefix = f"accumulators/{name}/"
+            values.update({k[len(prefix):]: v for k, v in arrays.items() if k.startswith(prefix)})
+            acc.load_state_dict(values)
+        self._resumed = True
+        self.solver = create_solver(cfg, self.grid, self.model)
+        if cfg.solver.adaptive and attrs.get("step_controller") is not None:
+            from .step_controller import StepController
+            self.solver.step_controller = StepController(self.solver)
+            self.solver.step_controller.load_state_dict(attrs["step_controller"])
 
     def _accumulator_schedules(self, dt, n_steps):
         """(accumulator, start, every, stop) in steps (simulation_controller.py:1015-1022)."""

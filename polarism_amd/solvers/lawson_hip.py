@@ -99,14 +99,20 @@ class IFRKFFTHipSolver(HipSolverBase):
         if f.ptr != b["psi"].ptr:
             b["psi"].copy_from(f.reshape(-1), self.stream)
             self._k_valid = False
+            fk = state.field_k
+            if fk is not None:  # a restored spectral cache: continue from it, not from FFT(psi)
+                b["psik"].copy_from(fk.reshape(-1), self.stream)
+                self._k_valid = True
         for i, r in enumerate(model.reservoir.get_state()):
             if r.ptr != b["res"][i].ptr:
                 b["res"][i].copy_from(r.reshape(-1), self.stream)
 
     def _publish(self, model, state):
+        """The real-space field, and its spectral cache when the solver holds psi^k of the
+        same state (state.field_k, simulation_state.py:92-192; a checkpoint saves it)."""
         b = self._bufs
         state._field = b["psi"].reshape(state._canonical_shape)
-        state._field_k = None
+        state._field_k = b["psik"].reshape(state._canonical_shape) if self._k_valid else None
         model.reservoir.assign_device(tuple(x.reshape(model.reservoir.shape) for x in b["res"]))
 
     # -- one trial ------------------------------------------------------------
