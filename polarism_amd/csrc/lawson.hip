@@ -99,27 +99,37 @@ __device__ __forceinline__ void apply_linear(const pgpe_member& mb, int comps, d
     spinor_propagate<T>(mb, kxv, kyv, tau, v, out);
 }
 
-// acc += s L(tau) v.  Scalar: (s * phase) * v as ifrk_fft_cuda_solver.py:173;
-// spinor: s * (M v) as :171.
+// The propagator of one stage term, evaluated once per k point and shared by every
+// combination using the term: the scalar phase (cos, sin) of the angle above, or the
+// spinor's M v.
 template <typename T>
-__device__ __forceinline__ void accumulate_term(const pgpe_member& mb, int comps, double kxv, double kyv, double tau,
-                                                T s, const c2<T>* v, c2<T>* acc) {
+__device__ __forceinline__ void term_propagator(const pgpe_member& mb, int comps, double kxv, double kyv,
+                                                double tau, const c2<T>* v, c2<T>* prop) {
     if (comps == 1) {
         const T rate = (T)(-mb.hbar / (2.0 * mb.m_eff));
         const T k2 = (T)(kxv * kxv + kyv * kyv);
         const T angle = (rate * (T)tau) * k2;
         T sn, cs;
         sin_cos(angle, &sn, &cs);
-        const c2<T> t = cmul<T>(make_c<T>(s * cs, s * sn), v[0]);
+        prop[0] = make_c<T>(cs, sn);
+        return;
+    }
+    spinor_propagate<T>(mb, kxv, kyv, tau, v, prop);
+}
+
+// acc += s L(tau) v from the term's propagator.  Scalar: (s * phase) * v as
+// ifrk_fft_cuda_solver.py:173; spinor: s * (M v) as :171.
+template <typename T>
+__device__ __forceinline__ void add_term(int comps, T s, const c2<T>* v, const c2<T>* prop, c2<T>* acc) {
+    if (comps == 1) {
+        const c2<T> t = cmul<T>(make_c<T>(s * prop[0].x, s * prop[0].y), v[0]);
         acc[0].x = acc[0].x + t.x;
         acc[0].y = acc[0].y + t.y;
         return;
     }
-    c2<T> t[2];
-    apply_linear<T>(mb, comps, kxv, kyv, tau, v, t);
     for (int c = 0; c < 2; ++c) {
-        acc[c].x = acc[c].x + s * t[c].x;
-        acc[c].y = acc[c].y + s * t[c].y;
+        acc[c].x = acc[c].x + s * prop[c].x;
+        acc[c].y = acc[c].y + s * prop[c].y;
     }
 }
 
@@ -151,9 +161,12 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
     const int cols_per = E / (n * seqs_per_col);
     c2<T>* s_tw = buf + cols_per * seqs_per_col * ld;  // twiddles staged once per group
     const int groups = (p.nx + cols_per - 1) / cols_per;
-    const int m = blockIdx.x / groups;
-    if (m >= p.members) return;
-    const int x0 = (blockIdx.x - m * groups) * cols_per;
+    // XCD-aware: the groups one XCD runs side by side are neighbours along x, so the
+    // narrow row segments they read share cache lines through the same L2
+    const int gi = xcd_tile(blockIdx.x, groups * p.members);
+    if (gi < 0) return;
+    const int m = gi / groups;
+    const int x0 = (gi - m * groups) * cols_per;
     const int cnt = min(cols_per, p.nx - x0);
     const int64_t plane = (int64_t)p.nx * n;
     const int64_t planes_total = (int64_t)p.members * p.comps;
@@ -183,14 +196,33 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
             const int64_t g = (int64_t)y * p.nx + x0 + i;
             const double kxv = p.kx[x0 + i], kyv = p.ky[y];
             c2<T> v[2], acc[2];
-            for (int c = 0; c < p.comps; ++c) v[c] = p.psi0k[(m * p.comps + c) * plane + g];
+            for (int c = 0; c < 2; ++c) if (c < p.comps) v[c] = p.psi0k[(m * p.comps + c) * plane + g];
             apply_linear<T>(mb, p.comps, kxv, kyv, p.frac0 * p.h, v, acc);
-            for (int j = 0; j < p.nterms; ++j) {
-                if (p.hc[j] == 0.0) continue;
-                for (int c = 0; c < p.comps; ++c) v[c] = p.f[j][(m * p.comps + c) * plane + g];
-                accumulate_term<T>(mb, p.comps, kxv, kyv, p.fr[j] * p.h, (T)p.hc[j], v, acc);
+            // error combinations of the final step (DP5: one, DOP853: two) share each term's
+            // propagator with the solution's combination
+            constexpr int QE = MODE == LC_FINAL ? 2 : 0;
+            c2<T> er[QE > 0 ? QE : 1][2];
+            for (int q = 0; q < QE; ++q) er[q][0] = er[q][1] = make_c<T>(0, 0);
+            // the stage terms of the k point, gathered first so that their loads overlap
+            c2<T> vt[kMaxStages][2];
+            bool used[kMaxStages];
+#pragma unroll
+            for (int j = 0; j < kMaxStages; ++j) {
+                used[j] = j < p.nterms && p.hc[j] != 0.0;
+                for (int q = 0; q < QE; ++q) used[j] |= j < p.nterms && q < p.nerr && p.he[q][j] != 0.0;
+                if (used[j])
+                    for (int c = 0; c < 2; ++c) if (c < p.comps) vt[j][c] = p.f[j][(m * p.comps + c) * plane + g];
             }
-            for (int c = 0; c < p.comps; ++c) {
+#pragma unroll
+            for (int j = 0; j < kMaxStages; ++j) {
+                if (!used[j]) continue;
+                c2<T> prop[2];
+                term_propagator<T>(mb, p.comps, kxv, kyv, p.fr[j] * p.h, vt[j], prop);
+                if (p.hc[j] != 0.0) add_term<T>(p.comps, (T)p.hc[j], vt[j], prop, acc);
+                for (int q = 0; q < QE; ++q)
+                    if (q < p.nerr && p.he[q][j] != 0.0) add_term<T>(p.comps, (T)p.he[q][j], vt[j], prop, er[q]);
+            }
+            for (int c = 0; c < 2; ++c) if (c < p.comps) {
                 buf[seq(0, c, i) + pad_at(y)] = acc[c];
                 if (MODE == LC_FINAL && p.psik_out) p.psik_out[(m * p.comps + c) * plane + g] = acc[c];
                 if (MODE == LC_COMBINE && p.lap) {
@@ -198,16 +230,10 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
                     buf[seq(1, c, i) + pad_at(y)] = make_c<T>(mk2 * acc[c].x, mk2 * acc[c].y);
                 }
             }
-            if (MODE == LC_FINAL) {
-                for (int q = 0; q < p.nerr; ++q) {
-                    c2<T> er[2] = {make_c<T>(0, 0), make_c<T>(0, 0)};
-                    for (int j = 0; j < p.nterms; ++j) {
-                        if (p.he[q][j] == 0.0) continue;
-                        for (int c = 0; c < p.comps; ++c) v[c] = p.f[j][(m * p.comps + c) * plane + g];
-                        accumulate_term<T>(mb, p.comps, kxv, kyv, p.fr[j] * p.h, (T)p.he[q][j], v, er);
-                    }
-                    for (int c = 0; c < p.comps; ++c) buf[seq(1 + q, c, i) + pad_at(y)] = er[c];
-                }
+            if constexpr (MODE == LC_FINAL) {
+                for (int q = 0; q < QE; ++q)
+                    if (q < p.nerr)
+                        for (int c = 0; c < 2; ++c) if (c < p.comps) buf[seq(1 + q, c, i) + pad_at(y)] = er[q][c];
             }
         }
         __syncthreads();
@@ -289,7 +315,7 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
         const int64_t cell = (int64_t)y0 * n + e;
         c2<T> psi[2], lap[2] = {make_c<T>(0, 0), make_c<T>(0, 0)};
         T rho[2] = {0, 0};
-        for (int c = 0; c < C; ++c) {
+        for (int c = 0; c < 2; ++c) if (c < C) {
             c2<T> v = buf[seq(0, c, 0) + slot(e)];
             if (MODE != LR_RHS_REAL) {
                 v.x *= p.scale;
@@ -302,41 +328,47 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
                 lap[c] = make_c<T>(l.x * p.scale, l.y * p.scale);
             }
         }
-        // reservoir values of this stage (or final) for each reservoir component
+        // reservoir values of this stage (or final) for each reservoir component; the stage
+        // derivatives of the cell are gathered first so that their loads overlap
         T nres[2][2] = {{0, 0}, {0, 0}};
-        for (int rcix = 0; rcix < nres_c; ++rcix) {
+        for (int rcix = 0; rcix < 2; ++rcix) if (rcix < nres_c) {
             for (int f = 0; f < NF; ++f) {
                 const int64_t g = (int64_t)(m * nres_c + rcix) * plane + cell;
-                T v = p.res0[f][g];
-                for (int j = 0; j < p.nr; ++j)
-                    if (p.rc[j] != 0.0) v += (T)p.rc[j] * p.dres[j][f][g];
+                T dv[kMaxStages];
+#pragma unroll
+                for (int j = 0; j < kMaxStages; ++j) {
+                    bool used = j < p.nr && p.rc[j] != 0.0;
+                    if constexpr (MODE == LR_FINAL)
+                        for (int q = 0; q < 2; ++q) used |= j < p.nr && q < p.nerr && p.he[q][j] != 0.0;
+                    dv[j] = used ? p.dres[j][f][g] : (T)0;
+                }
+                const T r0 = p.res0[f][g];
+                T v = r0;
+#pragma unroll
+                for (int j = 0; j < kMaxStages; ++j)
+                    if (j < p.nr && p.rc[j] != 0.0) v += (T)p.rc[j] * dv[j];
+                if constexpr (MODE == LR_FINAL) {
+                    // clamp, store the new state; reservoir error sum_j (h w_j) dn_j against
+                    // atol + rtol max(|n0|, |n1|)
+                    v = v > (T)0 ? v : (T)0;
+                    p.res_out[f][g] = v;
+                    for (int q = 0; q < p.nerr; ++q) {
+                        T ev = 0;
+#pragma unroll
+                        for (int j = 0; j < kMaxStages; ++j)
+                            if (j < p.nr && p.he[q][j] != 0.0) ev += (T)p.he[q][j] * dv[j];
+                        const double sc = p.atol + p.rtol * (double)fmax(fabs(r0), fabs(v));
+                        const double mag = fabs((double)ev);
+                        const double s = sc > 0 ? mag / sc : (mag == 0 ? 0.0 : INFINITY);
+                        if (q == 0) acc5 += s * s; else acc3 += s * s;
+                    }
+                }
                 nres[rcix][f] = v;
             }
         }
         if constexpr (MODE == LR_FINAL) {
-            // clamp, store the new state, error terms, noise
-            for (int rcix = 0; rcix < nres_c; ++rcix)
-                for (int f = 0; f < NF; ++f) {
-                    const int64_t g = (int64_t)(m * nres_c + rcix) * plane + cell;
-                    T v = nres[rcix][f];
-                    v = v > (T)0 ? v : (T)0;
-                    nres[rcix][f] = v;
-                    p.res_out[f][g] = v;
-                    if (p.nerr) {
-                        // reservoir error sum_j (h w_j) dn_j against atol + rtol max(|n0|, |n1|)
-                        for (int q = 0; q < p.nerr; ++q) {
-                            T ev = 0;
-                            for (int j = 0; j < p.nr; ++j)
-                                if (p.he[q][j] != 0.0) ev += (T)p.he[q][j] * p.dres[j][f][g];
-                            const T r0 = p.res0[f][g];
-                            const double sc = p.atol + p.rtol * (double)fmax(fabs(r0), fabs(v));
-                            const double mag = fabs((double)ev);
-                            const double s = sc > 0 ? mag / sc : (mag == 0 ? 0.0 : INFINITY);
-                            if (q == 0) acc5 += s * s; else acc3 += s * s;
-                        }
-                    }
-                }
-            for (int c = 0; c < C; ++c) {
+            // new field, its error terms, noise
+            for (int c = 0; c < 2; ++c) if (c < C) {
                 c2<T> v = psi[c];
                 const int64_t g = (m * C + c) * plane + cell;
                 if (p.nerr) {
@@ -351,7 +383,7 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
                     }
                 }
                 if (p.noise) {
-                    const T act = NF ? nres[nres_c == 2 ? c : 0][0] : (T)0;
+                    const T act = NF ? (nres_c == 2 && c == 1 ? nres[1][0] : nres[0][0]) : (T)0;
                     wigner_site<T>(v.x, v.y, mb.seed, (uint32_t)cell, step, 1u + (uint32_t)c, (T)mb.noise_strength,
                                    (T)mb.noise_gamma, (T)mb.noise_R, act, (T)mb.noise_dt_over_4v);
                 }
@@ -359,9 +391,9 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
             }
         } else {
             // stage right-hand sides (ifrk_fft_cuda_solver.py:128-142) and reservoir derivatives
-            for (int c = 0; c < C; ++c) {
-                const T nA = NF ? nres[nres_c == 2 ? c : 0][0] : (T)0;
-                const T nI = NF > 1 ? nres[nres_c == 2 ? c : 0][1] : (T)0;
+            for (int c = 0; c < 2; ++c) if (c < C) {
+                const T nA = NF ? (nres_c == 2 && c == 1 ? nres[1][0] : nres[0][0]) : (T)0;
+                const T nI = NF > 1 ? (nres_c == 2 && c == 1 ? nres[1][1] : nres[0][1]) : (T)0;
                 T er = gC * rho[c] + gR * nA + gI * nI, ei = 0;
                 if (p.potential) {
                     const c2<T> pot = p.potential[cell];
@@ -382,7 +414,7 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
                 }
                 buf[seq(0, c, 0) + slot(e)] = d;
             }
-            for (int rcix = 0; rcix < nres_c; ++rcix) {
+            for (int rcix = 0; rcix < 2; ++rcix) if (rcix < nres_c) {
                 T r = nres_c == 2 ? rho[rcix] : (C == 2 ? rho[0] + rho[1] : rho[0]);
                 T P = env[cell] * coef;
                 if (C == 2 && nres_c == 2) P = P * (T)(rcix == 0 ? mb.pump_w_plus : mb.pump_w_minus);
@@ -457,7 +489,7 @@ static int launch_cols_v(const pgpe_fft_plan* plan, LawsonParams<T> p, int seqs,
         if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(lawson_cols_kernel)");
     }
     const int groups = (p.nx + cols_per - 1) / cols_per;
-    hipLaunchKernelGGL(kern, dim3(groups * p.members), dim3(256), shm, s, p);
+    hipLaunchKernelGGL(kern, dim3(xcd_grid(groups * p.members)), dim3(256), shm, s, p);
     PGPE_LAUNCHED("lawson_cols_kernel");
     return PGPE_OK;
 }
