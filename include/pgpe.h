@@ -67,6 +67,9 @@ int pgpe_event_create(void** event);
 int pgpe_event_destroy(void* event);
 int pgpe_event_record(void* event, void* stream);
 int pgpe_event_elapsed_ms(void* start, void* stop, float* ms);
+/* The stream waits for the event's work (replaces the storage writer's                    */
+/* stream.wait_event, results/visitors/storage_visitor.py:188-189).                         */
+int pgpe_stream_wait_event(void* stream, void* event);
 /* Graph capture of a chunk of launches on one stream
  * (replaces rk4_cuda_solver.py:3396-3449 CUDA-graph chunks). */
 int pgpe_graph_begin(void* stream);
@@ -220,6 +223,12 @@ int pgpe_spectrum_line(const void* field_dev, void* slot_dev, int planes, int nx
 int pgpe_spectrum_block(const void* ring_dev, void* block_dev, const double* window_dev, int segment, int64_t width,
                         int position, int precision, void* stream);
 int pgpe_spectrum_power(const void* block_k_dev, double* psd_dev, int segment, int64_t width, int nx, void* stream);
+/* Result-frame observables (replaces _compute_density / _compute_stokes /                  */
+/* _compute_degree_of_polarization, simulation_controller.py:63-89) of a (members, components,  */
+/* plane) field: density (members, components, plane), and for two components the Stokes       */
+/* vector (members, 4, plane) and degree of polarisation (members, plane); null outputs skipped. */
+int pgpe_frame_fields(const void* psi_dev, int precision, int components, int64_t plane, int64_t members,
+                      void* density_dev, void* stokes_dev, void* dop_dev, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Batched 2-D FFT (LDS Stockham, radix 2-16) and the fused split-step  */

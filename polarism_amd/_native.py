@@ -133,6 +133,7 @@ _SIGNATURES = {
     "pgpe_event_destroy": (C.c_int, [vp]),
     "pgpe_event_record": (C.c_int, [vp, vp]),
     "pgpe_event_elapsed_ms": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
+    "pgpe_stream_wait_event": (C.c_int, [vp, vp]),
     "pgpe_graph_begin": (C.c_int, [vp]),
     "pgpe_graph_end": (C.c_int, [vp, C.POINTER(vp)]),
     "pgpe_graph_launch": (C.c_int, [vp, vp]),
@@ -161,6 +162,7 @@ _SIGNATURES = {
     "pgpe_spectrum_line": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, vp]),
     "pgpe_spectrum_block": (C.c_int, [vp, vp, vp, C.c_int, C.c_int64, C.c_int, C.c_int, vp]),
     "pgpe_spectrum_power": (C.c_int, [vp, vp, C.c_int, C.c_int64, C.c_int, vp]),
+    "pgpe_frame_fields": (C.c_int, [vp, C.c_int, C.c_int, C.c_int64, C.c_int64, vp, vp, vp, vp]),
 }
 
 _lib = None

@@ -164,6 +164,18 @@ class SimulationController:
         self._resumed = False
         if cfg.checkpoint.resume_from is not None:
             self.load_checkpoint(cfg.checkpoint.resume_from)
+        # result frames (simulation_controller.py:331-332, 395-420): NPY / JSON backends
+        from .results import FrameRecorder, StorageContext, storage_names
+        self.results = None
+        self.run_directory = None
+        if cfg.result.save_results and storage_names(cfg.result):
+            members = self.state.batch_size if self._member_cfgs is not None else None
+            context = StorageContext(
+                members=members,
+                member_indices=tuple(range(members)) if members is not None else None,
+                member_seeds=tuple(int(m.noise.seed) for m in self._member_cfgs) if members is not None else None)
+            self.results = FrameRecorder(self, context=context)
+            self.run_directory = self.results.output_dir
 
     # -- steady state ------------------------------------------------------------
     def _steady_buffers(self):
@@ -230,12 +242,17 @@ class SimulationController:
                 previous[signum] = signal.getsignal(signum)
                 signal.signal(signum, request_stop)
         schedules = self._accumulator_schedules(dt, n_steps)
+        recording = self.results is not None
+        save_steps = (_steps(self.cfg.result.save_every, dt, "result.save_every")
+                      if recording and self.cfg.result.save_every is not None else 1)
         try:
             step = self.state.step
             if step == 0 and not self._resumed:
                 for acc, start, _, stop in schedules:
                     if start == 0 and stop >= 0 and acc.samples == 0:
                         acc.update(self.state, self.state.t)
+                if recording:
+                    self.results.record(self.state.t)
             while step < n_steps and not self._stop_requested and not (steady and np.isfinite(self.stop_times).all()):
                 end = n_steps
                 if steady:
@@ -244,6 +261,8 @@ class SimulationController:
                     end = min(end, step + 1000)
                 if interval:
                     end = min(end, (step // interval + 1) * interval)
+                if recording:
+                    end = min(end, (step // save_steps + 1) * save_steps)
                 for _, start, every, stop in schedules:
                     due = _next_sample(step, start, every, stop)
                     if due is not None:
@@ -255,6 +274,9 @@ class SimulationController:
                         acc.update(self.state, self.state.t)
                 if steady and step % steady == 0 and self._steady_last_step != step:
                     self._check_steady_state()
+                stopped = bool(steady) and bool(np.isfinite(self.stop_times).all())
+                if recording and (step % save_steps == 0 or stopped):
+                    self.results.record(self.state.t)
                 if interval and step % interval == 0:
                     self.save_checkpoint(self._checkpoint_path())
                     last_checkpoint = step
@@ -264,14 +286,28 @@ class SimulationController:
             for signum, handler in previous.items():
                 signal.signal(signum, handler)
             if self._accumulators:
-                self.accumulated = {result.name: result for acc in self._accumulators.values() if acc.samples
-                                    for result in acc.results()}
+                finalized = [(acc, result) for acc in self._accumulators.values() if acc.samples
+                             for result in acc.results()]
+                self.accumulated = {result.name: result for _, result in finalized}
+            if recording:
+                try:
+                    if self._accumulators and self.accumulated:
+                        self.results.set_accumulated(self.accumulated, {
+                            result.name: {"samples": acc.samples, "t_first": acc.t_first, "t_last": acc.t_last}
+                            for acc, result in finalized})
+                    if steady:
+                        self.results.set_stop_times(self.stop_times)
+                finally:
+                    self.results.finalize()
 
     # -- checkpoints ---------------------------------------------------------------
     def _checkpoint_path(self) -> Path:
-        """checkpoint.path, else result.output_directory/checkpoint.npz."""
+        """checkpoint.path, else the run directory of the result frames, else
+        result.output_directory, holding checkpoint.npz."""
         if self.cfg.checkpoint.path is not None:
             return Path(self.cfg.checkpoint.path)
+        if self.run_directory is not None:
+            return Path(self.run_directory) / "checkpoint.npz"
         return Path(self.cfg.result.output_directory) / "checkpoint.npz"
 
     def _member_json(self):

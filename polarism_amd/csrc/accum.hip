@@ -87,6 +87,48 @@ __global__ void spectrum_power_kernel(const double2* __restrict__ bk, double* __
     }
 }
 
+// Result-frame observables of a field laid out (B, C, plane) (simulation_controller.py:63-89):
+// density |psi_c|^2 per component; for two components the Stokes vector in the circular basis
+// (S0, S1, S2, S3) = (|p|^2 + |m|^2, 2 Re(conj(p) m), 2 Im(conj(p) m), |p|^2 - |m|^2), laid out
+// (B, 4, plane), and the degree of polarisation |(S1, S2, S3)| / S0 (0 where S0 = 0), (B, plane).
+// Null outputs are skipped.
+template <typename T>
+__global__ void __launch_bounds__(256) frame_fields_kernel(const c2<T>* __restrict__ psi, int comps, int64_t plane,
+                                                           int64_t n, T* __restrict__ density,
+                                                           T* __restrict__ stokes, T* __restrict__ dop) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t b = i / plane, s = i - b * plane;
+        const c2<T> p = psi[b * comps * plane + s];
+        const T dp = p.x * p.x + p.y * p.y;
+        if (comps == 1) {
+            if (density) density[i] = dp;
+            continue;
+        }
+        const c2<T> m = psi[(b * 2 + 1) * plane + s];
+        const T dm = m.x * m.x + m.y * m.y;
+        if (density) {
+            density[b * 2 * plane + s] = dp;
+            density[(b * 2 + 1) * plane + s] = dm;
+        }
+        const T s0 = dp + dm;
+        const T s1 = (T)2 * (p.x * m.x + p.y * m.y);
+        const T s2 = (T)2 * (p.x * m.y - p.y * m.x);
+        const T s3 = dp - dm;
+        if (stokes) {
+            T* o = stokes + b * 4 * plane + s;
+            o[0] = s0;
+            o[plane] = s1;
+            o[2 * plane] = s2;
+            o[3 * plane] = s3;
+        }
+        if (dop) {
+            const T pol = sqrt(s1 * s1 + s2 * s2 + s3 * s3);
+            dop[i] = s0 > (T)0 ? pol / (s0 > (T)1e-30 ? s0 : (T)1e-30) : (T)0;
+        }
+    }
+}
+
 static dim3 accum_grid(int64_t n) {
     const int64_t blocks = (n + 4 * 256 - 1) / (4 * 256);
     return dim3((unsigned)(blocks < 1 ? 1 : (blocks > 65535 ? 65535 : blocks)));
@@ -172,5 +214,26 @@ extern "C" int pgpe_spectrum_power(const void* block_k, double* psd, int segment
     hipLaunchKernelGGL(spectrum_power_kernel, accum_grid((int64_t)segment * width), dim3(256), 0, (hipStream_t)stream,
                        (const double2*)block_k, psd, segment, width, nx);
     PGPE_LAUNCHED("spectrum_power_kernel");
+    return PGPE_OK;
+}
+
+extern "C" int pgpe_frame_fields(const void* psi, int precision, int components, int64_t plane, int64_t members,
+                                 void* density, void* stokes, void* dop, void* stream) {
+    if (!psi) return fail(PGPE_ERR_ARG, "pgpe_frame_fields: null field");
+    if (components != 1 && components != 2) return fail(PGPE_ERR_ARG, "pgpe_frame_fields: components must be 1 or 2");
+    if (components == 1 && (stokes || dop))
+        return fail(PGPE_ERR_ARG, "pgpe_frame_fields: Stokes fields need two components");
+    if (plane <= 0 || members <= 0) return fail(PGPE_ERR_ARG, "pgpe_frame_fields: bad size");
+    const int64_t n = plane * members;
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == 0)
+        hipLaunchKernelGGL(frame_fields_kernel<float>, accum_grid(n), dim3(256), 0, s, (const float2*)psi, components,
+                           plane, n, (float*)density, (float*)stokes, (float*)dop);
+    else if (precision == 1)
+        hipLaunchKernelGGL(frame_fields_kernel<double>, accum_grid(n), dim3(256), 0, s, (const double2*)psi,
+                           components, plane, n, (double*)density, (double*)stokes, (double*)dop);
+    else
+        return fail(PGPE_ERR_ARG, "pgpe_frame_fields: precision must be 0 or 1");
+    PGPE_LAUNCHED("frame_fields_kernel");
     return PGPE_OK;
 }

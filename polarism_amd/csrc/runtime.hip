@@ -170,6 +170,12 @@ int pgpe_event_record(void* event, void* stream) {
     return PGPE_OK;
 }
 
+int pgpe_stream_wait_event(void* stream, void* event) {
+    if (!event) return fail(PGPE_ERR_ARG, "pgpe_stream_wait_event: null event");
+    PGPE_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
+    return PGPE_OK;
+}
+
 int pgpe_event_elapsed_ms(void* start, void* stop, float* ms) {
     if (!start || !stop || !ms) return fail(PGPE_ERR_ARG, "pgpe_event_elapsed_ms: null argument");
     PGPE_HIP(hipEventSynchronize((hipEvent_t)stop));
