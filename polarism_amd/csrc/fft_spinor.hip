@@ -271,7 +271,9 @@ __global__ void __launch_bounds__(NT) fft_cols_spinor_kernel(const SplitParams<T
     lds_fft<T, V, NT>(buf, p.dims, ld, 2 * count, s_tw, false);
     const pgpe_member& mb = p.table[m];
     struct K { double kx, ky; };
-    staged<K, CH, NT>(
+    // the wave-number pairs are 16 bytes each: a shorter batch keeps them out of scratch at V = 32
+    constexpr int CK = CH > 8 ? 8 : CH;
+    staged<K, CK, NT>(
         count * n, [&](int i) { const int col = by_n.div(i); return K{p.kx[x0 + col], p.ky[i - col * n]}; },
         [&](int i, K q) {
             const int col = by_n.div(i);

@@ -157,6 +157,20 @@ def test_spinor_split_step_fp64_matches_oracle(method, kouter, kind, res_compone
         assert O.rel_l2(d.get(), r) <= 1e-10
 
 
+@pytest.mark.parametrize("ny,precision,tol", [(2048, "double", 1e-10), (2048, "single", 1e-5),
+                                               (4096, "single", 1e-5)])
+def test_tall_spinor_columns_match_oracle(ny, precision, tol):
+    """Columns of 2048 and 4096 points: the spinor column groups of 256 threads x 16 and x 32
+    elements (one column of psi+ and one of psi- per group); 10 steps against the oracle."""
+    cfg = _spinor_cfg(grid__nx=32, grid__ny=ny, grid__lx=16.0, grid__ly=0.5 * ny, solver__precision=precision,
+                      solver__total_time=0.05)
+    ctl = _run(cfg)
+    psi_ref, res_ref = _oracle_split_spinor(cfg, 10)
+    assert O.rel_l2(ctl.state.field.get(), psi_ref) <= tol
+    for d, r in zip(ctl.reservoir.get_state(), res_ref):
+        assert O.rel_l2(d.get(), r) <= tol
+
+
 @pytest.mark.parametrize("method,kouter", [("split-step-fft-hip", False), ("split-step-fft-kouter-hip", True)])
 @pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("single", 1e-5)])
 def test_spinor_split_step_with_wigner_noise_matches_oracle(method, kouter, precision, tol):
