@@ -279,11 +279,17 @@ int pgpe_split_step(const pgpe_split_args* args, void* stream);
 /* Lawson integrating-factor RK (replaces IFRKFFTCudaSolver._generic_step */
 /* and _adaptive_error, ifrk_fft_cuda_solver.py:148-293, and the fused   */
 /* kernels of ifrk_cuda_fused.py:10-374).                                */
-/* op STAGE: psi_i^k = L(frac0 h) psi0^k + sum_j hc[j] L(fr[j] h) f_j,    */
-/*   inverse transform, N(psi_i, n_i) with n_i = n0 + sum_j rc[j] dres_j,  */
-/*   writes dres_out and f_out = FFT(N).  stage_real = 1 takes psi_real    */
-/*   (real space) as psi_i instead of the combination.                    */
-/* op FINAL: psi_new^k = L(h) psi0^k + sum_j hc[j] L(fr[j] h) f_j, inverse */
+/* Stage values are kept in the interaction picture: the stage call with */
+/* frac0 = c_i writes f_out = g_i = L(-c_i h) FFT(N_i), so that           */
+/* L(c h) (psi0^k + sum_j hc[j] g_j) = L(c h) psi0^k                      */
+/*   + sum_j hc[j] L((c - c_j) h) FFT(N_j), the reference's combination   */
+/* with one propagator per k point instead of one per term (fr[] is not  */
+/* read).                                                                 */
+/* op STAGE: psi_i^k = L(frac0 h) (psi0^k + sum_j hc[j] f_j), inverse     */
+/*   transform, N(psi_i, n_i) with n_i = n0 + sum_j rc[j] dres_j,          */
+/*   writes dres_out and f_out = L(-frac0 h) FFT(N).  stage_real = 1 takes */
+/*   psi_real (real space) as psi_i instead of the combination.           */
+/* op FINAL: psi_new^k = L(h) (psi0^k + sum_j hc[j] f_j), inverse         */
 /*   into psi_out, res_out = max(n0 + sum_j rc[j] dres_j, 0), optional     */
 /*   noise; with nerr > 0 also the embedded error fields (he5, he3) and    */
 /*   per-member sums of (|e| / (atol + rtol max(|start|, |new|)))^2 into   */

@@ -83,7 +83,7 @@ struct LawsonParams {
 
 // Linear propagator applied to one k point.  For the scalar field the angle
 // follows ifrk_fft_cuda_solver.py:108 ((-hbar/2m) * (h f)) * k^2 in T; for the
-// spinor the 2x2 closed form of dispersion.py:66-83 is evaluated in double.
+// spinor the 2x2 closed form of dispersion.py:66-83 (spinor_propagate).
 template <typename T>
 __device__ __forceinline__ void apply_linear(const pgpe_member& mb, int comps, double kxv, double kyv, double tau,
                                              const c2<T>* v, c2<T>* out) {
@@ -97,40 +97,6 @@ __device__ __forceinline__ void apply_linear(const pgpe_member& mb, int comps, d
         return;
     }
     spinor_propagate<T>(mb, kxv, kyv, tau, v, out);
-}
-
-// The propagator of one stage term, evaluated once per k point and shared by every
-// combination using the term: the scalar phase (cos, sin) of the angle above, or the
-// spinor's M v.
-template <typename T>
-__device__ __forceinline__ void term_propagator(const pgpe_member& mb, int comps, double kxv, double kyv,
-                                                double tau, const c2<T>* v, c2<T>* prop) {
-    if (comps == 1) {
-        const T rate = (T)(-mb.hbar / (2.0 * mb.m_eff));
-        const T k2 = (T)(kxv * kxv + kyv * kyv);
-        const T angle = (rate * (T)tau) * k2;
-        T sn, cs;
-        sin_cos(angle, &sn, &cs);
-        prop[0] = make_c<T>(cs, sn);
-        return;
-    }
-    spinor_propagate<T>(mb, kxv, kyv, tau, v, prop);
-}
-
-// acc += s L(tau) v from the term's propagator.  Scalar: (s * phase) * v as
-// ifrk_fft_cuda_solver.py:173; spinor: s * (M v) as :171.
-template <typename T>
-__device__ __forceinline__ void add_term(int comps, T s, const c2<T>* v, const c2<T>* prop, c2<T>* acc) {
-    if (comps == 1) {
-        const c2<T> t = cmul<T>(make_c<T>(s * prop[0].x, s * prop[0].y), v[0]);
-        acc[0].x = acc[0].x + t.x;
-        acc[0].y = acc[0].y + t.y;
-        return;
-    }
-    for (int c = 0; c < 2; ++c) {
-        acc[c].x = acc[c].x + s * prop[c].x;
-        acc[c].y = acc[c].y + s * prop[c].y;
-    }
 }
 
 template <typename T, int RES>
@@ -189,17 +155,30 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
         }
         __syncthreads();
         lds_fft<T, V>(buf, p.dims, ld, cnt * p.comps, s_tw, false);
+        if (p.frac0 != 0.0) {
+            // stage values are kept as g_i = L(-c_i h) FFT(N_i): every later combination then
+            // needs one propagator instead of one per term
+            __syncthreads();
+            for (int e = threadIdx.x; e < cnt * n; e += 256) {
+                const int y = e / cnt, i = e - y * cnt;
+                c2<T> v[2], o[2];
+                for (int c = 0; c < 2; ++c) if (c < p.comps) v[c] = buf[seq(0, c, i) + pad_at(y)];
+                apply_linear<T>(mb, p.comps, p.kx[x0 + i], p.ky[y], -p.frac0 * p.h, v, o);
+                for (int c = 0; c < 2; ++c) if (c < p.comps) buf[seq(0, c, i) + pad_at(y)] = o[c];
+            }
+            __syncthreads();
+        }
     } else {
         // build the combination for every (k point, component) of the group
         for (int e = threadIdx.x; e < cnt * n; e += 256) {
             const int y = e / cnt, i = e - y * cnt;
             const int64_t g = (int64_t)y * p.nx + x0 + i;
             const double kxv = p.kx[x0 + i], kyv = p.ky[y];
-            c2<T> v[2], acc[2];
-            for (int c = 0; c < 2; ++c) if (c < p.comps) v[c] = p.psi0k[(m * p.comps + c) * plane + g];
-            apply_linear<T>(mb, p.comps, kxv, kyv, p.frac0 * p.h, v, acc);
-            // error combinations of the final step (DP5: one, DOP853: two) share each term's
-            // propagator with the solution's combination
+            // psi_i = L(c_i h) (psi0 + sum_j hc_j g_j) with g_j = L(-c_j h) f_j, i.e.
+            // L(c_i h) psi0 + sum_j hc_j L((c_i - c_j) h) f_j (ifrk_fft_cuda_solver.py:171-173);
+            // the error fields of the final step (DP5: one, DOP853: two) likewise
+            c2<T> sum[2];
+            for (int c = 0; c < 2; ++c) if (c < p.comps) sum[c] = p.psi0k[(m * p.comps + c) * plane + g];
             constexpr int QE = MODE == LC_FINAL ? 2 : 0;
             c2<T> er[QE > 0 ? QE : 1][2];
             for (int q = 0; q < QE; ++q) er[q][0] = er[q][1] = make_c<T>(0, 0);
@@ -216,12 +195,31 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
 #pragma unroll
             for (int j = 0; j < kMaxStages; ++j) {
                 if (!used[j]) continue;
-                c2<T> prop[2];
-                term_propagator<T>(mb, p.comps, kxv, kyv, p.fr[j] * p.h, vt[j], prop);
-                if (p.hc[j] != 0.0) add_term<T>(p.comps, (T)p.hc[j], vt[j], prop, acc);
+                if (p.hc[j] != 0.0) {
+                    const T w = (T)p.hc[j];
+                    for (int c = 0; c < 2; ++c) if (c < p.comps) {
+                        sum[c].x = sum[c].x + w * vt[j][c].x;
+                        sum[c].y = sum[c].y + w * vt[j][c].y;
+                    }
+                }
                 for (int q = 0; q < QE; ++q)
-                    if (q < p.nerr && p.he[q][j] != 0.0) add_term<T>(p.comps, (T)p.he[q][j], vt[j], prop, er[q]);
+                    if (q < p.nerr && p.he[q][j] != 0.0) {
+                        const T w = (T)p.he[q][j];
+                        for (int c = 0; c < 2; ++c) if (c < p.comps) {
+                            er[q][c].x = er[q][c].x + w * vt[j][c].x;
+                            er[q][c].y = er[q][c].y + w * vt[j][c].y;
+                        }
+                    }
             }
+            c2<T> acc[2];
+            apply_linear<T>(mb, p.comps, kxv, kyv, p.frac0 * p.h, sum, acc);
+            for (int q = 0; q < QE; ++q)
+                if (q < p.nerr) {
+                    c2<T> o[2] = {make_c<T>(0, 0), make_c<T>(0, 0)};
+                    apply_linear<T>(mb, p.comps, kxv, kyv, p.frac0 * p.h, er[q], o);
+                    er[q][0] = o[0];
+                    er[q][1] = o[1];
+                }
             for (int c = 0; c < 2; ++c) if (c < p.comps) {
                 buf[seq(0, c, i) + pad_at(y)] = acc[c];
                 if (MODE == LC_FINAL && p.psik_out) p.psik_out[(m * p.comps + c) * plane + g] = acc[c];
@@ -494,9 +492,8 @@ static int launch_cols_v(const pgpe_fft_plan* plan, LawsonParams<T> p, int seqs,
     return PGPE_OK;
 }
 
-// A column group holds kinds x comps sequences per column; tall spinor grids with error
-// fields take twice the elements per thread (V = 32 or 24) when one column's sequences
-// exceed 256 x V.
+// A column group holds kinds x comps sequences per column; it takes twice the elements per
+// thread (V = 32 or 24) when one column's sequences exceed 256 x V or fill it alone.
 template <typename T, int MODE>
 static int launch_cols(const pgpe_fft_plan* plan, LawsonParams<T> p, hipStream_t s) {
     const int kinds = 1 + (MODE == LC_COMBINE ? p.lap : (MODE == LC_FINAL ? p.nerr : 0));
@@ -504,7 +501,11 @@ static int launch_cols(const pgpe_fft_plan* plan, LawsonParams<T> p, hipStream_t
     const int need = plan->ny * seqs;
     const int V = plan->vy;
     auto fits = [&](int v) { return 256 * v >= need && lds_bytes<T>(256 * v / need * seqs, plan->ny) <= 163840; };
-    if (fits(V))
+    // a group of a single column reads 8- or 16-byte row segments: twice the elements per
+    // thread gives two columns (measured: the DP5 final combination on 1024^2 spinor fields
+    // 159 -> 128 us); groups of two or more columns keep V (wider ones measured slower)
+    const bool single = fits(V) && 256 * V / need < 2;
+    if (fits(V) && !(single && fits(2 * V)))
         return V == 16 ? launch_cols_v<T, 16, MODE>(plan, p, seqs, s) : launch_cols_v<T, 12, MODE>(plan, p, seqs, s);
     if (fits(2 * V))
         return V == 16 ? launch_cols_v<T, 32, MODE>(plan, p, seqs, s) : launch_cols_v<T, 24, MODE>(plan, p, seqs, s);
