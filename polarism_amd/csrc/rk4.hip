@@ -220,7 +220,8 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
     const int tile = xcd_tile(blockIdx.x, p.num_tiles);
     if (tile < 0) return;
     const int m = p.tile_map[2 * tile];
-    const pgpe_member& mb = p.table[m];
+    // a copy of the member row: its fields stay in registers past the global stores
+    const pgpe_member mb = p.table[m];
     const int nx = mb.nx, ny = mb.ny;
     const int local = p.tile_map[2 * tile + 1];
     const int x0 = (local % mb.tiles_x) * kTileW;
@@ -371,7 +372,8 @@ __global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T
     const int tile = xcd_tile(blockIdx.x, p.num_tiles);
     if (tile < 0) return;
     const int m = p.tile_map[2 * tile];
-    const pgpe_member& mb = p.table[m];
+    // a copy of the member row: its fields stay in registers past the global stores
+    const pgpe_member mb = p.table[m];
     const int nx = mb.nx, ny = mb.ny;
     const int local = p.tile_map[2 * tile + 1];
     const int x0 = (local % mb.tiles_x) * kTileW;
@@ -571,7 +573,8 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
     const int tile = xcd_tile(blockIdx.x, p.num_tiles);
     if (tile < 0) return;
     const int m = p.tile_map[2 * tile];
-    const pgpe_member& mb = p.table[m];
+    // a copy of the member row: its fields stay in registers past the global stores
+    const pgpe_member mb = p.table[m];
     const int nx = mb.nx, ny = mb.ny;
     const int local = p.tile_map[2 * tile + 1];
     const int x0 = (local % mb.tiles_x) * kTileW;
