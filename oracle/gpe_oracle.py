@@ -40,7 +40,13 @@ Pinning (what checks each function; tests/ file in brackets):
   [test_oracle_pins.py]; through it, the golden trajectories above.
 * LawsonProblem adaptive + error + StepController + lawson_advance_adaptive:
   closed-form uniform decay within 50 rtol (tests/unit/test_adaptive_dt.py:34-38),
-  rejection / tick landing, factor bounds, underflow guard [test_oracle_pins.py].
+  rejection / tick landing, factor bounds, underflow guard; the embedded error norm
+  (DP5 sqrt(S/count), DOP853 blend, psi and reservoir points counted, max over two
+  members) against the embedded stability functions of the tableaus on a linear
+  uniform problem [test_oracle_pins.py].
+* LawsonProblem.step with noise: the noisy step minus the plain one is exactly the
+  Wigner increment of that step index with the final reservoir; the reservoir is
+  untouched [test_oracle_pins.py].
 * oracle_tableau: order conditions, c = A 1, and an independent rational
   transcription [test_tableaus.py].
 * laplacian: discrete cosine eigenfunction, second order [test_oracle_pins.py].
@@ -49,7 +55,12 @@ Pinning (what checks each function; tests/ file in brackets):
 * fdm_rhs: uniform closed-form steady states of all three reservoirs (a fixed
   point with mu/hbar phase rotation), Zeeman sign vs a scalar potential
   [test_oracle_pins.py].
-* rk4_fdm_step: 4th-order convergence on the uniform closed form [test_oracle_pins.py].
+* rk4_fdm_step: 4th-order convergence on the uniform closed form; the kinetic,
+  eta, g_R and loss terms by plane waves (periodic, 5- and 9-point symbols) and
+  Neumann cosines (closed interval) after n steps, psi_n = R(dt rate)^n psi_0 with
+  R the RK4 stability polynomial; the spinor TE-TM / Zeeman 2x2 stencil matrix on a
+  plane wave (matrix polynomial); alpha_2 and Zeeman on a uniform spinor closed
+  form [test_oracle_pins.py].
 * reservoir_derivatives / reservoir_rk2 / thresholds: the threshold makes the
   gain vanish at the psi = 0 fixed point; exact RK2 decay factor; clamp.
 * split_step_*: exact for uniform fields, plane-wave phase, orderings
@@ -645,7 +656,13 @@ class LawsonProblem:
     def linear(self, fraction, h, fk):
         p = self.p
         if fk.shape[1] == 2:
-            M = spinor_propagator(self.grid, p, h * fraction).astype(self.cplx)
+            tau = float(h * fraction)
+            cache = self.__dict__.setdefault("_spinor_cache", {})
+            M = cache.get(tau)
+            if M is None:
+                if len(cache) > 48:
+                    cache.clear()
+                M = cache[tau] = spinor_propagator(self.grid, p, tau).astype(self.cplx)
             return apply_2x2(M, fk)
         rate = self.real(-p.hbar / (2.0 * p.m_eff))
         angle = rate * self.real(h * fraction) * self.k2
@@ -671,8 +688,14 @@ class LawsonProblem:
             return reservoir_derivatives(self.kind, self.p, rho.sum(axis=1, keepdims=True), pump, res)
         return reservoir_derivatives(self.kind, self.p, rho, pump, res)
 
-    def step(self, psi0, res0, pump_at, t, h):
-        """One Lawson trial.  Returns (accepted, error, psi_new, res_new)."""
+    def step(self, psi0, res0, pump_at, t, h, noise=None, step=0):
+        """One Lawson trial.  Returns (accepted, error, psi_new, res_new).
+
+        ``noise`` (a :func:`wigner_add` keyword dict without dt / step) adds the truncated-Wigner
+        increment of step index ``step`` to the combined field in real space, with the final
+        reservoir as the active density, and returns to k-space from there
+        (solver/ifrk_cuda_fused.py:908-923: inverse transform, add_wigner on res_out[0], forward
+        transform).  The fused fixed-step path is the only one that injects it."""
         tab = self._tab
         sc = self.real
         fft = lambda a: np.fft.fft2(a, axes=(-2, -1)).astype(self.cplx, copy=False)
@@ -720,6 +743,8 @@ class LawsonProblem:
             res_new.append(np.maximum(v, 0).astype(self.real, copy=False))
         res_new = tuple(res_new)
         psi_new = ifft(psi_new_k)
+        if noise is not None:
+            wigner_add(psi_new, res_new[0] if res_new else 0, dt=h, step=step, **noise)
         if not self.adaptive:
             return True, None, psi_new, res_new
         error = self.error(h, psi0, psi_new, psi0_k, f, res0, res_new, derivs)

@@ -518,3 +518,213 @@ def test_closed_grid_matches_reference_geometry():
     assert (g.dx, g.dy) == (1.0, 1.0)
     np.testing.assert_array_equal(g.X[0], [-2, -1, 0, 1, 2])
     np.testing.assert_array_equal(g.Y[:, 0], [-1, 0, 1])
+
+
+# ---------------------------------------------------------------- kinetic, eta and TE-TM terms of the FDM step
+def _rk4_poly(z):
+    """Stability polynomial of classic RK4: one step of y' = lambda y multiplies y by R(h lambda)."""
+    return 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+
+
+def _rk4_poly_matrix(Z):
+    eye = np.eye(Z.shape[0], dtype=complex)
+    return eye + Z + Z @ Z / 2 + Z @ Z @ Z / 6 + Z @ Z @ Z @ Z / 24
+
+
+def _linear_physics():
+    """No self-interaction and no stimulated scattering: the psi equation is linear and the
+    reservoir, held at its fixed point nA0 by P = gamma_R nA0, acts as a constant."""
+    return O.Physics(g_C=0.0, R=0.0, gamma_C=0.07, g_R=0.002, gamma_R=0.005, kinetic_relaxation_eta=0.3)
+
+
+def _linear_rate(p, lam, nA0):
+    """d psi / dt = rate psi for a Laplacian eigenfunction with eigenvalue -lam (lam >= 0):
+    kinetic -(hbar^2/2m) lap gives -i (hbar/2m) lam, g_R nA0 a phase, -gamma_C/2 the loss and
+    eta nA0 (hbar/2m) lap the damping -eta nA0 (hbar/2m) lam."""
+    hb2m = p.hbar / (2 * p.m_eff)
+    return -1j * (hb2m * lam + p.g_R * nA0 / p.hbar) - p.gamma_C / 2 - p.kinetic_relaxation_eta * nA0 * hb2m * lam
+
+
+@pytest.mark.parametrize("stencil", ["five-point", "isotropic-9pt"])
+def test_rk4_fdm_plane_wave_follows_the_discrete_symbol(stencil):
+    """A plane wave stays a plane wave: after n steps psi = R(dt rate)^n psi0, with rate from the
+    stencil's symbol.  A flipped kinetic sign, a wrong hbar/2m or a wrong eta term fails this."""
+    g = O.Grid(48, 48, 36.0, 36.0)
+    p = _linear_physics()
+    nA0, dt, n = 1.7, 0.05, 12
+    kx, ky = 2 * np.pi * 3 / g.lx, -2 * np.pi * 2 / g.ly
+    psi = (0.2 * np.exp(1j * (kx * g.X + ky * g.Y))).reshape(1, 1, *g.shape)
+    res = (np.full((1, 1) + g.shape, nA0),)
+    pump = np.full((1, 1) + g.shape, p.gamma_R * nA0)
+    sym = _sym5(kx, g.dx) + _sym5(ky, g.dy) if stencil == "five-point" else _sym9(kx, ky, g.dx)
+    z = dt * _linear_rate(p, -sym, nA0)
+    expect = _rk4_poly(z) ** n * psi
+    for s in range(n):
+        psi, res = O.rk4_fdm_step("single", p, g, psi, res, lambda t: pump, s * dt, dt, laplacian_kind=stencil)
+    np.testing.assert_allclose(psi, expect, rtol=1e-12, atol=1e-14)
+    np.testing.assert_array_equal(res[0], nA0)
+    # the phase turns the way exp(-i E t / hbar) does: clockwise for positive kinetic energy
+    assert np.angle(_rk4_poly(z)) < 0
+
+
+@pytest.mark.parametrize("stencil", ["five-point", "isotropic-9pt"])
+def test_rk4_fdm_neumann_cosine_on_a_closed_interval(stencil):
+    """On the closed interval the Neumann cosine is an eigenfunction of the mirrored stencil:
+    psi_n = R(dt rate)^n psi_0 as for the plane wave."""
+    g = O.Grid(33, 25, 16.0, 12.0, boundary="closed-interval")
+    assert g.dx == g.dy == 0.5
+    p = _linear_physics()
+    nA0, dt, n = 0.9, 0.05, 10
+    kx, ky = 3 * np.pi / g.lx, 2 * np.pi / g.ly
+    psi = (0.3 * np.cos(kx * (g.X + g.lx / 2)) * np.cos(ky * (g.Y + g.ly / 2))).astype(complex).reshape(1, 1, *g.shape)
+    res = (np.full((1, 1) + g.shape, nA0),)
+    pump = np.full((1, 1) + g.shape, p.gamma_R * nA0)
+    sym = _sym5(kx, g.dx) + _sym5(ky, g.dy) if stencil == "five-point" else _sym9(kx, ky, g.dx)
+    expect = _rk4_poly(dt * _linear_rate(p, -sym, nA0)) ** n * psi
+    for s in range(n):
+        psi, res = O.rk4_fdm_step("single", p, g, psi, res, lambda t: pump, s * dt, dt, laplacian_kind=stencil)
+    np.testing.assert_allclose(psi, expect, rtol=1e-12, atol=1e-14)
+
+
+def test_rk4_fdm_spinor_plane_wave_follows_the_discrete_te_tm_matrix():
+    """Both components on one plane wave: d psi/dt = (-i/hbar) H_d psi - gamma_C/2 psi with the
+    stencil's 2x2 matrix H_d = [[E + Z/2, T_+], [T_-, E - Z/2]], T_-/+ = -beta[(sx - sy) -/+ 2i sxy]
+    from the second differences and the 4-corner mixed difference.  n RK4 steps multiply the
+    spinor by R(dt M)^n.  This fixes the TE-TM sign and which component each row couples to, and
+    the Zeeman sign, in the whole step."""
+    g = O.Grid(40, 32, 40.0, 32.0)
+    p = O.Physics(g_C=0.0, R=0.0, gamma_C=0.05, te_tm=0.05, zeeman=0.02, alpha_2=0.0)
+    dt, n = 0.1, 8
+    kx, ky = 2 * np.pi * 2 / g.lx, 2 * np.pi * 3 / g.ly
+    wave = np.exp(1j * (kx * g.X + ky * g.Y))
+    a = np.array([0.3 + 0.1j, -0.2 + 0.25j])
+    psi = (a[:, None, None] * wave)[None]
+    sx, sy = _sym5(kx, g.dx), _sym5(ky, g.dy)
+    sxy = -np.sin(kx * g.dx) * np.sin(ky * g.dy) / (g.dx * g.dy)
+    E = -(p.hbar ** 2) / (2 * p.m_eff) * (sx + sy)
+    H = np.array([[E + p.zeeman / 2, -p.te_tm * ((sx - sy) - 2j * sxy)],
+                  [-p.te_tm * ((sx - sy) + 2j * sxy), E - p.zeeman / 2]])
+    M = (-1j / p.hbar) * H - p.gamma_C / 2 * np.eye(2)
+    amp = np.linalg.matrix_power(_rk4_poly_matrix(dt * M), n) @ a
+    expect = (amp[:, None, None] * wave)[None]
+    for s in range(n):
+        psi, _ = O.rk4_fdm_step("none", p, g, psi, (), _zero_pump(g), s * dt, dt)
+    np.testing.assert_allclose(psi, expect, rtol=1e-12, atol=1e-14)
+    # the continuum limit of H_d is the dispersion matrix: T_+ -> beta (kx - i ky)^2
+    Hc = O.spinor_hamiltonian(g, p)[:, :, 3, 2]
+    assert abs(H[0, 1] - Hc[0, 1]) < 0.1 * abs(Hc[0, 1])       # O((k dx)^2) apart; a sign error is O(1)
+    assert abs(H[1, 0] - Hc[1, 0]) < 0.1 * abs(Hc[1, 0])
+
+
+def test_rk4_fdm_uniform_spinor_alpha2_and_zeeman_closed_form():
+    """Without gain or loss each uniform component only turns its phase:
+    psi_+/-(t) = psi_+/-(0) exp(-i (g_C |psi_+/-|^2 + alpha_2 |psi_-/+|^2 +/- Z/2) t / hbar).
+    RK4 converges to it at fourth order; alpha_2 on the wrong density or a flipped Zeeman sign
+    leaves an O(1) error."""
+    g = O.Grid(4, 4, 4.0, 4.0)
+    p = O.Physics(g_C=0.4, gamma_C=0.0, R=0.0, alpha_2=-0.15, zeeman=0.05)
+    a = np.array([0.9 + 0.2j, 0.35 - 0.1j])
+    rho = np.abs(a) ** 2
+    T = 2.0
+    mu = np.array([p.g_C * rho[0] + p.alpha_2 * rho[1] + p.zeeman / 2,
+                   p.g_C * rho[1] + p.alpha_2 * rho[0] - p.zeeman / 2])
+    exact = a * np.exp(-1j * mu * T / p.hbar)
+    errs = []
+    for dt in (0.2, 0.1):
+        psi = np.broadcast_to(a[:, None, None], (2,) + g.shape)[None].astype(complex)
+        for s in range(int(round(T / dt))):
+            psi, _ = O.rk4_fdm_step("none", p, g, psi, (), _zero_pump(g), s * dt, dt)
+        errs.append(np.max(np.abs(psi[0, :, 0, 0] - exact)))
+    assert errs[1] < 1e-6
+    assert 13 < errs[0] / errs[1] < 19
+
+
+# ---------------------------------------------------------------- embedded error norm
+def _increment(tab, weights, z):
+    """z w^T (I - z A)^-1 1: one step's increment of y' = lambda y per unit y, z = h lambda."""
+    s = len(tab.c)
+    stage = np.linalg.solve(np.eye(s) - z * tab.A, np.ones(s, dtype=complex))
+    return z * np.dot(weights, stage)
+
+
+def _stability(tab, weights, z):
+    """1 + z w^T (I - z A)^-1 1 for the tableau's A and the weight vector w."""
+    return 1 + _increment(tab, weights, z)
+
+
+@pytest.mark.parametrize("tableau", ["dp5", "dop853"])
+def test_lawson_error_norm_closed_form_two_members_with_reservoir(tableau):
+    """Linear uniform problem: psi' = mu psi (mu = -i V/hbar - gamma_C/2) and, decoupled,
+    n' = P - gamma_R n.  The embedded error of each field is (R_b - R_bhat)(z) times its
+    deviation, so the per-member norm follows from the stability functions alone:
+    DP5 sqrt(S5 / count), DOP853 S5 / sqrt((S5 + 0.01 S3) count), S = sum of squared scaled
+    errors over psi and reservoir points, count = both point counts; the trial reports the
+    largest member.  A dropped count, a wrong blend or a missing member max fails."""
+    g = O.Grid(4, 4, 4.0, 4.0)
+    V = 0.05
+    p = O.Physics(g_C=0.0, R=0.0, g_R=0.0, gamma_C=0.6, gamma_R=0.3)
+    tab = O.oracle_tableau(tableau)
+    h, rtol, atol, P0 = (0.8 if tableau == "dp5" else 2.0), 1e-6, 1e-3, 0.02
+    a = np.array([0.04 - 0.02j, 1.0 + 0.5j])
+    n0 = np.array([0.01, 0.2])
+    psi = np.broadcast_to(a[:, None, None, None], (2, 1) + g.shape).astype(complex)
+    res = (np.broadcast_to(n0[:, None, None, None], (2, 1) + g.shape).astype(float),)
+    pump = np.full((1, 1) + g.shape, P0)
+    prob = O.LawsonProblem("single", p, g, potential=V, tableau=tableau, adaptive=True, rtol=rtol, atol=atol)
+    accepted, err, psi1, res1 = prob.step(psi, res, lambda t: pump, 0.0, h)
+
+    z = h * (-1j * V / p.hbar - p.gamma_C / 2)
+    zr = -p.gamma_R * h
+    nstar = P0 / p.gamma_R
+    b = np.asarray(tab.b, dtype=float)
+    np.testing.assert_allclose(psi1[:, 0, 0, 0], _stability(tab, b, z) * a, rtol=1e-13)
+    np.testing.assert_allclose(res1[0][:, 0, 0, 0], nstar + _stability(tab, b, zr).real * (n0 - nstar), rtol=1e-13)
+
+    def sums(bhat):
+        w = b - np.asarray(bhat, dtype=float)
+        e_psi = np.abs(_increment(tab, w, z) * a)
+        e_res = np.abs(_increment(tab, w, zr).real * (n0 - nstar))
+        sc_psi = atol + rtol * np.maximum(np.abs(a), np.abs(psi1[:, 0, 0, 0]))
+        sc_res = atol + rtol * np.maximum(np.abs(n0), np.abs(res1[0][:, 0, 0, 0]))
+        return 16 * (e_psi / sc_psi) ** 2 + 16 * (e_res / sc_res) ** 2
+
+    count = 32
+    s5 = sums(tab.b_hat)
+    if tab.b_hat3 is None:
+        member = np.sqrt(s5 / count)
+    else:
+        s3 = sums(tab.b_hat3)
+        member = s5 / np.sqrt((s5 + 0.01 * s3) * count)
+    assert member[1] > 1.5 * member[0]            # the second member decides
+    if tab.b_hat3 is not None:
+        assert np.all(0.01 * s3 > 0.1 * s5)       # the blend term matters
+    # the oracle sums the embedded terms field by field, so it carries their cancellation
+    assert math.isclose(err, float(member.max()), rel_tol=1e-7)
+    assert accepted == (err <= 1)
+
+
+# ---------------------------------------------------------------- Wigner noise in the Lawson step
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_lawson_noise_is_the_wigner_increment_on_the_final_state(precision):
+    """The noisy fixed step equals the noiseless one plus the Wigner increment of that step's
+    index, scaled by the final reservoir; the reservoir itself is not touched by the noise."""
+    g = O.Grid(16, 16, 20.0, 20.0)
+    p = O.Physics()
+    real = np.float32 if precision == "single" else np.float64
+    rng = np.random.default_rng(11)
+    psi = (0.2 * (rng.standard_normal((1, 1) + g.shape) + 1j * rng.standard_normal((1, 1) + g.shape))).astype(
+        np.complex64 if precision == "single" else np.complex128)
+    res = (np.abs(rng.standard_normal((1, 1) + g.shape)).astype(real),)
+    pump = np.full((1, 1) + g.shape, 0.05, dtype=real)
+    prob = O.LawsonProblem("single", p, g, precision=precision, tableau="rk4")
+    noise = dict(seeds=[1234], gamma=[p.gamma_C], scattering=[p.R], strength=[1.0], volume=g.dx * g.dy,
+                 precision=precision)
+    _, _, plain, res_plain = prob.step(psi, res, lambda t: pump, 0.3, 0.01)
+    _, _, noisy, res_noisy = prob.step(psi, res, lambda t: pump, 0.3, 0.01, noise=noise, step=7)
+    np.testing.assert_array_equal(res_noisy[0], res_plain[0])
+    inc = np.zeros_like(plain)
+    O.wigner_add(inc, res_plain[0], dt=0.01, step=7, **noise)
+    np.testing.assert_array_equal(noisy - plain, (plain + inc) - plain)
+    sigma = np.sqrt(p.gamma_C + p.R * res_plain[0].astype(np.float64)) * np.sqrt(0.01 / (4 * g.dx * g.dy))
+    z = (inc / sigma)[0, 0]
+    assert abs(np.mean(z.real)) < 0.3 and 0.6 < np.var(z.real) < 1.5 and 0.6 < np.var(z.imag) < 1.5
