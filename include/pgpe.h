@@ -201,6 +201,22 @@ int pgpe_density_delta(const void* psi_dev, void* old_density_dev, int precision
                        const int32_t* tile_map_dev, int num_tiles, int members,
                        double* partials_dev, double* sums_dev, int update, void* stream);
 
+/* Allocation-free variant of pgpe_density_delta.  No counterpart in the    */
+/* reference: it is for callers that cannot allocate device memory (code    */
+/* under HIP graph capture, tight serving loops).  The per-tile partial      */
+/* sums live in a file-scope device array of the library (1.5 MiB per       */
+/* device, held until the process ends) instead of partials_dev, so no      */
+/* workspace argument is taken and nothing is allocated.  total_tiles is    */
+/* the member table's tile count (the partial slots it addresses); more     */
+/* than 65536 is rejected with PGPE_ERR_ARG.  The partials are cleared at   */
+/* the start of every call (members without tiles in the map sum to 0).     */
+/* Every call on one device shares that array: calls that overlap on one    */
+/* device (other streams or threads) race and must be serialised.           */
+int pgpe_density_delta_noalloc(const void* psi_dev, void* old_density_dev, int precision,
+                               int components, const pgpe_member* table_dev,
+                               const int32_t* tile_map_dev, int num_tiles, int members,
+                               int total_tiles, double* sums_dev, int update, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* In-run accumulators (replace the device sums of                     */
 /* results/accumulators.py:94-188).  n points of a complex field; the   */

@@ -157,6 +157,8 @@ _SIGNATURES = {
     "pgpe_mode_scale_grad": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
     "pgpe_colsum": (C.c_int, [vp, vp, C.c_int64, C.c_int64, C.c_int64, C.c_int64, vp]),
     "pgpe_density_delta": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp]),
+    "pgpe_density_delta_noalloc": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, vp,
+                                             C.c_int, vp]),
     "pgpe_accumulate_power": (C.c_int, [vp, vp, C.c_int64, C.c_int, vp]),
     "pgpe_accumulate_moments": (C.c_int, [vp, vp, vp, C.c_int64, C.c_int, vp]),
     "pgpe_spectrum_line": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, vp]),

@@ -14,6 +14,10 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t err, const char* what);
 
+// Looks up (once per device, with that device current) the device address of the
+// allocation-free reduction's scratch array; `out` may be null.
+int reduce_scratch_setup(double** out);
+
 #define PGPE_HIP(call)                                          \
     do {                                                        \
         hipError_t e_ = (call);                                 \

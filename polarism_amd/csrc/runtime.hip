@@ -57,7 +57,7 @@ int pgpe_device_count(int* count) {
 int pgpe_device_init(int device) {
     PGPE_HIP(hipSetDevice(device));
     PGPE_HIP(hipFree(nullptr));
-    return PGPE_OK;
+    return reduce_scratch_setup(nullptr);
 }
 
 int pgpe_device_name(int device, char* buf, size_t len) {
