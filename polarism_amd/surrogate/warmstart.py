@@ -13,6 +13,10 @@
 * ``warm_start``    -- predicted (psi, n_R) as the initial state of the same
   steady-state run; a member predicted to hold no condensate (peak density
   below ``ZERO_STATE_PEAK``) starts from psi = 0 with the predicted reservoir.
+  psi = 0 is a fixed point of the noiseless dynamics, so such a member meets the
+  criterion at once and a wrong "no condensate" prediction cannot correct itself:
+  the noise-seeded cold start of the same member is the judge of every warm
+  result (a member counts only where both converge to the same density).
 
 The inputs carry no coordinate channels, so the network is equivariant to
 the translations of the periodic box, and training augments every batch with
@@ -36,8 +40,11 @@ IN_FEATURES = 4
 # A member whose predicted peak density stays below this (|psi|^2, in the units of the solver)
 # is started from psi = 0, the steady state of a pump below threshold: a predicted residue of
 # order 1e-3 would otherwise have to decay to the norm < 1e-30 test like a cold start does.
-# Condensed steady states of the training distribution peak above ~0.05; the surrogate's
-# residue on below-threshold members stays under ~0.01.
+# (Starting such a member from the cold start's noise field instead was measured slower than
+# the cold start itself: with the predicted reservoir already at P / gamma_R the noise decays
+# more slowly than from an empty reservoir.)  Condensed steady states of the training
+# distribution peak above ~0.05; the surrogate's residue on below-threshold members stays
+# under ~0.01.
 ZERO_STATE_PEAK = 1e-2
 
 
@@ -249,7 +256,8 @@ def warm_start(members, y_pred, norm: Normaliser | None = None, states=None):
 
     The cold start's noise field (RMS 1e-3) is not added: the prediction's own error is larger,
     so it seeds any symmetry breaking the dynamics needs.  A member predicted to hold no
-    condensate (peak density below ``ZERO_STATE_PEAK``) starts from psi = 0 exactly."""
+    condensate (peak density below ``ZERO_STATE_PEAK``) starts from psi = 0 exactly; only the
+    cold start of the same member can confirm that prediction."""
     norm = norm or default_normaliser()
     B = len(members)
     if states is not None:

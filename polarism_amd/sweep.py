@@ -53,6 +53,7 @@ class SweepResult:
     final_res: list
     relative_density: np.ndarray    # last relative density change per member
     wall_to_last_s: float = float("nan")  # wall time from launch to the last convergence
+    wall_at_stop: np.ndarray | None = None  # s from launch to the check that found each member converged
 
 
 class PackedSweep:
@@ -144,6 +145,7 @@ class PackedSweep:
         points = np.asarray([m.nx * m.ny for m in self.members], dtype=np.float64)
         active = np.ones(M, dtype=bool)
         stop = np.full(M, np.nan)
+        wall_at_stop = np.full(M, np.nan)
         rel_d_last = np.full(M, np.inf)
         prev_norm = None
         point_steps = 0.0
@@ -171,6 +173,7 @@ class PackedSweep:
                     stop[newly] = t
                     if newly.any():
                         last_conv = time.perf_counter() - t0
+                        wall_at_stop[newly] = last_conv
                     if newly.any() and self.retire:
                         active &= ~newly
                         self.engine.set_active_tiles(self.table.tiles_of(np.flatnonzero(active)))
@@ -193,4 +196,5 @@ class PackedSweep:
             n = m.nx * m.ny
             final_psi.append(psi_all[o:o + n].reshape(m.ny, m.nx))
             final_res.append(tuple(r[o:o + n].reshape(m.ny, m.nx) for r in res_all))
-        return SweepResult(stop, wall, step, checks, point_steps, final_psi, final_res, rel_d_last, last_conv)
+        return SweepResult(stop, wall, step, checks, point_steps, final_psi, final_res, rel_d_last, last_conv,
+                           wall_at_stop)

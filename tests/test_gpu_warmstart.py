@@ -1,4 +1,7 @@
-"""What the trained surrogate gains: warm-started members reach the same steady state sooner."""
+"""What the trained surrogate gains: warm-started members reach the cold start's steady state sooner.
+
+The cold (noise-seeded) run of each member is the judge: a warm member counts only where both runs
+meet the steady-state criterion and their final densities agree within 1e-3 relative L2."""
 import os
 
 import numpy as np
@@ -9,10 +12,22 @@ pytestmark = pytest.mark.gpu
 WEIGHTS = os.path.join(os.path.dirname(__file__), "..", "polarism_amd", "surrogate", "weights", "fno_steady_256.npz")
 
 
+def _density_error(a, b):
+    da, db = np.abs(a) ** 2, np.abs(b) ** 2
+    den = np.linalg.norm(db)
+    if den < 1e-30 and np.linalg.norm(da) < 1e-30:
+        return 0.0
+    return float(np.linalg.norm(da - db) / max(den, 1e-30))
+
+
 def test_warm_start_reaches_the_cold_steady_state_sooner():
-    """16 held-out members (a seed the training set never used): the warm start must reach the
-    reference steady-state criterion (rtol 1e-6 per 1 ps) no later in total simulated time than
-    the noise-seeded cold start, and land on the same density (rel-L2 <= 1e-3) where both converge."""
+    """16 held-out members (a seed the training set never used), rtol 1e-6 per 1 ps:
+    * every member whose warm and cold runs both converge lands within 1e-3 of the cold density
+      (the gate, on the maximum);
+    * every member predicted to hold no condensate (psi = 0 start) is one whose cold run decayed
+      to the zero state, where that cold run converged;
+    * on the members passing the gate, the warm runs need less simulated time in total and the
+      last of them converges sooner in wall time than the last cold one."""
     from polarism_amd.surrogate import warmstart as ws
     from polarism_amd.sweep import PackedSweep
     if not os.path.exists(WEIGHTS):
@@ -28,22 +43,20 @@ def test_warm_start_reaches_the_cold_steady_state_sooner():
         runs[name] = sw.run()
     cold, hot = runs["cold"], runs["warm"]
     both = np.isfinite(cold.stop_times) & np.isfinite(hot.stop_times)
-    print(f"converged cold {np.isfinite(cold.stop_times).sum()} warm {np.isfinite(hot.stop_times).sum()}; "
-          f"median stop cold {np.nanmedian(cold.stop_times):.0f} ps warm {np.nanmedian(hot.stop_times):.0f} ps; "
-          f"steps cold {cold.steps_run} warm {hot.steps_run}")
-    assert np.isfinite(hot.stop_times).sum() >= np.isfinite(cold.stop_times).sum()
-    assert np.nansum(np.where(both, hot.stop_times, 0)) < np.nansum(np.where(both, cold.stop_times, 0))
-    agree = []
+    err = np.full(16, np.nan)
     for i in np.flatnonzero(both):
-        dc = np.abs(cold.final_psi[i]) ** 2
-        dw = np.abs(hot.final_psi[i]) ** 2
-        nc, nw = np.linalg.norm(dc), np.linalg.norm(dw)
-        # both at the zero steady state (norm below the criterion's 1e-30): they agree
-        agree.append(0.0 if nc < 1e-30 and nw < 1e-30 else np.linalg.norm(dw - dc) / max(nc, 1e-30))
-    agree = np.asarray(agree)
-    print("density rel-L2 warm vs cold:", np.round(agree, 6))
-    assert both.sum() >= max(4, int(0.8 * np.isfinite(cold.stop_times).sum()))
-    assert np.median(agree) <= 1e-3
+        err[i] = _density_error(hot.final_psi[i], cold.final_psi[i])
+    zero = np.asarray([m.label["warm"] == "zero" for m in warm])
+    print(f"converged cold {np.isfinite(cold.stop_times).sum()} warm {np.isfinite(hot.stop_times).sum()}; "
+          f"stop cold {cold.stop_times} warm {hot.stop_times}; zero starts {np.flatnonzero(zero)}")
+    print("density rel-L2 warm vs cold:", np.round(err, 6))
+    assert both.sum() >= 6
+    assert np.nanmax(err) <= 1e-3
+    cold_zero = np.asarray([np.linalg.norm(np.abs(p) ** 2) < 1e-30 for p in cold.final_psi])
+    assert np.all(cold_zero[zero & both])
+    gated = both & (err <= 1e-3)
+    assert hot.stop_times[gated].sum() < cold.stop_times[gated].sum()
+    assert np.max(hot.wall_at_stop[gated]) < np.max(cold.wall_at_stop[gated])
 
 
 def test_device_pipeline_equals_host_pipeline():
