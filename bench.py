@@ -356,10 +356,12 @@ def fft_hbm_leg(n=2048, planes=16, reps=10):
             "frac_of_hbm_peak": round(ideal / HBM_PEAK_GBS, 4), "two_pass_GBps": round(2 * ideal, 1)}
 
 
-def lawson_leg(ticks=5):
+def lawson_leg(ticks=20):
     """Spinor Lawson IFRK, DP5 adaptive (1 ps ticks), 1024^2, lx = ly = 300 um, TE-TM 0.05,
     Zeeman 0.02, alpha_2 = -0.1 g_C, elliptic ring pump (r 15 um, width 3 um), double reservoir,
-    complex64 fields with FP64 error norms: a window of ``ticks`` ps after one warm-up tick.
+    complex64 fields with FP64 error norms: a window of ``ticks`` ps after one warm-up tick, with
+    the trial controller on the device (captured trial graphs) and on the host, advancing one
+    tick per call and all ``ticks`` in one call (which lands only on the last tick).
     rtol is 1e-5, the smallest the reference's validation accepts in single precision
     (validation.py:241-242); 1e-6 is accepted in double precision only."""
     from polarism_amd import Config, SimulationController
@@ -384,26 +386,34 @@ def lawson_leg(ticks=5):
     cfg.solver.total_time = float(ticks + 1)
     cfg.solver.precision = "single"
     cfg.noise.seed = 3
-    ctl = SimulationController(cfg)
-    sol = ctl.solver
-    sol.advance(ctl.model, ctl.state, 1)
-    sc = sol.step_controller
-    a0, r0 = sc.accepted, sc.rejected
-    sol.stream.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(ticks):
-        sol.advance(ctl.model, ctl.state, 1)
-    sol.stream.synchronize()
-    wall = time.perf_counter() - t0
-    trials = (sc.accepted - a0) + (sc.rejected - r0)
+    out = {}
     pts = 1024 * 1024 * 2
     # per trial: 7 stages, each a forward and an inverse 2-D transform (2 passes each), plus
     # the final combination's inverse (+ the error field's inverse) -> ~ (2 * 7 + 2) transforms
     transforms = 2 * 7 + 2
-    return {"ticks": ticks, "rtol": 1e-5, "trials": trials, "accepted": sc.accepted - a0, "rejected": sc.rejected - r0,
-            "ms_per_trial": round(wall * 1e3 / max(trials, 1), 3), "ps_per_s": round(ticks / wall, 3),
-            "dt_last_ps": sc.dt_last,
-            "fft_GBps_16_transforms_per_trial": round(transforms * 16.0 * pts * trials / wall / 1e9, 1)}
+    for device in (True, False):
+        for chunk in (1, ticks):
+            ctl = SimulationController(cfg)
+            sol = ctl.solver
+            sol.device_trials = device
+            sol.advance(ctl.model, ctl.state, 1)
+            sc = sol.step_controller
+            a0, r0 = sc.accepted, sc.rejected
+            sol.stream.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(ticks // chunk):
+                sol.advance(ctl.model, ctl.state, chunk)
+            sol.stream.synchronize()
+            wall = time.perf_counter() - t0
+            trials = (sc.accepted - a0) + (sc.rejected - r0)
+            name = ("device" if device else "host") + ("_controller" if chunk == 1 else f"_controller_advance_{chunk}")
+            out[name] = {
+                "ticks": ticks, "ticks_per_advance": chunk, "rtol": 1e-5, "trials": trials,
+                "accepted": sc.accepted - a0, "rejected": sc.rejected - r0,
+                "ms_per_trial": round(wall * 1e3 / max(trials, 1), 3), "ps_per_s": round(ticks / wall, 3),
+                "dt_last_ps": sc.dt_last,
+                "fft_GBps_16_transforms_per_trial": round(transforms * 16.0 * pts * trials / wall / 1e9, 1)}
+    return out
 
 
 GATE = 1e-3

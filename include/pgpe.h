@@ -57,6 +57,11 @@ int pgpe_free(void* ptr_dev);
 int pgpe_memcpy_h2d(void* dst_dev, const void* src_host, size_t nbytes, void* stream);
 int pgpe_memcpy_d2h(void* dst_host, const void* src_dev, size_t nbytes, void* stream);
 int pgpe_memcpy_d2d(void* dst_dev, const void* src_dev, size_t nbytes, void* stream);
+/* Pinned host memory (hipHostMalloc) and a device-to-host copy that returns without waiting
+ * (into pinned memory; wait on an event recorded after it). */
+int pgpe_host_alloc(void** ptr_host, size_t nbytes);
+int pgpe_host_free(void* ptr_host);
+int pgpe_memcpy_d2h_async(void* dst_host, const void* src_dev, size_t nbytes, void* stream);
 /* Fills with a small kernel (not hipMemsetAsync) so the call may sit in a captured graph. */
 int pgpe_fill_zero(void* ptr_dev, size_t nbytes, void* stream);
 int pgpe_stream_create(void** stream);
@@ -67,6 +72,7 @@ int pgpe_event_create(void** event);
 int pgpe_event_destroy(void* event);
 int pgpe_event_record(void* event, void* stream);
 int pgpe_event_elapsed_ms(void* start, void* stop, float* ms);
+int pgpe_event_sync(void* event);
 /* The stream waits for the event's work (replaces the storage writer's                    */
 /* stream.wait_event, results/visitors/storage_visitor.py:188-189).                         */
 int pgpe_stream_wait_event(void* stream, void* event);
@@ -342,9 +348,52 @@ typedef struct pgpe_lawson_args {
     double* error_sums;                      /* 2 * members doubles */
     uint64_t step;
     const uint64_t* step_base;
+    /* Device-driven adaptive trials (both NULL otherwise): the trial's h is read from
+     * *h_dev and hc, rc, he5, he3 then hold the bare tableau weights a_ij, b_j, b_j - b_hat_j
+     * (the kernels form h * weight); every kernel returns at once while *skip_dev != 0. */
+    const double* h_dev;
+    const int32_t* skip_dev;
 } pgpe_lawson_args;
 
 int pgpe_lawson(const pgpe_lawson_args* args, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* On-device control of adaptive trials: replaces the host loop of      */
+/* abstract_solver.py:86-96 with solver/step_controller.py:48-81 (PI     */
+/* update, tick landing, underflow guard) and the host-side member max  */
+/* of ifrk_fft_cuda_solver.py:293.  The state lives in device memory.   */
+/* propose: h from h_next, clamped onto target.  decide: member norms   */
+/* from the (err5, err3) sums of pgpe_lawson's final op (DP5            */
+/* sqrt(err5 / count); blend != 0: DOP853 err5 / sqrt((err5 + 0.01 err3) */
+/* count)), max over members, accept when <= 1, PI update; done when t  */
+/* reaches target.  commit: copies each src region over dst when the     */
+/* last trial was accepted.  All three do nothing once done is set.     */
+/* ------------------------------------------------------------------ */
+enum { PGPE_ADAPTIVE_OK = 0, PGPE_ADAPTIVE_UNDERFLOW = 1 };
+#define PGPE_COMMIT_MAX 8
+
+typedef struct pgpe_adaptive_state {
+    double t, target;          /* current time; the tick this advance lands on */
+    double h_next, err_prev;   /* controller memory */
+    double dt_max;             /* <= 0: no limit */
+    double k;                  /* min(order, embedded order) + 1 */
+    double h, prop, error;     /* the last trial: step, proposal, norm */
+    double dt_min, dt_max_used, dt_last;
+    double count;              /* points per member in the error norm */
+    int32_t members, blend;
+    int32_t clamped, no_growth, accepted, rejected, done, commit, status, pad0;
+} pgpe_adaptive_state;
+
+typedef struct pgpe_commit_list {
+    void* dst[PGPE_COMMIT_MAX];
+    const void* src[PGPE_COMMIT_MAX];
+    uint64_t bytes[PGPE_COMMIT_MAX];   /* multiples of 4; regions 16-byte aligned */
+    int32_t count, pad0;
+} pgpe_commit_list;
+
+int pgpe_adaptive_propose(pgpe_adaptive_state* state_dev, void* stream);
+int pgpe_adaptive_decide(pgpe_adaptive_state* state_dev, const double* error_sums_dev, void* stream);
+int pgpe_adaptive_commit(const pgpe_adaptive_state* state_dev, const pgpe_commit_list* list, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Surrogate (Fourier neural operator) kernels: BF16 MFMA GEMM with fp32 */

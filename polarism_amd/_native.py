@@ -95,7 +95,30 @@ class LawsonArgs(C.Structure):
         ("work", vp), ("aux", vp), ("f_out", vp), ("psi_real", vp), ("res0", vp * 2), ("dres_out", vp * 2),
         ("pump", vp), ("coef", vp), ("potential", vp), ("psi_out", vp), ("res_out", vp * 2), ("psik_out", vp),
         ("psi_start", vp), ("partials", vp), ("error_sums", vp), ("step", C.c_uint64), ("step_base", vp),
+        ("h_dev", vp), ("skip_dev", vp),
     ]
+
+
+COMMIT_MAX = 8
+ADAPTIVE_UNDERFLOW = 1
+
+
+class AdaptiveState(C.Structure):
+    """Mirror of ``pgpe_adaptive_state`` (the device-resident trial controller)."""
+    _fields_ = [
+        ("t", C.c_double), ("target", C.c_double), ("h_next", C.c_double), ("err_prev", C.c_double),
+        ("dt_max", C.c_double), ("k", C.c_double), ("h", C.c_double), ("prop", C.c_double),
+        ("error", C.c_double), ("dt_min", C.c_double), ("dt_max_used", C.c_double), ("dt_last", C.c_double),
+        ("count", C.c_double), ("members", C.c_int32), ("blend", C.c_int32),
+        ("clamped", C.c_int32), ("no_growth", C.c_int32), ("accepted", C.c_int32), ("rejected", C.c_int32),
+        ("done", C.c_int32), ("commit", C.c_int32), ("status", C.c_int32), ("pad0", C.c_int32),
+    ]
+
+
+class CommitList(C.Structure):
+    """Mirror of ``pgpe_commit_list``."""
+    _fields_ = [("dst", vp * COMMIT_MAX), ("src", vp * COMMIT_MAX), ("bytes", C.c_uint64 * COMMIT_MAX),
+                ("count", C.c_int32), ("pad0", C.c_int32)]
 
 
 class GemmArgs(C.Structure):
@@ -123,6 +146,9 @@ _SIGNATURES = {
     "pgpe_free": (C.c_int, [vp]),
     "pgpe_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "pgpe_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "pgpe_memcpy_d2h_async": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "pgpe_host_alloc": (C.c_int, [C.POINTER(vp), C.c_size_t]),
+    "pgpe_host_free": (C.c_int, [vp]),
     "pgpe_memcpy_d2d": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "pgpe_fill_zero": (C.c_int, [vp, C.c_size_t, vp]),
     "pgpe_stream_create": (C.c_int, [C.POINTER(vp)]),
@@ -133,6 +159,7 @@ _SIGNATURES = {
     "pgpe_event_destroy": (C.c_int, [vp]),
     "pgpe_event_record": (C.c_int, [vp, vp]),
     "pgpe_event_elapsed_ms": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
+    "pgpe_event_sync": (C.c_int, [vp]),
     "pgpe_stream_wait_event": (C.c_int, [vp, vp]),
     "pgpe_graph_begin": (C.c_int, [vp]),
     "pgpe_graph_end": (C.c_int, [vp, C.POINTER(vp)]),
@@ -149,6 +176,9 @@ _SIGNATURES = {
     "pgpe_fft_columns": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, vp]),
     "pgpe_split_step": (C.c_int, [C.POINTER(SplitArgs), vp]),
     "pgpe_lawson": (C.c_int, [C.POINTER(LawsonArgs), vp]),
+    "pgpe_adaptive_propose": (C.c_int, [vp, vp]),
+    "pgpe_adaptive_decide": (C.c_int, [vp, vp, vp]),
+    "pgpe_adaptive_commit": (C.c_int, [vp, C.POINTER(CommitList), vp]),
     "pgpe_gemm_bf16": (C.c_int, [C.POINTER(GemmArgs), vp]),
     "pgpe_f32_to_bf16": (C.c_int, [vp, vp, C.c_int64, vp]),
     "pgpe_gelu_fwd": (C.c_int, [vp, vp, vp, C.c_int64, vp]),

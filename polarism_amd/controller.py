@@ -442,8 +442,8 @@ class SimulationController:
         self._resumed = True
         self.solver = create_solver(cfg, self.grid, self.model)
         if cfg.solver.adaptive and attrs.get("step_controller") is not None:
-            from .step_controller import StepController
-            self.solver.step_controller = StepController(self.solver)
+            from .adaptive import PIControl
+            self.solver.step_controller = PIControl.for_solver(self.solver)
             self.solver.step_controller.load_state_dict(attrs["step_controller"])
 
     def _accumulator_schedules(self, dt, n_steps):

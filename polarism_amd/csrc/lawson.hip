@@ -79,7 +79,22 @@ struct LawsonParams {
     int noise;
     uint64_t step;
     const uint64_t* step_base;
+    // device-driven adaptive trials: the trial's h is read from h_dev (the h-dependent
+    // coefficients above then hold the bare tableau weights), and every kernel of the trial
+    // returns at once while *skip is nonzero (the controller has reached its target)
+    const double* h_dev;
+    const int32_t* skip;
 };
+
+// The trial's step size, and the factor that turns the coefficient arrays (hc, he, rc)
+// into h-weighted ones: 1 when the host folded h in already.  The products are the same
+// IEEE operations as the host's h * a_ij, so both modes give identical coefficients.
+template <typename T>
+__device__ __forceinline__ double trial_h(const LawsonParams<T>& p) { return p.h_dev ? *p.h_dev : p.h; }
+template <typename T>
+__device__ __forceinline__ double coef_h(const LawsonParams<T>& p) { return p.h_dev ? *p.h_dev : 1.0; }
+template <typename T>
+__device__ __forceinline__ bool trial_skipped(const LawsonParams<T>& p) { return p.skip && *p.skip; }
 
 // Linear propagator applied to one k point.  For the scalar field the angle
 // follows ifrk_fft_cuda_solver.py:108 ((-hbar/2m) * (h f)) * k^2 in T; for the
@@ -117,6 +132,8 @@ __device__ __forceinline__ void lawson_res_rhs(const pgpe_member& mb, T rho, T P
 // Sequences per column group in LDS: kind a (0 main, 1.. aux) x component c x column i.
 template <typename T, int V, int MODE>
 __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> p) {
+    if (trial_skipped(p)) return;
+    const double hh = trial_h(p), hm = coef_h(p);
     extern __shared__ unsigned char smem_raw[];
     c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
     constexpr int E = 256 * V;
@@ -163,7 +180,7 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
                 const int y = e / cnt, i = e - y * cnt;
                 c2<T> v[2], o[2];
                 for (int c = 0; c < 2; ++c) if (c < p.comps) v[c] = buf[seq(0, c, i) + pad_at(y)];
-                apply_linear<T>(mb, p.comps, p.kx[x0 + i], p.ky[y], -p.frac0 * p.h, v, o);
+                apply_linear<T>(mb, p.comps, p.kx[x0 + i], p.ky[y], -p.frac0 * hh, v, o);
                 for (int c = 0; c < 2; ++c) if (c < p.comps) buf[seq(0, c, i) + pad_at(y)] = o[c];
             }
             __syncthreads();
@@ -196,7 +213,7 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
             for (int j = 0; j < kMaxStages; ++j) {
                 if (!used[j]) continue;
                 if (p.hc[j] != 0.0) {
-                    const T w = (T)p.hc[j];
+                    const T w = (T)(p.hc[j] * hm);
                     for (int c = 0; c < 2; ++c) if (c < p.comps) {
                         sum[c].x = sum[c].x + w * vt[j][c].x;
                         sum[c].y = sum[c].y + w * vt[j][c].y;
@@ -204,7 +221,7 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
                 }
                 for (int q = 0; q < QE; ++q)
                     if (q < p.nerr && p.he[q][j] != 0.0) {
-                        const T w = (T)p.he[q][j];
+                        const T w = (T)(p.he[q][j] * hm);
                         for (int c = 0; c < 2; ++c) if (c < p.comps) {
                             er[q][c].x = er[q][c].x + w * vt[j][c].x;
                             er[q][c].y = er[q][c].y + w * vt[j][c].y;
@@ -212,11 +229,11 @@ __global__ void __launch_bounds__(256) lawson_cols_kernel(const LawsonParams<T> 
                     }
             }
             c2<T> acc[2];
-            apply_linear<T>(mb, p.comps, kxv, kyv, p.frac0 * p.h, sum, acc);
+            apply_linear<T>(mb, p.comps, kxv, kyv, p.frac0 * hh, sum, acc);
             for (int q = 0; q < QE; ++q)
                 if (q < p.nerr) {
                     c2<T> o[2] = {make_c<T>(0, 0), make_c<T>(0, 0)};
-                    apply_linear<T>(mb, p.comps, kxv, kyv, p.frac0 * p.h, er[q], o);
+                    apply_linear<T>(mb, p.comps, kxv, kyv, p.frac0 * hh, er[q], o);
                     er[q][0] = o[0];
                     er[q][1] = o[1];
                 }
@@ -262,6 +279,8 @@ __device__ __forceinline__ double wsum(double v) {
 
 template <typename T, int V, int MODE, int RES>
 __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> p, int rows_per) {
+    if (trial_skipped(p)) return;
+    const double hm = coef_h(p);
     extern __shared__ unsigned char smem_raw[];
     c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
     constexpr int NF = RES == 0 ? 0 : (RES == 1 ? 1 : 2);
@@ -344,7 +363,7 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
                 T v = r0;
 #pragma unroll
                 for (int j = 0; j < kMaxStages; ++j)
-                    if (j < p.nr && p.rc[j] != 0.0) v += (T)p.rc[j] * dv[j];
+                    if (j < p.nr && p.rc[j] != 0.0) v += (T)(p.rc[j] * hm) * dv[j];
                 if constexpr (MODE == LR_FINAL) {
                     // clamp, store the new state; reservoir error sum_j (h w_j) dn_j against
                     // atol + rtol max(|n0|, |n1|)
@@ -354,7 +373,7 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
                         T ev = 0;
 #pragma unroll
                         for (int j = 0; j < kMaxStages; ++j)
-                            if (j < p.nr && p.he[q][j] != 0.0) ev += (T)p.he[q][j] * dv[j];
+                            if (j < p.nr && p.he[q][j] != 0.0) ev += (T)(p.he[q][j] * hm) * dv[j];
                         const double sc = p.atol + p.rtol * (double)fmax(fabs(r0), fabs(v));
                         const double mag = fabs((double)ev);
                         const double s = sc > 0 ? mag / sc : (mag == 0 ? 0.0 : INFINITY);
@@ -447,7 +466,8 @@ __global__ void __launch_bounds__(256) lawson_rows_kernel(const LawsonParams<T> 
 
 // Per-member sums of the row partials, fixed order.
 __global__ void lawson_member_error(const double* __restrict__ partials, int groups_per_member, int members,
-                                    double* __restrict__ sums) {
+                                    double* __restrict__ sums, const int32_t* skip) {
+    if (skip && *skip) return;
     const int m = blockIdx.x;
     if (m >= members) return;
     double a5 = 0.0, a3 = 0.0;
@@ -599,6 +619,8 @@ static LawsonParams<T> make_params(const pgpe_lawson_args* a) {
     p.noise = a->noise;
     p.step = a->step;
     p.step_base = a->step_base;
+    p.h_dev = a->h_dev;
+    p.skip = a->skip_dev;
     return p;
 }
 
@@ -625,7 +647,7 @@ static int lawson_dispatch(const pgpe_lawson_args* a, hipStream_t s) {
             if ((rc = launch_rows<T, LR_FINAL, RES>(plan, p, s, &gpm))) return rc;
             if (a->nerr && a->error_sums) {
                 hipLaunchKernelGGL(lawson_member_error, dim3(a->members), dim3(64), 0, s, a->partials, gpm, a->members,
-                                   a->error_sums);
+                                   a->error_sums, a->skip_dev);
                 PGPE_LAUNCHED("lawson_member_error");
             }
             return PGPE_OK;

@@ -110,6 +110,26 @@ int pgpe_memcpy_d2d(void* dst, const void* src, size_t nbytes, void* stream) {
     return PGPE_OK;
 }
 
+int pgpe_host_alloc(void** ptr, size_t nbytes) {
+    if (!ptr) return fail(PGPE_ERR_ARG, "pgpe_host_alloc: null output");
+    *ptr = nullptr;
+    PGPE_HIP(hipHostMalloc(ptr, nbytes == 0 ? 64 : nbytes, hipHostMallocDefault));
+    return PGPE_OK;
+}
+
+int pgpe_host_free(void* ptr) {
+    if (!ptr) return PGPE_OK;
+    PGPE_HIP(hipHostFree(ptr));
+    return PGPE_OK;
+}
+
+int pgpe_memcpy_d2h_async(void* dst, const void* src, size_t nbytes, void* stream) {
+    if (nbytes == 0) return PGPE_OK;
+    if (!dst || !src) return fail(PGPE_ERR_ARG, "pgpe_memcpy_d2h_async: null pointer");
+    PGPE_HIP(hipMemcpyAsync(dst, src, nbytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    return PGPE_OK;
+}
+
 int pgpe_fill_zero(void* ptr, size_t nbytes, void* stream) {
     if (nbytes == 0) return PGPE_OK;
     if (!ptr) return fail(PGPE_ERR_ARG, "pgpe_fill_zero: null pointer");
@@ -180,6 +200,12 @@ int pgpe_event_elapsed_ms(void* start, void* stop, float* ms) {
     if (!start || !stop || !ms) return fail(PGPE_ERR_ARG, "pgpe_event_elapsed_ms: null argument");
     PGPE_HIP(hipEventSynchronize((hipEvent_t)stop));
     PGPE_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return PGPE_OK;
+}
+
+int pgpe_event_sync(void* event) {
+    if (!event) return fail(PGPE_ERR_ARG, "pgpe_event_sync: null event");
+    PGPE_HIP(hipEventSynchronize((hipEvent_t)event));
     return PGPE_OK;
 }
 

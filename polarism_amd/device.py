@@ -209,6 +209,9 @@ class Event:
     def record(self, stream=None) -> None:
         N.call("pgpe_event_record", self.handle, _sp(stream))
 
+    def synchronize(self) -> None:
+        N.call("pgpe_event_sync", self.handle)
+
     def elapsed_ms(self, later: "Event") -> float:
         ms = C.c_float()
         N.call("pgpe_event_elapsed_ms", self.handle, later.handle, C.byref(ms))
@@ -219,6 +222,32 @@ class Event:
         if h and N._lib is not None:
             N._lib.pgpe_event_destroy(h)
             self.handle = None
+
+
+class PinnedBuffer:
+    """Page-locked host memory (hipHostMalloc) for copies the host waits on through an event."""
+
+    def __init__(self, nbytes: int):
+        init()
+        p = C.c_void_p()
+        N.call("pgpe_host_alloc", C.byref(p), int(nbytes))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    def fetch(self, src: "DeviceArray", offset: int, stream: "Stream") -> None:
+        """Queue a copy of ``src`` into this buffer at ``offset`` bytes; returns at once."""
+        if offset + src.nbytes > self.nbytes:
+            raise ValueError("pinned buffer too small")
+        N.call("pgpe_memcpy_d2h_async", self.ptr + offset, src.ptr, src.nbytes, stream.handle)
+
+    def read(self, offset: int, nbytes: int) -> bytes:
+        return C.string_at(self.ptr + offset, nbytes)
+
+    def __del__(self):
+        ptr = getattr(self, "ptr", None)
+        if ptr and N._lib is not None:
+            N._lib.pgpe_host_free(ptr)
+            self.ptr = None
 
 
 class Graph:

@@ -171,14 +171,23 @@ class AbstractSolver(ABC):
     def step(self, model, state, dt) -> StepReport:
         raise NotImplementedError
 
+    def adaptive_control(self):
+        """The solver's PI controller (``step_controller``), created on first use."""
+        from .adaptive import PIControl
+        if getattr(self, "step_controller", None) is None:
+            self.step_controller = PIControl.for_solver(self)
+        return self.step_controller
+
+    def run_trials(self, model, state, target: float) -> None:
+        """Adaptive trials until ``state.t`` reaches ``target``; solvers with device-side
+        control override this."""
+        from .adaptive import host_trials
+        host_trials(self.adaptive_control(), self, model, state, target)
+
     def advance(self, model, state, n_steps: int) -> None:
         if self.config.solver.adaptive:
-            from .step_controller import StepController
-            if not hasattr(self, "step_controller"):
-                self.step_controller = StepController(self)
             target = state.t0 + (state.step + n_steps) * self.config.solver.dt
-            while state.t < target:
-                self.step_controller.trial(self, model, state, target)
+            self.run_trials(model, state, target)
             state.step += n_steps
             state.t = state.t0 + state.step * self.config.solver.dt
             return

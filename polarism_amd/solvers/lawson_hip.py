@@ -15,11 +15,13 @@ only 16 bytes per member return to the host per trial.
 from __future__ import annotations
 
 import ctypes as C
+import math
+from collections import deque
 
 import numpy as np
 
 from .. import _native as N
-from ..device import DeviceArray
+from ..device import DeviceArray, Event, PinnedBuffer
 from ..solver_api import ACCEPTED, SolverCapabilities, StepReport, register_solver
 from ..tableaus import TABLEAUS
 from .base import HipSolverBase
@@ -50,10 +52,14 @@ class IFRKFFTHipSolver(HipSolverBase):
         self.lap = 1 if self.batch_physics.any_nonzero("kinetic_relaxation_eta") else 0
         self._bufs = None
         self._pot_key = None
+        self._pot_obj = None
         self._pot_dev = None
         self._args = N.LawsonArgs()
         self._k_valid = False
         self._coef_key = None   # (buffer, values) of the last coefficient upload
+        self._ctl_dev = None    # device pgpe_adaptive_state of the device-controlled trials
+        self._ctl_host = None   # pinned copies of it, one per queued trial
+        self._ctl_events = None
 
     # -- buffers -------------------------------------------------------------
     def _ensure(self, model, batch):
@@ -88,11 +94,14 @@ class IFRKFFTHipSolver(HipSolverBase):
                 raise ValueError("a constant nonzero potential must be given as an array")
             return None
         pot = np.asarray(pot)
-        key = (id(model.potential), pot.shape, pot.dtype.str)
-        if self._pot_key != key:
+        key = (pot.shape, pot.dtype.str)
+        # the cache holds the host object itself (compared by identity, as the reference does,
+        # ifrk_cuda_fused.py:643-654), so a new array can never reuse a stale device copy
+        if self._pot_key != key or self._pot_obj is not model.potential:
             arr = np.broadcast_to(pot, self.grid.shape).astype(self.cdtype)
             self._pot_dev = DeviceArray.from_numpy(arr)
             self._pot_key = key
+            self._pot_obj = model.potential
         return self._pot_dev.ptr
 
     def _adopt(self, model, state):
@@ -118,25 +127,29 @@ class IFRKFFTHipSolver(HipSolverBase):
         model.reservoir.assign_device(tuple(x.reshape(model.reservoir.shape) for x in b["res"]))
 
     # -- one trial ------------------------------------------------------------
-    def _trial(self, model, state, h, adaptive, noise, step):
-        tab = self._ensure(model, state.batch_size)
+    def _upload_coefficients(self, model, t, h, batch):
+        """Pump coefficients of the stage times t + c_i h.  The upload synchronises the stream,
+        so it is skipped when the values are already there (time-independent pumps) and fixed
+        steps queue back to back."""
+        b = self._bufs
+        coefs = self.pump_coefficients(model, [t + float(ci) * h for ci in self._c], batch).astype(self.rdtype)
+        key = (b["coef"].ptr, coefs.tobytes())
+        if key != self._coef_key:
+            b["coef"].upload(coefs.reshape(-1), self.stream)
+            self._coef_key = key
+
+    def _queue_trial(self, model, tab, h, adaptive, noise, step, ctl_ptr=None):
+        """Queue the stage and final launches of one trial on the solver stream.
+
+        With ``ctl_ptr`` (a device ``pgpe_adaptive_state``) the kernels read h from it and skip
+        once it is done; the coefficients passed are then the bare tableau weights."""
         b = self._bufs
         batch = tab.members
         stages = self.tableau.stages
         A, bw, c = self._A, self._b, self._c
         pot = self._potential_ptr(model)
-        times = [state.t + float(ci) * h for ci in c]
-        coefs = self.pump_coefficients(model, times, batch).astype(self.rdtype)
-        # the upload synchronises the stream: skip it when the values are already there
-        # (time-independent pumps), so fixed steps queue back to back
-        key = (b["coef"].ptr, coefs.tobytes())
-        if key != self._coef_key:
-            b["coef"].upload(coefs.reshape(-1), self.stream)
-            self._coef_key = key
         item = np.dtype(self.rdtype).itemsize
-        if not self._k_valid:
-            N.call("pgpe_fft2", self.plan.handle, b["psi"].ptr, b["psik"].ptr, batch * self.components, 0,
-                   self.stream.handle)
+        hw = 1.0 if ctl_ptr is not None else float(h)     # weight scale folded in on the host
         a = self._args
         a.precision = self.prec_code
         a.reservoir = N.RESERVOIR_KIND[self.reservoir_kind]
@@ -155,6 +168,8 @@ class IFRKFFTHipSolver(HipSolverBase):
         a.psi_real = b["psi"].ptr
         a.pump = self._pump_dev.ptr
         a.potential = pot
+        a.h_dev = None if ctl_ptr is None else ctl_ptr + N.AdaptiveState.h.offset
+        a.skip_dev = None if ctl_ptr is None else ctl_ptr + N.AdaptiveState.done.offset
         for f in range(self.nres):
             a.res0[f] = b["res"][f].ptr
         for j in range(stages):
@@ -170,9 +185,9 @@ class IFRKFFTHipSolver(HipSolverBase):
             a.frac0 = ci
             a.nterms = i
             for j in range(i):
-                a.hc[j] = h * float(A[i, j])
+                a.hc[j] = hw * float(A[i, j])
                 a.fr[j] = ci - float(c[j])
-                a.rc[j] = h * float(A[i, j])
+                a.rc[j] = hw * float(A[i, j])
             a.stage_real = 1 if (i == 0 and not self.lap and self._k_valid is False) else 0
             a.coef = b["coef"].ptr + i * batch * item
             a.f_out = b["f"][i].ptr
@@ -184,11 +199,11 @@ class IFRKFFTHipSolver(HipSolverBase):
         a.frac0 = 1.0
         a.nterms = stages
         for j in range(stages):
-            a.hc[j] = h * float(bw[j])
+            a.hc[j] = hw * float(bw[j])
             a.fr[j] = 1.0 - float(c[j])
-            a.rc[j] = h * float(bw[j])
-            a.he5[j] = h * float(self._w5[j]) if (adaptive and self._w5 is not None) else 0.0
-            a.he3[j] = h * float(self._w3[j]) if (adaptive and self._w3 is not None) else 0.0
+            a.rc[j] = hw * float(bw[j])
+            a.he5[j] = hw * float(self._w5[j]) if (adaptive and self._w5 is not None) else 0.0
+            a.he3[j] = hw * float(self._w3[j]) if (adaptive and self._w3 is not None) else 0.0
         a.nerr = 0 if not adaptive else (2 if self._w3 is not None else 1)
         a.coef = None
         a.noise = 1 if noise else 0
@@ -211,12 +226,26 @@ class IFRKFFTHipSolver(HipSolverBase):
             a.partials = None
             a.error_sums = None
         N.check(lib.pgpe_lawson(C.byref(a), self.stream.handle), "pgpe_lawson final")
+
+    def _error_count(self):
+        """Points per member in the embedded error norm: field and reservoir values."""
+        return self.components * self.grid.nx * self.grid.ny + self.nres * self.res_components * \
+            self.grid.nx * self.grid.ny
+
+    def _trial(self, model, state, h, adaptive, noise, step):
+        tab = self._ensure(model, state.batch_size)
+        b = self._bufs
+        batch = tab.members
+        self._upload_coefficients(model, state.t, h, batch)
+        if not self._k_valid:
+            N.call("pgpe_fft2", self.plan.handle, b["psi"].ptr, b["psik"].ptr, batch * self.components, 0,
+                   self.stream.handle)
+        self._queue_trial(model, tab, h, adaptive, noise, step)
         if not adaptive:
             self._k_valid = not noise
             return None
         sums = b["sums"].get(self.stream).reshape(batch, 2)
-        count = self.components * self.grid.nx * self.grid.ny + self.nres * self.res_components * \
-            self.grid.nx * self.grid.ny
+        count = self._error_count()
         err5, err3 = sums[:, 0], sums[:, 1]
         if self._w3 is not None:
             den = err5 + 0.01 * err3
@@ -225,6 +254,94 @@ class IFRKFFTHipSolver(HipSolverBase):
         else:
             member = np.sqrt(err5 / count)
         return float(np.max(member))
+
+    # -- adaptive trials controlled on the device ----------------------------------------
+    TRIALS_PER_GRAPH = 1
+    device_trials = True      # False: decide every trial on the host (the reference's loop)
+
+    def run_trials(self, model, state, target):
+        """Trials until ``state.t`` reaches ``target``.  With a time-independent pump the whole
+        loop runs on the device: a captured graph of ``TRIALS_PER_GRAPH`` trials, each bracketed
+        by the controller kernels (propose, decide, commit), is replayed until the controller
+        reports the target reached; the host reads the 144-byte state once per replay.  Pumps
+        whose coefficient depends on the trial time take the host loop."""
+        if not state.t < target:
+            return
+        if not (self.device_trials and self.pump_time_independent(model)):
+            return super().run_trials(model, state, target)
+        ctl = self.adaptive_control()
+        batch = state.batch_size
+        self.batch_physics.check(state._canonical_shape, state.components)
+        tab = self._ensure(model, batch)
+        self._adopt(model, state)
+        b = self._bufs
+        self._upload_coefficients(model, state.t, 0.0, batch)
+        if not self._k_valid:
+            N.call("pgpe_fft2", self.plan.handle, b["psi"].ptr, b["psik"].ptr, batch * self.components, 0,
+                   self.stream.handle)
+            self._k_valid = True
+        if self._ctl_dev is None:
+            self._ctl_dev = DeviceArray.empty((C.sizeof(N.AdaptiveState),), np.uint8)
+        st = ctl.to_device(state.t, target, batch, self._error_count(), self._w3 is not None)
+        self._ctl_dev.upload(np.frombuffer(bytes(st), dtype=np.uint8), self.stream)
+        ptr = self._ctl_dev.ptr
+        commit = N.CommitList()
+        regions = [(b["psi"], b["psi_new"]), (b["psik"], b["psik_new"])]
+        regions += [(b["res"][f], b["res_new"][f]) for f in range(self.nres)]
+        for r, (dst, src) in enumerate(regions):
+            commit.dst[r], commit.src[r], commit.bytes[r] = dst.ptr, src.ptr, dst.nbytes
+        commit.count = len(regions)
+        pot = self._potential_ptr(model)
+        key = ("trials", self.TRIALS_PER_GRAPH, ptr, b["psi"].ptr, b["coef"].ptr, pot)
+
+        def body():
+            for _ in range(self.TRIALS_PER_GRAPH):
+                N.call("pgpe_adaptive_propose", ptr, self.stream.handle)
+                self._queue_trial(model, tab, 0.0, True, False, state.step, ctl_ptr=ptr)
+                N.call("pgpe_adaptive_decide", ptr, b["sums"].ptr, self.stream.handle)
+                N.check(N.lib().pgpe_adaptive_commit(ptr, C.byref(commit), self.stream.handle),
+                        "pgpe_adaptive_commit")
+
+        graph = self.capture(key, body)
+        st = self._replay_trials(graph, st)
+        ctl.from_device(st)
+        state.t = st.t
+        self._publish(model, state)
+
+    SPECULATION = 2   # trial graphs kept queued ahead of the controller state the host has read
+
+    def _replay_trials(self, graph, st):
+        """Replay the one-trial graph until the device controller is done.  The host keeps up to
+        ``SPECULATION`` trials queued (as many as the controller's last state says are still
+        needed), each followed by an asynchronous copy of the state into pinned memory, so the
+        GPU does not wait for the host between trials; a trial queued past the target finds
+        the state done and its kernels return at once."""
+        size = C.sizeof(N.AdaptiveState)
+        if self._ctl_host is None:
+            self._ctl_host = PinnedBuffer(self.SPECULATION * size)
+            self._ctl_events = [Event() for _ in range(self.SPECULATION)]
+
+        def expected(x):
+            if x.done or x.status:
+                return 0
+            return max(1, math.ceil((x.target - x.t) / x.h_next))
+
+        pending = deque()
+        slot = 0
+        want = expected(st)
+        while want:
+            while len(pending) < min(self.SPECULATION, want):
+                graph.launch(self.stream)
+                self._ctl_host.fetch(self._ctl_dev, slot * size, self.stream)
+                self._ctl_events[slot].record(self.stream)
+                pending.append(slot)
+                slot = (slot + 1) % self.SPECULATION
+            k = pending.popleft()
+            self._ctl_events[k].synchronize()
+            st = N.AdaptiveState.from_buffer_copy(self._ctl_host.read(k * size, size))
+            want = expected(st)
+        self.stream.synchronize()
+        return st
 
     # -- solver contract ---------------------------------------------------------
     def step(self, model, state, dt):

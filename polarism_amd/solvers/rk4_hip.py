@@ -42,6 +42,7 @@ class RK4FDMHipSolver(HipSolverBase):
         self._const_coef = None
         self._coef_chunk = None
         self._pot_key = None
+        self._pot_obj = None
         self._pot_dev = None
         # rk4_cuda_solver.py:2280-2294: the two stencils, and the 9-point one needs dx == dy
         self.laplacian = getattr(config.solver, "laplacian", "five-point")
@@ -75,8 +76,10 @@ class RK4FDMHipSolver(HipSolverBase):
         if self.components == 2:
             raise SolverCompatibilityError("rk4-fdm-hip takes no potential for spinor fields")
         pot = np.asarray(pot)
-        key = (id(model.potential), pot.shape, pot.dtype.str, batch)
-        if self._pot_key != key:
+        key = (pot.shape, pot.dtype.str, batch)
+        # the cache holds the host object itself (compared by identity, as the reference does,
+        # ifrk_cuda_fused.py:643-654), so a new array can never reuse a stale device copy
+        if self._pot_key != key or self._pot_obj is not model.potential:
             ny, nx = self.grid.shape
             arr = pot.reshape(-1, ny, nx) if pot.size % (ny * nx) == 0 else np.broadcast_to(pot, (ny, nx))[None]
             if arr.shape[0] not in (1, batch):
@@ -84,6 +87,7 @@ class RK4FDMHipSolver(HipSolverBase):
             self._pot_dev = DeviceArray.from_numpy(np.ascontiguousarray(arr, dtype=self.cdtype).reshape(-1))
             self._pot_shared = arr.shape[0] == 1
             self._pot_key = key
+            self._pot_obj = model.potential
         return self._pot_dev
 
     def _ensure_engine(self, model, batch):

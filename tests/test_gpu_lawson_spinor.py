@@ -209,3 +209,36 @@ def test_error_norm_of_two_member_batch_matches_closed_form(tableau):
     print(f"{tableau} two-member error: HIP {report.error:.12e} closed form {member.max():.12e}", flush=True)
     assert abs(report.error - member.max()) <= 1e-7 * member.max()
     assert report.accepted == (member.max() <= 1)
+
+
+# ---------------------------------------------------------------- device-side trial control
+@pytest.mark.parametrize("tableau", ["dp5", "dop853"])
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_device_controlled_trials_equal_host_controlled_trials(tableau, precision):
+    """The same adaptive run with the PI controller on the device (captured trial graphs) and on
+    the host (a read-back per trial): the same accept / reject sequence, time and statistics, and
+    states equal to rounding.  The two differ only in rounding: the controller's pow may differ in
+    the last bit between libraries, and the host path's first stage reads the real-space field
+    where the device path transforms it first.  In single precision the embedded error is a
+    difference of FP32 stage values, so such rounding moves the error, and with it the next h,
+    by ~1e-4 relative."""
+    runs = []
+    for device in (True, False):
+        cfg = spinor_ring_cfg(n=64, lx=80.0, precision=precision, rtol=1e-5 if precision == "single" else 1e-7,
+                              dt=10.0, ticks=3)
+        cfg.solver.tableau = tableau
+        ctl = _ctl(cfg)
+        ctl.solver.device_trials = device
+        sc = _run_ticks(ctl, 3)
+        runs.append((ctl, dict(sc.statistics), sc.h_next, ctl.state.field.get(),
+                     [r.get() for r in ctl.reservoir.get_state()]))
+    (dev, s_dev, h_dev, psi_dev, res_dev), (host, s_host, h_host, psi_host, res_host) = runs
+    print(f"{tableau} {precision}: device {s_dev} host {s_host}", flush=True)
+    assert s_dev["rejected"] > 0
+    assert (s_dev["accepted"], s_dev["rejected"]) == (s_host["accepted"], s_host["rejected"])
+    assert dev.state.t == host.state.t == 30.0 and dev.state.step == host.state.step == 3
+    assert abs(h_dev - h_host) <= (1e-9 if precision == "double" else 1e-3) * h_host
+    tol = 1e-12 if precision == "double" else 1e-5
+    assert O.rel_l2(psi_dev, psi_host) <= tol
+    for a, b in zip(res_dev, res_host):
+        assert O.rel_l2(a, b) <= tol

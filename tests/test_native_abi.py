@@ -46,7 +46,8 @@ def _c_layout(struct, fields):
 
 @pytest.mark.parametrize("py,cname", [(N.Member, "pgpe_member"), (N.Rk4Args, "pgpe_rk4_args"),
                                      (N.SplitArgs, "pgpe_split_args"), (N.LawsonArgs, "pgpe_lawson_args"),
-                                     (N.GemmArgs, "pgpe_gemm_args")])
+                                     (N.GemmArgs, "pgpe_gemm_args"), (N.AdaptiveState, "pgpe_adaptive_state"),
+                                     (N.CommitList, "pgpe_commit_list")])
 def test_ctypes_struct_layout_matches_c(py, cname):
     names = [f[0] for f in py._fields_]
     got = _c_layout(cname, names)

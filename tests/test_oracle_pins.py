@@ -728,3 +728,31 @@ def test_lawson_noise_is_the_wigner_increment_on_the_final_state(precision):
     sigma = np.sqrt(p.gamma_C + p.R * res_plain[0].astype(np.float64)) * np.sqrt(0.01 / (4 * g.dx * g.dy))
     z = (inc / sigma)[0, 0]
     assert abs(np.mean(z.real)) < 0.3 and 0.6 < np.var(z.real) < 1.5 and 0.6 < np.var(z.imag) < 1.5
+
+
+# ---------------------------------------------------------------- the package's PI controller
+def test_package_controller_follows_the_oracle_controller():
+    """polarism_amd.adaptive.PIControl (host half of the device controller) and the oracle's
+    StepController give the same proposals for the same error sequence, rejections,
+    clamped landings and a dt_max included."""
+    import random
+    from polarism_amd.adaptive import PIControl
+    rnd = random.Random(3)
+    for dt_max in (None, 0.7):
+        pkg = PIControl(k=5.0, h_next=min(1.0, dt_max or 1.0), dt_max=dt_max)
+        ora = O.StepController(1.0, 5, 4, dt_max=dt_max)
+        t, target = 0.0, 3.3
+        for _ in range(60):
+            if t >= target:
+                target += 1.7
+            h1, c1, p1 = pkg.propose(t, target)
+            h2, c2, p2 = ora.propose(t, target)
+            assert (h1, c1, p1) == (h2, c2, p2)
+            err = rnd.choice([0.0, 1e-6, 0.3, 0.99, 1.0, 1.01, 4.0, 80.0, float("nan")])
+            ok = err <= 1
+            pkg.settle(h1, c1, p1, ok, err)
+            ora.update(ok, err, h2, c2, p2)
+            assert pkg.h_next == ora.h_next and pkg.no_growth == ora.no_growth
+            if ok:
+                t = target if c1 else t + h1
+        assert (pkg.accepted, pkg.rejected) == (ora.accepted, ora.rejected)
