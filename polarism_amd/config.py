@@ -7,8 +7,8 @@ HIP methods registered by this package.
 """
 from __future__ import annotations
 
-from dataclasses import asdict, dataclass, field, fields
-from difflib import get_close_matches
+from dataclasses import asdict, dataclass, field, fields, replace
+from difflib import SequenceMatcher
 from pathlib import Path
 from typing import Any
 
@@ -19,27 +19,43 @@ class ConfigValidationError(ValueError):
     """A configuration value is invalid (config/validation.py)."""
 
 
-def unknown_field_message(owner: str, name: str, known) -> str:
-    known = list(known)
-    matches = [k for k in known if k.lower() == name.lower()]
-    matches += [k for k in get_close_matches(name, known, n=3, cutoff=0.6) if k not in matches]
-    hint = f"; did you mean {', '.join(repr(m) for m in matches)}?" if matches else ""
-    key = name if owner == "Config" else f"{owner}.{name}"
-    return f"Unknown config key '{key}'{hint}"
+def suggest_keys(name: str, known, limit: int = 3) -> list[str]:
+    """Known keys a mistyped ``name`` most likely meant: a case-insensitive match first, then
+    the closest spellings (similarity ratio >= 0.6), best first."""
+    exact = sorted(k for k in known if k.lower() == name.lower())
+    scored = sorted(((SequenceMatcher(None, name, k).ratio(), k) for k in known if k not in exact),
+                    key=lambda rk: (-rk[0], rk[1]))
+    return (exact + [k for r, k in scored if r >= 0.6])[:limit]
 
 
-class StrictFields:
-    """Reject attributes that are not declared fields (simulation_parameters.py:19-27)."""
-
-    def __setattr__(self, name, value):
-        if name not in self.__dataclass_fields__:
-            owner = _SECTION_NAMES.get(type(self), type(self).__name__)
-            raise AttributeError(unknown_field_message(owner, name, self.__dataclass_fields__))
-        object.__setattr__(self, name, value)
+def unknown_key_error(path: str, name: str, known) -> str:
+    """Message for a key ``name`` that section ``path`` (empty for the top level) does not declare."""
+    full = f"{path}.{name}" if path else name
+    hints = suggest_keys(name, known)
+    tail = f"; did you mean {', '.join(repr(h) for h in hints)}?" if hints else ""
+    return f"Unknown config key '{full}'{tail}"
 
 
-@dataclass
-class GridParameters(StrictFields):
+def config_section(name: str):
+    """Declare a dataclass config section whose instances accept only their declared fields, so
+    that a misspelt attribute assignment fails instead of being silently ignored."""
+    def make(cls):
+        cls = dataclass(cls)
+        cls._section_path = name
+        cls._declared = frozenset(f.name for f in fields(cls))
+
+        def __setattr__(self, key, value):
+            if key not in type(self)._declared:
+                raise AttributeError(unknown_key_error(type(self)._section_path, key, type(self)._declared))
+            object.__setattr__(self, key, value)
+
+        cls.__setattr__ = __setattr__
+        return cls
+    return make
+
+
+@config_section("grid")
+class GridParameters:
     nx: int = 256
     ny: int = 256
     lx: float = 100.0
@@ -49,8 +65,8 @@ class GridParameters(StrictFields):
     extra: dict[str, Any] = field(default_factory=dict)
 
 
-@dataclass
-class PhysicsConstants(StrictFields):
+@config_section("physics")
+class PhysicsConstants:
     hbar: float = 0.6582119514
     m_eff: float = 0.284
     gamma_R: float = 0.005
@@ -75,8 +91,8 @@ class PhysicsConstants(StrictFields):
     te_tm: float = 0.0
 
 
-@dataclass
-class NoiseParameters(StrictFields):
+@config_section("noise")
+class NoiseParameters:
     enabled: bool = True
     seed: int | None = None
     initial: str = "complex_gaussian_zero_mean"
@@ -86,8 +102,8 @@ class NoiseParameters(StrictFields):
     stochastic_strength: float = 1.0
 
 
-@dataclass
-class InitialConditionParameters(StrictFields):
+@config_section("initial_condition")
+class InitialConditionParameters:
     type: str = "zero"
     amplitude: float = 1.0
     x0: float = 0.0
@@ -101,16 +117,16 @@ class InitialConditionParameters(StrictFields):
     extra: dict[str, Any] = field(default_factory=dict)
 
 
-@dataclass
-class BoundaryConditionParameters(StrictFields):
+@config_section("boundary_condition")
+class BoundaryConditionParameters:
     profile_type: str = "sin2"
     strength: float = 1.0
     absorption: str = "no-absorption"
     mask_width_percent: float = 0.2
 
 
-@dataclass
-class PotentialParameters(StrictFields):
+@config_section("potential")
+class PotentialParameters:
     expose_results: bool = True
     potential_type: str = "zero"
     profile: dict[str, Any] = field(default_factory=dict)
@@ -126,8 +142,8 @@ class PotentialParameters(StrictFields):
     order: float = 2.0
 
 
-@dataclass
-class LaserParameters(StrictFields):
+@config_section("laser")
+class LaserParameters:
     expose_results: bool = True
     mode: str = "single"
     config_file: str | None = None
@@ -150,8 +166,8 @@ class LaserParameters(StrictFields):
     extra: dict[str, Any] = field(default_factory=dict)
 
 
-@dataclass
-class DriveParameters(StrictFields):
+@config_section("drive")
+class DriveParameters:
     enabled: bool = False
     amplitude: float = 0.0
     energy: float = 0.0
@@ -162,16 +178,16 @@ class DriveParameters(StrictFields):
     envelope: dict[str, Any] = field(default_factory=lambda: {"type": "constant"})
 
 
-@dataclass
-class ReservoirParameters(StrictFields):
+@config_section("reservoir")
+class ReservoirParameters:
     expose_results: bool = True
     reservoir_type: str = "double"
     components: int | None = None
     extra: dict[str, Any] = field(default_factory=dict)
 
 
-@dataclass
-class SolverParameters(StrictFields):
+@config_section("solver")
+class SolverParameters:
     total_time: float = 5.0
     dt: float = 1e-3
     method: str = "rk4-fdm-hip"
@@ -185,8 +201,8 @@ class SolverParameters(StrictFields):
     dt_max: float | None = None
 
 
-@dataclass
-class ResultParameters(StrictFields):
+@config_section("result")
+class ResultParameters:
     real_time_view: bool = False
     view_member: int = 0
     real_time_refresh_interval: float = 0.1
@@ -212,37 +228,37 @@ class ResultParameters(StrictFields):
     animation_output: str | None = None
 
 
-@dataclass
-class ComputeEngineParameters(StrictFields):
+@config_section("compute_engine")
+class ComputeEngineParameters:
     use_gpu: bool = True
     gpu_device: int = 0
 
 
-@dataclass
-class CheckpointParameters(StrictFields):
+@config_section("checkpoint")
+class CheckpointParameters:
     every: float = 0.0
     path: str | None = None
     resume_from: str | None = None
     on_signal: bool = True
 
 
-@dataclass
-class StopParameters(StrictFields):
+@config_section("stop")
+class StopParameters:
     steady_state: bool = False
     rtol: float = 1e-6
     check_every: float = 10.0
     min_time: float = 0.0
 
 
-@dataclass
-class SweepParameters(StrictFields):
+@config_section("sweep")
+class SweepParameters:
     batch_size: int | None = None
     memory_fraction: float = 0.8
     max_batch: int | None = None
 
 
-@dataclass
-class Config(StrictFields):
+@config_section("")
+class Config:
     grid: GridParameters = field(default_factory=GridParameters)
     boundary_condition: BoundaryConditionParameters = field(default_factory=BoundaryConditionParameters)
     potential: PotentialParameters = field(default_factory=PotentialParameters)
@@ -262,25 +278,28 @@ class Config(StrictFields):
 
     @classmethod
     def from_dict(cls, data: dict) -> "Config":
-        cfg = cls()
-        sections = {f.name for f in fields(cls)}
+        """Build a configuration from a nested mapping (section -> {key: value}).  Unknown
+        sections or keys raise ValueError; list values of tuple-typed keys become tuples;
+        ``terms`` is taken as a free mapping.  Values are not validated here."""
+        defaults = cls()
+        updates = {}
         for section, values in data.items():
-            if section not in sections:
-                raise ValueError(unknown_field_message("Config", section, sections))
+            if section not in cls._declared:
+                raise ValueError(unknown_key_error("", section, cls._declared))
             if not isinstance(values, dict):
                 raise ValueError(f"Config section '{section}' must be a mapping")
             if section == "terms":
-                cfg.terms = values
+                updates["terms"] = dict(values)
                 continue
-            target = getattr(cfg, section)
-            known = {f.name for f in fields(target)}
-            for name, value in values.items():
-                if name not in known:
-                    raise ValueError(unknown_field_message(section, name, known))
-                if isinstance(getattr(target, name), tuple) and isinstance(value, list):
-                    value = tuple(value)
-                setattr(target, name, value)
-        return cfg
+            current = getattr(defaults, section)
+            declared = type(current)._declared
+            stray = [k for k in values if k not in declared]
+            if stray:
+                raise ValueError(unknown_key_error(section, stray[0], declared))
+            coerced = {k: tuple(v) if isinstance(getattr(current, k), tuple) and isinstance(v, list) else v
+                       for k, v in values.items()}
+            updates[section] = replace(current, **coerced)
+        return replace(defaults, **updates)
 
     @classmethod
     def from_yaml(cls, path) -> "Config":
@@ -300,8 +319,5 @@ class Config(StrictFields):
             yaml.safe_dump(asdict(self), handle, sort_keys=False)
         return out
 
-
-_SECTION_NAMES = {f.default_factory: f.name for f in fields(Config)
-                  if isinstance(f.default_factory, type) and issubclass(f.default_factory, StrictFields)}
 
 FLOAT_PHYSICS_FIELDS = frozenset(f.name for f in fields(PhysicsConstants) if f.type in (float, "float"))

@@ -9,6 +9,7 @@ of the reference's ``compute_engine.to_cpu`` for CuPy arrays
 from __future__ import annotations
 
 import ctypes as C
+import gc
 import threading
 
 import numpy as np
@@ -45,6 +46,19 @@ def available() -> bool:
     return lib.pgpe_device_count(C.byref(count)) == 0 and count.value > 0
 
 
+# Graph capture in progress on this thread: hipFree is not allowed inside a capture (it would
+# invalidate it), so buffers released meanwhile are freed when the capture ends.
+_capture_depth = 0
+_deferred_frees: list[int] = []
+
+
+def _release(ptr: int) -> None:
+    if _capture_depth:
+        _deferred_frees.append(ptr)
+    elif N._lib is not None:
+        N._lib.pgpe_free(ptr)
+
+
 class DeviceBuffer:
     """Owning device allocation (freed when garbage collected)."""
 
@@ -59,8 +73,8 @@ class DeviceBuffer:
 
     def __del__(self):
         ptr = getattr(self, "ptr", None)
-        if ptr and N._lib is not None:
-            N._lib.pgpe_free(ptr)
+        if ptr:
+            _release(ptr)
             self.ptr = None
 
 
@@ -258,18 +272,32 @@ class Graph:
 
     @staticmethod
     def capture(stream: Stream, body) -> "Graph":
-        N.call("pgpe_graph_begin", stream.handle)
+        """Capture the launches ``body`` queues on ``stream``.  Python's cyclic collector is held
+        off and buffers released meanwhile are freed after the capture: a hipFree (or a graph
+        destruction) inside it would invalidate the capture."""
+        global _capture_depth
+        collecting = gc.isenabled()
+        gc.disable()
+        _capture_depth += 1
         try:
-            body()
-        except BaseException:
+            N.call("pgpe_graph_begin", stream.handle)
+            try:
+                body()
+            except BaseException:
+                out = C.c_void_p()
+                N.lib().pgpe_graph_end(stream.handle, C.byref(out))
+                if out.value:
+                    N.lib().pgpe_graph_destroy(out.value)
+                raise
             out = C.c_void_p()
-            N.lib().pgpe_graph_end(stream.handle, C.byref(out))
-            if out.value:
-                N.lib().pgpe_graph_destroy(out.value)
-            raise
-        out = C.c_void_p()
-        N.call("pgpe_graph_end", stream.handle, C.byref(out))
-        return Graph(out.value)
+            N.call("pgpe_graph_end", stream.handle, C.byref(out))
+            return Graph(out.value)
+        finally:
+            _capture_depth -= 1
+            if collecting:
+                gc.enable()
+            while not _capture_depth and _deferred_frees:
+                _release(_deferred_frees.pop())
 
     def launch(self, stream: Stream) -> None:
         N.call("pgpe_graph_launch", self.handle, stream.handle)

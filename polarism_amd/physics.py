@@ -11,24 +11,29 @@ import numpy as np
 from .config import FLOAT_PHYSICS_FIELDS
 
 
+def _member_column(name: str, value) -> np.ndarray:
+    """One float64 value per member for physics constant ``name``, or ValueError."""
+    if name not in FLOAT_PHYSICS_FIELDS:
+        raise ValueError(f"Unknown or non-float batch_physics field: {name!r}")
+    col = np.asarray(value, dtype=np.float64)
+    if col.ndim != 1 or col.size == 0:
+        raise ValueError(f"batch_physics.{name} must be a nonempty 1-D array")
+    if not np.isfinite(col).all():
+        raise ValueError(f"batch_physics.{name} must contain finite values")
+    return col
+
+
 class BatchPhysics:
     def __init__(self, physics, values=None, precision="double"):
         self.physics = physics
-        self.host: dict[str, np.ndarray] = {}
-        self.size = None
-        for name, value in (values or {}).items():
-            if name not in FLOAT_PHYSICS_FIELDS:
-                raise ValueError(f"Unknown or non-float batch_physics field: {name!r}")
-            arr = np.asarray(value, dtype=np.float64)
-            if arr.ndim != 1 or arr.size == 0:
-                raise ValueError(f"batch_physics.{name} must be a nonempty 1-D array")
-            if self.size is not None and arr.size != self.size:
-                raise ValueError(f"batch_physics.{name} has length {arr.size}; expected {self.size}")
-            if not np.all(np.isfinite(arr)):
-                raise ValueError(f"batch_physics.{name} must contain finite values")
-            self.size = arr.size
-            self.host[name] = arr
         self.precision = precision
+        self.host: dict[str, np.ndarray] = {k: _member_column(k, v) for k, v in (values or {}).items()}
+        lengths = {k: c.size for k, c in self.host.items()}
+        if len(set(lengths.values())) > 1:
+            first = next(iter(lengths.values()))
+            odd = next(k for k, n in lengths.items() if n != first)
+            raise ValueError(f"batch_physics.{odd} has length {lengths[odd]}; expected {first}")
+        self.size = next(iter(lengths.values()), None)
 
     def get(self, name):
         """Return the per-member column if the value varies, else the scalar."""
