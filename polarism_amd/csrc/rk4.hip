@@ -50,6 +50,7 @@ struct Rk4Params {
     uint64_t step;
     const uint64_t* step_base;
     const c2<T>* potential;   // complex V packed per member at pot_off, or null
+    const int32_t* active_tiles;  // device count of tile_map entries to run, or null (num_tiles)
 };
 
 // Reservoir right-hand side with the condensate frozen.  Rate sums are
@@ -217,7 +218,7 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
     constexpr int W = kTileW + 2;
     __shared__ c2<T> s_psi[(TH + 2) * W];
 
-    const int tile = xcd_tile(blockIdx.x, p.num_tiles);
+    const int tile = xcd_tile(blockIdx.x, p.active_tiles ? *p.active_tiles : p.num_tiles);
     if (tile < 0) return;
     const int m = p.tile_map[2 * tile];
     // a copy of the member row: its fields stay in registers past the global stores
@@ -369,7 +370,7 @@ __global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T
     constexpr int PL = (TH + 2) * W;
     __shared__ c2<T> s_psi[2 * PL];
 
-    const int tile = xcd_tile(blockIdx.x, p.num_tiles);
+    const int tile = xcd_tile(blockIdx.x, p.active_tiles ? *p.active_tiles : p.num_tiles);
     if (tile < 0) return;
     const int m = p.tile_map[2 * tile];
     // a copy of the member row: its fields stay in registers past the global stores
@@ -550,6 +551,7 @@ struct Rk4StepParams {
     const T* coef;       // 4 x members: pump coefficient of stage s of member m at coef[s * members + m]
     uint64_t step;
     const uint64_t* step_base;
+    const int32_t* active_tiles;  // device count of tile_map entries to run, or null (num_tiles)
 };
 
 constexpr int kStepHalo = 4;
@@ -570,7 +572,7 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
     c2<T>* buf0 = reinterpret_cast<c2<T>*>(smem_raw) + PAD;
     c2<T>* buf1 = buf0 + NP + 2 * PAD;
 
-    const int tile = xcd_tile(blockIdx.x, p.num_tiles);
+    const int tile = xcd_tile(blockIdx.x, p.active_tiles ? *p.active_tiles : p.num_tiles);
     if (tile < 0) return;
     const int m = p.tile_map[2 * tile];
     // a copy of the member row: its fields stay in registers past the global stores
@@ -736,6 +738,7 @@ static int launch_step_typed(const pgpe_rk4_args* a, hipStream_t s) {
     p.coef = (const T*)a->coef;
     p.step = a->step;
     p.step_base = a->step_base;
+    p.active_tiles = a->active_tiles;
     switch (a->reservoir) {
         case PGPE_RES_NONE: return launch_step_res<T, PGPE_RES_NONE>(a, p, s);
         case PGPE_RES_SINGLE: return launch_step_res<T, PGPE_RES_SINGLE>(a, p, s);
@@ -808,6 +811,7 @@ static int launch_typed(const pgpe_rk4_args* a, hipStream_t s) {
     p.step = a->step;
     p.step_base = a->step_base;
     p.potential = (const c2<T>*)a->potential;
+    p.active_tiles = a->active_tiles;
     switch (a->reservoir) {
         case PGPE_RES_NONE: return launch_th<T, PGPE_RES_NONE>(a, p, s);
         case PGPE_RES_SINGLE: return launch_th<T, PGPE_RES_SINGLE>(a, p, s);

@@ -59,21 +59,36 @@ class Rk4Engine:
         self._res_alt = [DeviceArray.zeros((nr,), r) for _ in range(self.nres)] if self.whole_step else None
         self.step_base = DeviceArray.zeros((1,), np.uint64)
         self.potential = None      # packed complex potential (DeviceArray) or None; stage kernels only
-        # active tile list: starts as every tile; the sweep driver may retire members
-        self.tile_map = table.tile_map_dev
+        # active tile list: a map of fixed capacity (every tile of the table) whose leading
+        # ``num_tiles`` entries run; the count also lives on the device, where the kernels read
+        # it, so retiring members rewrites both in place and captured graphs stay valid
+        self.capacity = table.num_tiles
+        self.tile_map = DeviceArray.from_numpy(table.tile_map_host.reshape(-1))
         self.num_tiles = table.num_tiles
+        # workgroups per launch: the blocks past the active count return at once, but each still
+        # costs a dispatch (with its LDS), so the launch shrinks when the count halves: O(log n)
+        # graph captures over a sweep, never an allocation
+        self.launch_tiles = table.num_tiles
+        self.active = DeviceArray.from_numpy(np.asarray([self.num_tiles], dtype=np.int32))
         self._graphs: dict = {}
         self._args = N.Rk4Args()
 
     def set_active_tiles(self, pairs: np.ndarray) -> None:
-        """Restrict launches to the listed (member, local tile) pairs."""
-        pairs = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
-        if pairs.shape[0] == 0:
-            self.num_tiles = 0
-            return
-        self.tile_map = DeviceArray.from_numpy(pairs)
-        self.num_tiles = int(pairs.shape[0])
-        self._graphs.clear()
+        """Restrict launches to the listed (member, local tile) pairs: the pairs are written into
+        the head of the existing map and the count into device memory (no allocation, no new
+        graph).  The upload waits for the stream, so launches already queued finish first."""
+        pairs = np.asarray(pairs, dtype=np.int32).reshape(-1)
+        count = pairs.size // 2
+        if count > self.capacity:
+            raise ValueError("more active tiles than the member table holds")
+        if count:
+            self.tile_map.view_range(0, (pairs.size,)).upload(pairs, self.stream)
+        self.active.upload(np.asarray([count], dtype=np.int32), self.stream)
+        self.num_tiles = count
+        if count < self.launch_tiles // 2 or count > self.launch_tiles:
+            # the uploads above waited for the stream: no captured graph is in flight
+            self.launch_tiles = count
+            self._graphs.clear()
 
     def launch_step(self, psi_in: DeviceArray, psi_out: DeviceArray, coef_ptrs, noise: bool, step: int,
                     step_base_ptr=None, stages=(0, 1, 2, 3), res_in=None, res_out=None) -> None:
@@ -85,7 +100,8 @@ class Rk4Engine:
         a = self._args
         a.precision = self.prec_code
         a.reservoir = N.RESERVOIR_KIND[self.reservoir]
-        a.num_tiles = self.num_tiles
+        a.num_tiles = self.launch_tiles
+        a.active_tiles = self.active.ptr
         a.tile_y = tab.tile_h
         a.members = tab.members
         a.components = self.components
@@ -143,6 +159,8 @@ class Rk4Engine:
         m = self.table.members
         return [coef.ptr + ((index * 4 + s) * m) * item for s in range(4)]
 
+    MAX_GRAPH_STEPS = 1000
+
     def run_steps(self, count: int, coef: DeviceArray, const: bool, noise: bool, step0: int,
                   graphs: bool = True) -> None:
         """Advance ``count`` steps from ``psi[0]``; the result ends in ``psi[0]``.
@@ -157,8 +175,10 @@ class Rk4Engine:
             rem = count - done
             if graphs and rem >= 2:
                 chunk = rem - rem % 2
+                if const:   # constant coefficients: replay one graph of at most MAX_GRAPH_STEPS steps
+                    chunk = min(chunk, self.MAX_GRAPH_STEPS)
                 p0, p1 = self.psi
-                key = (chunk, const, noise, self.num_tiles, self.tile_map.ptr, p0.ptr, coef.ptr,
+                key = (chunk, const, noise, self.launch_tiles, p0.ptr, coef.ptr,
                        self.res[0].ptr if self.whole_step and self.nres else 0,
                        self.potential.ptr if self.potential is not None else 0)
                 g = self._graphs.get(key)

@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 
 from oracle import gpe_oracle as O
+from tests.helpers import quickstart_config
 
 pytestmark = pytest.mark.gpu
 
@@ -99,3 +100,33 @@ def test_autotuner_picks_a_valid_form_and_caches_it(tmp_path):
     assert choice in candidates("single", 1)
     assert (tmp_path / "tune.json").exists()
     assert tune_rk4(specs, "single", "single", steps=4, tuner=Autotuner(tmp_path / "tune.json")) == choice
+
+
+def test_reference_sweep_packs_grid_sizes_and_matches_single_runs(tmp_path):
+    """Sweep(cfg, params).run() with grid.nx and laser.P0 varied: one packed launch for all four
+    members (two grid shapes), each member equal to its own SimulationController run; the run
+    directory holds sweep.json, and a resume reads the completed batch back unchanged."""
+    import json
+    from polarism_amd import SimulationController
+    from polarism_amd.controller import apply_overrides
+    from polarism_amd.sweep import Sweep
+    cfg = quickstart_config(grid__nx=32, grid__ny=32, grid__lx=24.0, grid__ly=24.0,
+                            laser__laser_type="continuous-gaussian", laser__sigma_space=5.0, solver__dt=0.01,
+                            solver__total_time=0.5, noise__seed=11)
+    cfg.result.save_results = True
+    cfg.result.output_directory = str(tmp_path)
+    sw = Sweep(cfg, {"grid.nx": [32, 48], "laser.P0": [0.02, 0.05]})
+    assert len(sw.batches) == 1 and sw.batches[0]["packed"]
+    out = sw.run()
+    for i, overrides in enumerate(sw.members):
+        single_cfg = apply_overrides(cfg, overrides)
+        single_cfg.result.save_results = False
+        single = SimulationController(single_cfg)
+        single.run()
+        assert O.rel_l2(out[i]["psi"], single.state.psi) <= 1e-12
+        assert O.rel_l2(out[i]["reservoir"][0], single.reservoir.get_state()[0].get()[0, 0]) <= 1e-12
+    manifest = json.loads((sw.run_directory / "sweep.json").read_text())
+    assert manifest["batch_sizes"] == [4] and manifest["seed"] == 11
+    again = Sweep(cfg, {"grid.nx": [32, 48], "laser.P0": [0.02, 0.05]}).run(resume=sw.run_directory)
+    for a, b in zip(out, again):
+        assert (a["psi"] == b["psi"]).all()
