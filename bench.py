@@ -23,14 +23,15 @@ Workload (one "step" = one pass over the whole held-out batch, plain and warm):
   (outside the timed steps) to report the FP32-vs-FP64 and dt sensitivity of
   the steady density.
 
-Roofline leg: the middle RK4 stage kernel (FP32, single reservoir) on one
-8192 x 8192 member (537 MB per complex field, beyond the 256 MB Infinity
-Cache), timed with HIP events on the solver stream.  Algorithmic bytes per
-point: psi0, k_prev (2 x 8 B), n0, kn_prev, P (3 x 4 B) in; k (8 B), kn (4 B)
-out = 40 B.  The whole-step kernel (all four stages on halo tiles, one
-launch per step) is timed on the same member; its effective rate uses the
-per-stage traffic model (172 B per point-step), its own algorithmic bytes
-are psi0, n0, P in and psi1, n1 out = 28 B per point-step.
+Roofline leg: the headline's dominant kernel, the whole-step RK4 kernel (all
+four stages of a step on halo tiles, one launch per step; FP32, single
+reservoir) on the sweep's 256-member table, timed with HIP events on the
+engine's stream.  Its algorithmic bytes are psi0, n0, P in and psi1, n1 out =
+28 B per point-step.  Beside it: the middle RK4 stage kernel on one 8192 x 8192
+member (537 MB per complex field, beyond the 256 MB Infinity Cache; psi0,
+k_prev, n0, kn_prev, P in, k, kn out = 40 B per point) and the whole-step
+kernel on that member.  HBM traffic counters come from separate rocprofv3
+--pmc runs (not part of this program), so ``traffic`` is null here.
 
 Extra legs (reported beside the headline, not part of value):
   uneven: 256 members with grids drawn from {64..512}^2, double reservoir,
@@ -117,21 +118,41 @@ def _time_launches(e, launch, reps):
     return total / reps
 
 
-def _profiled_traffic(kernel):
-    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", "rk4_stage_fp32_8192_traffic.json")
-    try:
-        with open(path) as f:
-            data = json.load(f)
-    except (OSError, ValueError):
-        return None
-    entry = data.get("kernels", {}).get(kernel)
-    return None if entry is None else int(entry["traffic_bytes"])
+def sweep_step_kernel(reps=20):
+    """Average duration of the whole-step kernel on the headline workload's member table (all 256
+    members of 256^2 active, complex64, single reservoir, the autotuner's tile height), timed with
+    HIP events on the engine's stream around ``reps`` direct launches.  Algorithmic bytes per
+    point-step: psi0 (8), n0 (4), P (4) in, psi1 (8), n1 (4) out = 28 B."""
+    from polarism_amd.autotune import tune_rk4
+    from polarism_amd.device import DeviceArray
+    from polarism_amd.engine import Rk4Engine
+    from polarism_amd.layout import MemberSpec, MemberTable
+    from polarism_amd.surrogate import warmstart as ws
+    members = ws.make_members(SWEEP["members"], SWEEP["seed"])
+    specs, off = [], 0
+    for m in members:
+        specs.append(MemberSpec(m.nx, m.ny, m.lx / m.nx, m.ly / m.ny, m.physics, pump_off=off, seed=m.seed,
+                                dt=SWEEP["dt"]))
+        off += m.nx * m.ny
+    choice = tune_rk4(specs, "single", "single")
+    th = choice.tile_h if choice.whole_step else 32
+    tab = MemberTable(specs, th)
+    pump = DeviceArray.from_numpy(np.concatenate([m.envelope.reshape(-1) for m in members]).astype(np.float32))
+    e = Rk4Engine(tab, pump, "single", "single", whole_step=True)
+    e.set_dt(SWEEP["dt"])
+    rng = np.random.default_rng(0)
+    e.psi[0].upload((0.1 * (rng.standard_normal(off) + 1j * rng.standard_normal(off))).astype(np.complex64))
+    coef = DeviceArray.from_numpy(np.tile(np.asarray([m.P0 for m in members]), (4, 1)).astype(np.float32))
+    ptrs = e.coef_ptrs(coef, 0)
+    r0, r1 = e.res, e._res_alt
+    ms = _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0, res_in=r0, res_out=r1), reps)
+    return ms, off, th
 
 
 def roofline_leg(n=8192, reps=20):
-    """Average durations of the FP32 middle-stage kernel (fastest tile height) and of the whole-step kernel."""
+    """The roofline object is the headline's dominant kernel, the whole-step RK4 kernel on the
+    sweep's 256-member table; the 8192^2 single-member figures (beyond the Infinity Cache) of
+    the middle stage kernel and the whole-step kernel are reported beside it."""
     def stage_kernel_ms(th):
         e, coef = _member_engine(n, th, False)
         ptrs = e.coef_ptrs(coef, 0)
@@ -147,28 +168,32 @@ def roofline_leg(n=8192, reps=20):
         return _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0, res_in=r0, res_out=r1),
                               reps)
 
-    stage = {th: stage_kernel_ms(th) for th in (16, 32)}
-    tile_h = min(stage, key=lambda th: stage[th][0])
-    avg_ms, stage_step_ms = stage[tile_h]
-    bytes_per_launch = 40.0 * n * n
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    pts = float(n) * n
-    traffic = _profiled_traffic(f"rk4_stage_kernel<float, 1, {tile_h}, 1>")
+    ms, points, th = sweep_step_kernel(reps)
+    per_launch = 28.0 * points
+    achieved = per_launch / (ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": f"rk4_stage_kernel<float, SINGLE, {tile_h}, middle>", "grid": f"{n}x{n}",
-            "avg_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch),
-            "avg_us_by_tile_h": {str(k): round(v[0] * 1e3, 2) for k, v in stage.items()},
-            "stage_path_step_us": round(stage_step_ms * 1e3, 2),
-            "stage_path_effective_GBps_172B_model": round(172.0 * pts / (stage_step_ms * 1e-3) / 1e9, 1)}
-    ws = {}
-    for th in (32, 64):
-        ms = step_kernel_ms(th)
-        ws[f"tile_h_{th}"] = {"avg_us": round(ms * 1e3, 2),
-                              "point_steps_per_s": round(pts / (ms * 1e-3), 1),
-                              "effective_GBps_172B_model": round(172.0 * pts / (ms * 1e-3) / 1e9, 1),
-                              "algorithmic_GBps_28B": round(28.0 * pts / (ms * 1e-3) / 1e9, 1)}
-    roof["whole_step_kernel"] = ws
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": f"rk4_step_kernel<float, SINGLE, {th}>", "workload": "256 members of 256x256 (sweep table)",
+            "avg_us": round(ms * 1e3, 2), "algorithmic_bytes_per_launch": int(per_launch),
+            "bytes_per_point_step": 28, "point_steps_per_s": round(points / (ms * 1e-3), 1),
+            "effective_GBps_172B_per_stage_model": round(172.0 * points / (ms * 1e-3) / 1e9, 1)}
+    stage = {th_: stage_kernel_ms(th_) for th_ in (16, 32)}
+    best = min(stage, key=lambda t: stage[t][0])
+    mid_ms, step_ms = stage[best]
+    pts = float(n) * n
+    roof["stage_kernel_8192"] = {
+        "kernel": f"rk4_stage_kernel<float, SINGLE, {best}, middle>", "avg_us": round(mid_ms * 1e3, 2),
+        "algorithmic_bytes_per_launch": int(40.0 * pts),
+        "frac_of_hbm_peak": round(40.0 * pts / (mid_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "avg_us_by_tile_h": {str(k): round(v[0] * 1e3, 2) for k, v in stage.items()},
+        "four_stage_step_us": round(step_ms * 1e3, 2)}
+    ws_ = {}
+    for th_ in (32, 64):
+        t = step_kernel_ms(th_)
+        ws_[f"tile_h_{th_}"] = {"avg_us": round(t * 1e3, 2), "point_steps_per_s": round(pts / (t * 1e-3), 1),
+                                "frac_of_hbm_peak_28B": round(28.0 * pts / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                "effective_GBps_172B_per_stage_model": round(172.0 * pts / (t * 1e-3) / 1e9, 1)}
+    roof["step_kernel_8192"] = ws_
     return roof
 
 
@@ -506,6 +531,8 @@ def main():
     ap.add_argument("--legs", default="sweep,copy,roofline,uneven,fft,ffthbm,lawson,cpu")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--dump", default=None, help="write per-member stop times and density errors to this .npz")
+    ap.add_argument("--max-time", type=float, default=SWEEP["max_time"],
+                    help="sweep horizon in ps (the workload's is 4000; shorter only to bound profiler traces)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the sweep's steps one by one instead of as captured graph chunks (same "
                          "kernels; for kernel-trace profilers that cannot follow graph launches)")
@@ -523,6 +550,7 @@ def main():
               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
               "data": "synthetic (seeded random pumps, noise-seeded initial fields; surrogate trained on "
                       "solver-generated steady states)"}
+    SWEEP["max_time"] = args.max_time
     members = ws.make_members(args.members, SWEEP["seed"])
     have_weights = os.path.exists(args.weights)
     if "sweep" in legs:
