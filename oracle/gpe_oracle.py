@@ -68,6 +68,10 @@ Pinning (what checks each function; tests/ file in brackets):
 * spinor_propagator: scipy.linalg.expm per k, unitarity [test_oracle_pins.py].
 * steady_metrics, initial_psi, pulse_gaussian_coeff: invariants and closed
   forms [test_oracle_pins.py].
+* lattice_sites / lattice_profile / supergaussian_profile: hand-listed square sites,
+  nearest-neighbour distances and coordination numbers of the five lattices, defects,
+  a 90-degree rotation of the square lattice, the disorder draws, super-Gaussian order 1
+  equal to the Gaussian [test_pumps_host.py].
 Parity unpinned (no reference execution possible; restatement-pinned only):
 the full multi-step trajectories of rk4_fdm_step and split_step_* at the
 quickstart / split-step configurations, and the exact DP5 / DOP853 trial
@@ -235,6 +239,60 @@ def ring_profile(grid, radius, width, x0=0.0, y0=0.0):
     """Gaussian radial cross-section (profiles/builtin_profiles.py:66-74)."""
     r = np.sqrt((grid.X - x0) ** 2 + (grid.Y - y0) ** 2)
     return np.exp(-0.5 * ((r - radius) / width) ** 2)
+
+
+def supergaussian_profile(grid, sigma, order, x0=0.0, y0=0.0):
+    """exp(-0.5 (r^2 / sigma^2)^order) (profiles/builtin_profiles.py:45-56)."""
+    r2 = (grid.X - x0) ** 2 + (grid.Y - y0) ** 2
+    return np.exp(-0.5 * (r2 / sigma ** 2) ** order)
+
+
+_LATTICE_GEOMETRY = {
+    # primitive vectors and sublattice offsets in units of the constant (builtin_profiles.py:181-197)
+    "square": ([[1, 0], [0, 1]], [[0, 0]]),
+    "triangular": ([[1, 0], [0.5, np.sqrt(3) / 2]], [[0, 0]]),
+    "honeycomb": ([[1, 0], [0.5, np.sqrt(3) / 2]], [[0, 0], [0.5, 0.5 / np.sqrt(3)]]),
+    "kagome": ([[2, 0], [1, np.sqrt(3)]], [[0, 0], [1, 0], [0.5, np.sqrt(3) / 2]]),
+    "lieb": ([[2, 0], [0, 2]], [[0, 0], [1, 0], [0, 1]]),
+}
+
+
+def lattice_sites(lx, ly, lattice, constant, rotation=0.0, x0=0.0, y0=0.0, defects=()):
+    """(N, 5) rows (i, j, s, x, y) of the sites covering the box plus one cell, in i, j, s order
+    (builtin_profiles.py:200-224): cell range from the box corners in cell coordinates +-2,
+    kept where |x| <= lx/2 + m and |y| <= ly/2 + m, m = the longer primitive vector."""
+    vec, offs = _LATTICE_GEOMETRY[lattice]
+    c, s_ = np.cos(np.radians(rotation)), np.sin(np.radians(rotation))
+    R = np.array([[c, -s_], [s_, c]])
+    B = constant * np.asarray(vec, dtype=np.float64) @ R.T
+    Off = constant * np.asarray(offs, dtype=np.float64) @ R.T
+    o = np.array([x0, y0])
+    corners = np.array([[-lx / 2, -ly / 2], [-lx / 2, ly / 2], [lx / 2, -ly / 2], [lx / 2, ly / 2]]) - o
+    cc = corners @ np.linalg.inv(B)
+    lo = np.floor(cc.min(axis=0)).astype(int) - 2
+    hi = np.ceil(cc.max(axis=0)).astype(int) + 2
+    I, J, S = np.meshgrid(np.arange(lo[0], hi[0] + 1), np.arange(lo[1], hi[1] + 1), np.arange(len(Off)),
+                          indexing="ij")
+    I, J, S = I.ravel(), J.ravel(), S.ravel()
+    P = o + I[:, None] * B[0] + J[:, None] * B[1] + Off[S]
+    m = max(np.hypot(*B[0]), np.hypot(*B[1]))
+    keep = (np.abs(P[:, 0]) <= lx / 2 + m) & (np.abs(P[:, 1]) <= ly / 2 + m)
+    for d in defects:
+        keep &= ~((I == d[0]) & (J == d[1]) & (S == d[2]))
+    return np.column_stack([I, J, S, P[:, 0], P[:, 1]])[keep]
+
+
+def lattice_profile(grid, lattice, constant, width, well="gaussian", order=1.0, disorder=0.0, seed=0,
+                    rotation=0.0, x0=0.0, y0=0.0, defects=()):
+    """Sum of unit wells exp(-0.5 q), q = r^2/w^2 (q^order for super-Gaussian wells), depth
+    1 + N(0, disorder) per site in site order when disorder > 0 (builtin_profiles.py:227-240)."""
+    sites = lattice_sites(grid.lx, grid.ly, lattice, constant, rotation, x0, y0, defects)
+    depth = (1.0 + np.random.default_rng(seed).normal(0.0, disorder, len(sites))) if disorder else np.ones(len(sites))
+    out = np.zeros(grid.shape)
+    for (_, _, _, x, y), d in zip(sites, depth):
+        q = ((grid.X - x) ** 2 + (grid.Y - y) ** 2) / width ** 2
+        out = out + d * np.exp(-0.5 * (q ** order if well == "supergaussian" else q))
+    return out
 
 
 def pulse_gaussian_coeff(t, P0=0.4, Pmax=1.2, sigma_time=0.1, cutoff_sigma=3.0,

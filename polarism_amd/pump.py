@@ -5,9 +5,10 @@ A laser is ``P(x, y, t) = envelope(x, y) * coefficient(t)``
 
 * ``continuous-gaussian`` -- laser/continuous_gaussian.py:14-45
 * ``pulse-gaussian`` (peak-amplitude definition) -- laser/pulse_gaussian.py:19-133
-* ``profile`` with a constant envelope and the ``gaussian``, ``ring``,
-  ``lattice`` (square/triangular) or ``uniform`` profiles --
-  laser/profile_laser.py:26-50, profiles/builtin_profiles.py:38-74, 176-240
+* ``profile`` with a constant envelope and the ``gaussian``, ``supergaussian``, ``ring``,
+  ``lattice`` (square, triangular, honeycomb, kagome, lieb; Gaussian or super-Gaussian wells,
+  disorder, defects, rotation) or ``uniform`` profiles -- laser/profile_laser.py:26-50,
+  profiles/builtin_profiles.py:28-240
 
 ``PumpField.separable_terms()`` exposes ``[(envelope, coefficient_fn)]`` as the
 reference does (laser/pump_field.py:37-89); the HIP solvers upload the
@@ -25,55 +26,120 @@ def _real(precision):
 
 
 # ------------------------------------------------------------------ profiles
+# Spatial profiles of profile lasers (profiles/builtin_profiles.py:28-240).  Each takes the
+# grid and the profile mapping (with x0 / y0 filled in) and returns a real (ny, nx) array.
+def _r2(grid, params):
+    return (grid.X - params.get("x0", 0.0)) ** 2 + (grid.Y - params.get("y0", 0.0)) ** 2
+
+
+def _positive(params, key):
+    v = params.get(key)
+    if isinstance(v, bool) or not isinstance(v, (int, float)) or not math.isfinite(v) or v <= 0:
+        raise ValueError(f"profile.{key} must be a positive finite number")
+    return float(v)
+
+
 def profile_gaussian(grid, params):
-    r2 = (grid.X - params.get("x0", 0.0)) ** 2 + (grid.Y - params.get("y0", 0.0)) ** 2
-    return np.exp(-0.5 * r2 / params["sigma"] ** 2)
+    return np.exp(-0.5 * _r2(grid, params) / _positive(params, "sigma") ** 2)
+
+
+def profile_supergaussian(grid, params):
+    sigma, order = _positive(params, "sigma"), _positive(params, "order")
+    return np.exp(-0.5 * (_r2(grid, params) / sigma ** 2) ** order)
 
 
 def profile_ring(grid, params):
-    r = np.sqrt((grid.X - params.get("x0", 0.0)) ** 2 + (grid.Y - params.get("y0", 0.0)) ** 2)
-    return np.exp(-0.5 * ((r - params["radius"]) / params["width"]) ** 2)
+    radius, width = _positive(params, "radius"), _positive(params, "width")
+    return np.exp(-0.5 * ((np.sqrt(_r2(grid, params)) - radius) / width) ** 2)
 
 
 def profile_uniform(grid, params):
     return np.ones(grid.shape)
 
 
-def _lattice_basis(kind, a):
-    if kind == "square":
-        return np.array([[a, 0.0], [0.0, a]])
-    if kind == "triangular":
-        return np.array([[a, 0.0], [a / 2, math.sqrt(3) * a / 2]])
-    raise ValueError(f"unsupported lattice {kind!r} (square or triangular)")
+# Bravais lattices: primitive vectors and sublattice offsets in units of the lattice constant a
+# (a is the nearest-neighbour distance except for honeycomb, where it is sqrt(3) times it).
+_R3 = math.sqrt(3.0)
+LATTICES = {
+    "square": (((1.0, 0.0), (0.0, 1.0)), ((0.0, 0.0),)),
+    "triangular": (((1.0, 0.0), (0.5, _R3 / 2)), ((0.0, 0.0),)),
+    "honeycomb": (((1.0, 0.0), (0.5, _R3 / 2)), ((0.0, 0.0), (0.5, 0.5 / _R3))),
+    "kagome": (((2.0, 0.0), (1.0, _R3)), ((0.0, 0.0), (1.0, 0.0), (0.5, _R3 / 2))),
+    "lieb": (((2.0, 0.0), (0.0, 2.0)), ((0.0, 0.0), (1.0, 0.0), (0.0, 1.0))),
+}
 
 
-def profile_lattice(grid, params):
-    """Sum of unit Gaussian wells on a Bravais lattice covering the box plus a margin."""
-    basis = _lattice_basis(params["lattice"], float(params["constant"]))
-    th = math.radians(params.get("rotation", 0.0))
+def _lattice_params(params):
+    kind = params.get("lattice")
+    if kind not in LATTICES:
+        raise ValueError(f"profile.lattice must be one of {sorted(LATTICES)}")
+    a, width = _positive(params, "constant"), _positive(params, "width")
+    well = params.get("well", "gaussian")
+    if well not in ("gaussian", "supergaussian"):
+        raise ValueError("profile.well must be gaussian or supergaussian")
+    order = _positive(params, "order") if well == "supergaussian" else 1.0
+    if well == "gaussian" and "order" in params:
+        raise ValueError("profile.order requires well: supergaussian")
+    disorder = params.get("disorder", 0.0)
+    if isinstance(disorder, bool) or not isinstance(disorder, (int, float)) or not math.isfinite(disorder) \
+            or disorder < 0:
+        raise ValueError("profile.disorder must be a nonnegative finite number")
+    seed = params.get("seed", 0)
+    if type(seed) is not int or seed < 0:
+        raise ValueError("profile.seed must be a nonnegative integer")
+    defects = params.get("defects", [])
+    subl = len(LATTICES[kind][1])
+    if not isinstance(defects, list) or any(
+            not isinstance(d, (list, tuple)) or len(d) != 3 or any(type(v) is not int for v in d)
+            or not 0 <= d[2] < subl for d in defects):
+        raise ValueError("profile.defects must be a list of [i, j, s] integer indices with a valid sublattice")
+    return kind, a, width, order, float(disorder), seed, {tuple(d) for d in defects}
+
+
+def lattice_sites(lx, ly, params):
+    """(i, j, s, x, y) of the lattice sites that cover the box plus one primitive cell, in the
+    order i, then j, then sublattice s (profiles/builtin_profiles.py:200-224): the cell index
+    range spans the box corners mapped into cell coordinates, widened by two cells; a site is kept
+    when both its coordinates are within half the box plus the longer primitive vector."""
+    kind, a, _, _, _, _, defects = _lattice_params(params)
+    vec, offs = LATTICES[kind]
+    th = math.radians(float(params.get("rotation", 0.0)))
     rot = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
-    basis = basis @ rot.T
+    basis = (a * np.asarray(vec)) @ rot.T
+    offsets = (a * np.asarray(offs)) @ rot.T
     origin = np.array([params.get("x0", 0.0), params.get("y0", 0.0)])
-    corners = np.array([[x, y] for x in (-grid.lx / 2, grid.lx / 2) for y in (-grid.ly / 2, grid.ly / 2)])
-    cells = (corners - origin) @ np.linalg.inv(basis)
-    lo = np.floor(cells.min(axis=0)).astype(int) - 2
-    hi = np.ceil(cells.max(axis=0)).astype(int) + 2
-    margin = max(np.linalg.norm(basis[0]), np.linalg.norm(basis[1]))
-    width = float(params["width"])
-    disorder = float(params.get("disorder", 0.0))
-    rng = np.random.default_rng(params.get("seed", 0))
-    out = np.zeros(grid.shape)
+    corners = np.array([[sx * lx / 2, sy * ly / 2] for sx in (-1, 1) for sy in (-1, 1)])
+    cell = (corners - origin) @ np.linalg.inv(basis)
+    lo = np.floor(cell.min(axis=0)).astype(int) - 2
+    hi = np.ceil(cell.max(axis=0)).astype(int) + 2
+    reach = max(np.linalg.norm(basis[0]), np.linalg.norm(basis[1]))
+    out = []
     for i in range(lo[0], hi[0] + 1):
         for j in range(lo[1], hi[1] + 1):
-            x, y = origin + i * basis[0] + j * basis[1]
-            if abs(x) <= grid.lx / 2 + margin and abs(y) <= grid.ly / 2 + margin:
-                depth = 1.0 + rng.normal(0.0, disorder) if disorder else 1.0
-                out += depth * np.exp(-0.5 * ((grid.X - x) ** 2 + (grid.Y - y) ** 2) / width ** 2)
+            for k, off in enumerate(offsets):
+                if (i, j, k) in defects:
+                    continue
+                x, y = origin + i * basis[0] + j * basis[1] + off
+                if abs(x) <= lx / 2 + reach and abs(y) <= ly / 2 + reach:
+                    out.append((i, j, k, float(x), float(y)))
     return out
 
 
-PROFILES = {"gaussian": profile_gaussian, "ring": profile_ring, "lattice": profile_lattice,
-            "uniform": profile_uniform}
+def profile_lattice(grid, params):
+    """Sum of wells exp(-0.5 (r^2 / w^2)^order) on the lattice sites, each of depth 1, or
+    1 + N(0, disorder) drawn in site order from default_rng(seed) when disorder > 0."""
+    _, _, width, order, disorder, seed, _ = _lattice_params(params)
+    rng = np.random.default_rng(seed)
+    out = np.zeros(grid.shape, dtype=np.float64)
+    for *_, x, y in lattice_sites(grid.lx, grid.ly, params):
+        depth = 1.0 + rng.normal(0.0, disorder) if disorder else 1.0
+        q = ((grid.X - x) ** 2 + (grid.Y - y) ** 2) / width ** 2
+        out += depth * np.exp(-0.5 * (q ** order if order != 1.0 else q))
+    return out
+
+
+PROFILES = {"gaussian": profile_gaussian, "supergaussian": profile_supergaussian, "ring": profile_ring,
+            "lattice": profile_lattice, "uniform": profile_uniform}
 
 
 def create_profile(spec, grid):
