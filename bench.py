@@ -441,6 +441,58 @@ def lawson_leg(ticks=20):
     return out
 
 
+def fno_leg(n=8192, reps=10):
+    """The surrogate's MFMA kernel and network: a square BF16 GEMM (n^3, FP32 accumulate) in
+    TFLOP/s against the 2.5 PFLOP/s dense BF16 peak, and the FNO's forward pass and one training
+    step (forward, hand-written backward, AdamW) at batch 16 on 256^2 maps."""
+    import torch
+    from polarism_amd.surrogate.fno import FNOConfig, FNOModule
+    from polarism_amd.surrogate.kernels import gemm
+    dev = torch.device("cuda")
+    a = torch.randn(n, n, device=dev).to(torch.bfloat16)
+    b = torch.randn(n, n, device=dev).to(torch.bfloat16)
+    c = torch.empty(n, n, device=dev)
+    run = lambda: gemm(a, b, c, n, n, n, sa=(n, 1, 0), sb=(n, 1, 0), sc=(n, 1, 0))
+    run()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        run()
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / reps
+    tflops = 2.0 * n ** 3 / (ms * 1e-3) / 1e12
+    cfg = FNOConfig()
+    mod = FNOModule(cfg, seed=0)
+    x = torch.randn(cfg.fin, 16, cfg.H, cfg.W, device=dev)
+    y = torch.randn(cfg.fout, 16, cfg.H, cfg.W, device=dev)
+    opt = torch.optim.AdamW(mod.parameters(), lr=1e-4)
+
+    def train_step():
+        opt.zero_grad(set_to_none=True)
+        loss = ((mod(x) - y) ** 2).mean()
+        loss.backward()
+        opt.step()
+
+    for fn in (train_step,):
+        fn()
+    torch.cuda.synchronize()
+    t = {}
+    for name, fn in (("forward", lambda: mod(x)), ("train_step", train_step)):
+        start.record()
+        for _ in range(reps):
+            with torch.no_grad() if name == "forward" else torch.enable_grad():
+                fn()
+        stop.record()
+        torch.cuda.synchronize()
+        t[name] = start.elapsed_time(stop) / reps
+    return {"gemm_bf16": {"n": n, "avg_us": round(ms * 1e3, 1), "TFLOPs": round(tflops, 1),
+                          "frac_of_2500_TFLOPs_dense_bf16": round(tflops / 2500.0, 4)},
+            "fno_batch16_256": {"params": sum(p.numel() for p in mod.parameters()),
+                                "forward_ms": round(t["forward"], 2), "train_step_ms": round(t["train_step"], 2)}}
+
+
 GATE = 1e-3
 
 
@@ -528,7 +580,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--members", type=int, default=SWEEP["members"])
-    ap.add_argument("--legs", default="sweep,copy,roofline,uneven,fft,ffthbm,lawson,cpu")
+    ap.add_argument("--legs", default="sweep,copy,roofline,uneven,fft,ffthbm,lawson,fno,cpu")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--dump", default=None, help="write per-member stop times and density errors to this .npz")
     ap.add_argument("--max-time", type=float, default=SWEEP["max_time"],
@@ -598,6 +650,8 @@ def main():
         result["fft_batched_2048"] = fft_hbm_leg()
     if "lawson" in legs:
         result["lawson_spinor_dp5"] = lawson_leg()
+    if "fno" in legs:
+        result["surrogate_fno"] = fno_leg()
     result["config"] = {"workload": "steady-state pump sweep, 256 members of 256x256, rk4-fdm-hip complex64, "
                                     "single reservoir, dt 0.05 ps; warm start (FNO) vs plain cold start",
                         "members": len(members), "grid": "256x256", "lx_um": SWEEP["L"], "dt_ps": SWEEP["dt"],

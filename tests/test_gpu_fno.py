@@ -48,7 +48,7 @@ def test_fno_forward_matches_float64_reference():
     ref = ref_forward(x.double().cpu(), p64, cfg)
     err = ((out.double().cpu() - ref).norm() / ref.norm()).item()
     print("forward rel-L2 vs float64:", err)
-    assert err < 3e-2
+    assert err < 3e-3          # BF16 operands (unit roundoff 2^-8), FP32 accumulation: measured 8.5e-4
 
 
 def test_fno_gradients_match_float64_autograd():
@@ -70,8 +70,8 @@ def test_fno_gradients_match_float64_autograd():
         cos = torch.nn.functional.cosine_similarity(g.flatten(), r.flatten(), dim=0).item()
         print(f"{k}: rel-L2 {rel:.3e} cos {cos:.6f}")
         worst = max(worst, rel)
-        assert cos > 0.995, k
-    assert worst < 0.1
+        assert cos > 0.9999, k
+    assert worst < 2e-2        # measured worst 9.6e-3 (d2_im) with BF16 products
 
 
 def test_fno_parameter_count_near_1p5M():
