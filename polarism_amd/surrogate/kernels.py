@@ -85,7 +85,20 @@ def gelu_bwd(x, dy, dx=None, dxb=None):
 
 
 def colsum(x, out, rows, cols, row_stride, col_stride):
-    N.call("pgpe_colsum", x.data_ptr(), out.data_ptr(), rows, cols, row_stride, col_stride, _stream())
+    """out[c] = sum_r x[r row_stride + c col_stride].  Long contiguous columns (one workgroup
+    each would leave the chip idle) are first summed in ``chunks`` contiguous pieces, each its own
+    workgroup, then the pieces of each column are summed: a fixed order either way."""
+    chunks = 1
+    if row_stride == 1 and col_stride == rows:
+        while chunks < 256 and rows % (2 * chunks) == 0 and rows // (2 * chunks) >= 4096 and cols * chunks < 2048:
+            chunks *= 2
+    if chunks == 1:
+        N.call("pgpe_colsum", x.data_ptr(), out.data_ptr(), rows, cols, row_stride, col_stride, _stream())
+        return
+    piece = rows // chunks
+    part = torch.empty(cols * chunks, dtype=torch.float32, device=out.device)
+    N.call("pgpe_colsum", x.data_ptr(), part.data_ptr(), piece, cols * chunks, 1, piece, _stream())
+    N.call("pgpe_colsum", part.data_ptr(), out.data_ptr(), chunks, cols, 1, chunks, _stream())
 
 
 def mode_scale(vr, vi, dr, di, ur=None, ui=None, urb=None, uib=None, conj=False):
