@@ -179,6 +179,15 @@ typedef struct pgpe_rk4_args {
     const uint64_t* step_base;       /* device or NULL */
     const void* potential;           /* complex V packed per member at pot_off, or NULL for V = 0;
                                         enters as (V + gC|psi|^2 + gR nA + gI nI) psi (model.py:89-127) */
+    const int32_t* active_tiles;     /* device or NULL: how many leading tile_map entries run; the
+                                        launch is then sized for num_tiles entries (the map's
+                                        capacity), so a captured graph stays valid while members
+                                        retire and the map is rewritten in place */
+    int32_t strips;                  /* pgpe_rk4_step only: 0 = 64 x tile_y tiles of the member
+                                        table; 1 = row strips of 56 output columns x tile_y rows
+                                        (16, 24, or 32 in single precision), local index
+                                        (row * ceil(nx / 56) + column), from the strip map */
+    int32_t pad1;
 } pgpe_rk4_args;
 
 int pgpe_rk4_stage(const pgpe_rk4_args* args, void* stream);

@@ -2,8 +2,9 @@
 
 For a batch of members it times every launch form the device supports --
 the four fused stage launches (1-point halo, tile heights 8 / 16 / 32) and
-the whole-step launch (4-point halo recomputed on chip, tile heights 16 / 32
-and 64 in single precision) -- on the batch's own member table, and keeps the
+the whole-step launch (4-point halo recomputed on chip, LDS tile heights 16 / 32
+and 64 in single precision, or register row strips 16 / 24 and 32 in single
+precision high) -- on the batch's own member table, and keeps the
 fastest.  Every form gives the same result (bit for bit on periodic grids,
 to rounding on closed intervals; the kernels share their arithmetic), so the
 choice only changes speed.  Choices are stored in a
@@ -27,20 +28,30 @@ import numpy as np
 
 @dataclass(frozen=True)
 class LaunchChoice:
-    form: str      # "step" (whole-step kernel) or "stage" (four stage kernels)
-    tile_h: int
+    form: str      # "step" (whole-step kernel on LDS tiles), "strip" (whole-step kernel on
+    tile_h: int    # register row strips) or "stage" (four stage kernels)
 
     @property
     def whole_step(self) -> bool:
-        return self.form == "step"
+        return self.form in ("step", "strip")
+
+    @property
+    def table_h(self) -> int:
+        """Tile height of the member table (row strips keep the table at 16 for the reductions)."""
+        return 16 if self.form == "strip" else self.tile_h
+
+    @property
+    def strip_h(self) -> int | None:
+        return self.tile_h if self.form == "strip" else None
 
 
 def candidates(precision: str, components: int) -> list[LaunchChoice]:
     out = [LaunchChoice("stage", h) for h in (8, 16, 32)]
     if components == 1:
         out += [LaunchChoice("step", h) for h in (16, 32)]
+        out += [LaunchChoice("strip", h) for h in (16, 24)]
         if precision == "single":
-            out.append(LaunchChoice("step", 64))
+            out += [LaunchChoice("step", 64), LaunchChoice("strip", 32)]
     return out
 
 
@@ -135,8 +146,8 @@ def tune_rk4(specs, precision: str, reservoir: str, *, steps: int = 20, tuner: A
     psi = (1e-2 * (rng.standard_normal(n_field) + 1j * rng.standard_normal(n_field)))
 
     def timer(opt: LaunchChoice) -> float:
-        tab = MemberTable(sub_specs, opt.tile_h)
-        e = Rk4Engine(tab, pump, precision, reservoir, whole_step=opt.whole_step)
+        tab = MemberTable(sub_specs, opt.table_h)
+        e = Rk4Engine(tab, pump, precision, reservoir, whole_step=opt.whole_step, strip_h=opt.strip_h)
         e.set_dt(sub_specs[0].dt)
         e.psi[0].upload(psi.astype(e.cdtype))
         e.run_steps(steps, coef, True, False, 0)      # captures the graph and warms up

@@ -689,6 +689,184 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
     }
 }
 
+// ------------------------------------------------------------------ whole step, row strips
+// The same four-stage temporal block as rk4_step_kernel with a layout that keeps the stencil's
+// neighbours in registers.  A strip is 64 columns (one per lane of a wavefront: 56 output
+// columns and 4 halo columns each side) by 4R rows (TH = 4R - 8 output rows and 4 halo rows each
+// side); wavefront w of the 256-thread workgroup owns the R consecutive rows [wR, wR + R) of
+// every column, so a point's y neighbours are the same thread's adjacent registers, its x
+// neighbours the adjacent lanes' (read with DPP wave shifts, no LDS), and only the first and
+// last row of each wavefront cross to the neighbouring wavefronts through LDS, once per stage.
+// Per-point arithmetic is rk4_step_kernel's (lap5 / lap9, scalar_rhs, the same accumulation),
+// so the two whole-step kernels and the stage kernels agree bit for bit on periodic grids.
+constexpr int kStripW = kWave - 2 * kStepHalo;   // 56 output columns per strip
+
+// Value of the next (lane + 1) or previous (lane - 1) lane of the wavefront (DPP wave_shl:1 /
+// wave_shr:1); the last / first lane receives 0, a halo column whose value is never used.
+__device__ __forceinline__ int lane_next_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false); }
+__device__ __forceinline__ int lane_prev_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false); }
+__device__ __forceinline__ float lane_next(float v) { return __builtin_bit_cast(float, lane_next_i(__builtin_bit_cast(int, v))); }
+__device__ __forceinline__ float lane_prev(float v) { return __builtin_bit_cast(float, lane_prev_i(__builtin_bit_cast(int, v))); }
+__device__ __forceinline__ double lane_next(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)lane_next_i((int)(uint32_t)u), hi = (uint32_t)lane_next_i((int)(uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double lane_prev(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)lane_prev_i((int)(uint32_t)u), hi = (uint32_t)lane_prev_i((int)(uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <typename T>
+__device__ __forceinline__ c2<T> lane_next_c(const c2<T>& v) { return make_c<T>(lane_next(v.x), lane_next(v.y)); }
+template <typename T>
+__device__ __forceinline__ c2<T> lane_prev_c(const c2<T>& v) { return make_c<T>(lane_prev(v.x), lane_prev(v.y)); }
+
+template <typename T, int RES, int R>
+__global__ void __launch_bounds__(256) rk4_strip_kernel(const Rk4StepParams<T> p) {
+#pragma clang fp contract(on)
+    constexpr int NF = ResFields<RES>::n;
+    constexpr int EH = 4 * R;
+    constexpr int TH = EH - 2 * kStepHalo;
+    // first and last row of each wavefront, double buffered over the stages
+    __shared__ c2<T> edge[2][4][2][kWave];
+
+    const int tile = xcd_tile(blockIdx.x, p.active_tiles ? *p.active_tiles : p.num_tiles);
+    if (tile < 0) return;
+    const int m = p.tile_map[2 * tile];
+    const pgpe_member mb = p.table[m];
+    const int nx = mb.nx, ny = mb.ny;
+    const int local = p.tile_map[2 * tile + 1];
+    const int strips_x = (nx + kStripW - 1) / kStripW;
+    const int x0 = (local % strips_x) * kStripW;
+    const int y0 = (local / strips_x) * TH;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    const int64_t off = mb.off;
+    const ScalarConsts<T> kc(mb);
+    const bool nine = mb.laplacian == PGPE_LAP_NINE;
+    const T inv_6h2 = kc.inv_dx2 / (T)6;
+    T coef[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) coef[s] = p.coef[s * p.members + m];
+
+    const int gx = map_index(x0 - kStepHalo + lane, nx, mb.boundary);
+    const bool periodic = mb.boundary != PGPE_BC_CLOSED;
+    c2<T> z[R], cur[R], acc[R];
+    T env[R], n0[R][2], ns[R][2], accn[R][2];
+    int gy = map_index(y0 - kStepHalo + w * R, ny, mb.boundary);
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        if (j > 0) {
+            // the next row: periodic rows step with a wrap; mirrored rows need the full map
+            if (periodic) gy = gy + 1 == ny ? 0 : gy + 1;
+            else gy = map_index(y0 - kStepHalo + w * R + j, ny, mb.boundary);
+        }
+        const int64_t g = (int64_t)gy * nx + gx;
+        z[j] = p.psi0[off + g];
+        cur[j] = z[j];
+        env[j] = p.pump[mb.pump_off + g];
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            n0[j][q] = p.res0[q][off + g];
+            ns[j][q] = n0[j][q];
+        }
+    }
+
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const T a_next = s < 2 ? p.a_half : p.a_full;
+        edge[s & 1][w][0][lane] = cur[0];
+        edge[s & 1][w][1][lane] = cur[R - 1];
+        __syncthreads();
+        // wavefront 0 has no row below its first and wavefront 3 none above its last: they read
+        // their own edge rows instead, which only feed halo points that are stale by the end of
+        // the step (a plain clamped index: no select between LDS and register values)
+        c2<T> south = edge[s & 1][w > 0 ? w - 1 : 0][1][lane];
+        const c2<T> above = edge[s & 1][w < 3 ? w + 1 : 3][0][lane];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const c2<T> ps = cur[j];
+            const c2<T> pn = j + 1 < R ? cur[j + 1] : above;
+            const c2<T> pso = south;
+            const c2<T> pe = lane_next_c<T>(ps), pw = lane_prev_c<T>(ps);
+            c2<T> lap;
+            if (nine) {
+                lap = lap9<T>(ps, pn, pso, pe, pw, lane_prev_c<T>(pso), lane_next_c<T>(pso), lane_prev_c<T>(pn),
+                              lane_next_c<T>(pn), inv_6h2);
+            } else {
+                lap = lap5<T>(ps, pn, pso, pe, pw, kc.inv_dx2, kc.inv_dy2);
+            }
+            south = ps;
+            const T P = env[j] * coef[s];
+            c2<T> d;
+            T dn[2];
+            scalar_rhs<T, RES>(mb, kc, ps, lap, ns[j], P, nullptr, d, dn);
+            if (s == 0) {
+                acc[j] = d;
+#pragma unroll
+                for (int q = 0; q < NF; ++q) accn[j][q] = dn[q];
+            } else if (s < 3) {
+                acc[j].x = acc[j].x + (T)2 * d.x;
+                acc[j].y = acc[j].y + (T)2 * d.y;
+#pragma unroll
+                for (int q = 0; q < NF; ++q) accn[j][q] = accn[j][q] + (T)2 * dn[q];
+            }
+            if (s < 3) {
+                cur[j].x = z[j].x + a_next * d.x;
+                cur[j].y = z[j].y + a_next * d.y;
+#pragma unroll
+                for (int q = 0; q < NF; ++q) ns[j][q] = n0[j][q] + a_next * dn[q];
+            } else {
+                const int ly = w * R + j;
+                const int ox = x0 + lane - kStepHalo, oy = y0 + ly - kStepHalo;
+                if (lane < kStepHalo || lane >= kStepHalo + kStripW || ly < kStepHalo || ly >= kStepHalo + TH ||
+                    ox >= nx || oy >= ny)
+                    continue;
+                const int64_t idx = (int64_t)oy * nx + ox;
+                c2<T> out;
+                out.x = z[j].x + p.dt6 * (acc[j].x + d.x);
+                out.y = z[j].y + p.dt6 * (acc[j].y + d.y);
+                T n_new[2] = {(T)0, (T)0};
+#pragma unroll
+                for (int q = 0; q < NF; ++q) {
+                    T v = n0[j][q] + p.dt6 * (accn[j][q] + dn[q]);
+                    v = v > (T)0 ? v : (T)0;  // set_state clamp (single_reservoir.py:70-73)
+                    n_new[q] = v;
+                    p.res_out[q][off + idx] = v;
+                }
+                if (p.noise) {
+                    uint64_t step = p.step;
+                    if (p.step_base) step += *p.step_base;
+                    wigner_site<T>(out.x, out.y, mb.seed, (uint32_t)idx, step, 1u, (T)mb.noise_strength,
+                                   (T)mb.noise_gamma, (T)mb.noise_R, NF > 0 ? n_new[0] : (T)0,
+                                   (T)mb.noise_dt_over_4v);
+                }
+                p.psi_out[off + idx] = out;
+            }
+        }
+    }
+}
+
+template <typename T, int RES, int R>
+static int launch_strip_r(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
+    hipLaunchKernelGGL((rk4_strip_kernel<T, RES, R>), dim3(xcd_grid(a->num_tiles)), dim3(256), 0, s, p);
+    PGPE_LAUNCHED("rk4_strip_kernel");
+    return PGPE_OK;
+}
+
+template <typename T, int RES>
+static int launch_strip_res(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
+    switch (a->tile_y) {
+        case 16: return launch_strip_r<T, RES, 6>(a, p, s);
+        case 24: return launch_strip_r<T, RES, 8>(a, p, s);
+        case 32:
+            if constexpr (sizeof(T) == 4) return launch_strip_r<T, RES, 10>(a, p, s);
+            return fail(PGPE_ERR_ARG, "pgpe_rk4_step: strip height 32 is single precision only");
+        default: return fail(PGPE_ERR_ARG, "pgpe_rk4_step: strip height (tile_y) must be 16, 24 or 32");
+    }
+}
+
 template <typename T, int RES, int TH>
 static int launch_step_th(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
     constexpr int NP = (kTileW + 2 * kStepHalo) * (TH + 2 * kStepHalo);
@@ -707,6 +885,7 @@ static int launch_step_th(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hip
 
 template <typename T, int RES>
 static int launch_step_res(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
+    if (a->strips) return launch_strip_res<T, RES>(a, p, s);
     switch (a->tile_y) {
         case 16: return launch_step_th<T, RES, 16>(a, p, s);
         case 32: return launch_step_th<T, RES, 32>(a, p, s);

@@ -4,7 +4,8 @@ Owns the buffers of one batch (uniform or uneven grids) and issues either the
 four fused ``pgpe_rk4_stage`` launches of a step or, with ``whole_step``, one
 ``pgpe_rk4_step`` launch that runs all four stages on halo tiles (same
 result, bit for bit on periodic grids; scalar fields; the reservoir then ping-pongs between two
-buffers like the field).  Used by the ``rk4-fdm-hip`` solver and by the
+buffers like the field), on 64 x tile_h halo tiles staged in LDS or, with ``strip_h``, on row
+strips of 56 output columns held in registers.  Used by the ``rk4-fdm-hip`` solver and by the
 packed sweep driver.
 """
 from __future__ import annotations
@@ -28,7 +29,7 @@ def complex_dtype(precision):
 
 class Rk4Engine:
     def __init__(self, table: MemberTable, pump: DeviceArray, precision: str, reservoir: str,
-                 stream: Stream | None = None, whole_step: bool = False):
+                 stream: Stream | None = None, whole_step: bool = False, strip_h: int | None = None):
         self.table = table
         self.pump = pump
         self.precision = precision
@@ -51,6 +52,13 @@ class Rk4Engine:
                 raise ValueError("whole_step needs tile_h 16 or 32 (or 64 in single precision)")
         elif table.tile_h not in (8, 16, 32):
             raise ValueError("the stage kernels need tile_h 8, 16 or 32")
+        # row strips: the whole-step kernel with one lane per column and rows in registers
+        self.strip_h = None if strip_h is None else int(strip_h)
+        if self.strip_h is not None:
+            if not self.whole_step:
+                raise ValueError("row strips run the whole-step kernel (whole_step=True)")
+            if self.strip_h not in (16, 24, 32) or (self.strip_h == 32 and precision != "single"):
+                raise ValueError("strip_h must be 16 or 24 (or 32 in single precision)")
         self.psi = [DeviceArray.empty((n,), c), DeviceArray.empty((n,), c)]
         stage_bufs = 0 if self.whole_step else 3
         self.k = [DeviceArray.empty((n,), c) for _ in range(stage_bufs)]
@@ -62,18 +70,24 @@ class Rk4Engine:
         # active tile list: a map of fixed capacity (every tile of the table) whose leading
         # ``num_tiles`` entries run; the count also lives on the device, where the kernels read
         # it, so retiring members rewrites both in place and captured graphs stay valid
-        self.capacity = table.num_tiles
-        self.tile_map = DeviceArray.from_numpy(table.tile_map_host.reshape(-1))
-        self.num_tiles = table.num_tiles
+        launch_map = table.tile_map_host if self.strip_h is None else table.strip_pairs(self.strip_h)
+        self.capacity = int(launch_map.shape[0])
+        self.tile_map = DeviceArray.from_numpy(launch_map.reshape(-1))
+        self.num_tiles = self.capacity
         # workgroups per launch.  Kernels read the active count from the device, so a graph
         # sized for more tiles stays correct, but each surplus block still costs a dispatch with
         # its LDS: a launch sized to the next power of two above the count ran 15-18% slower
         # over the 256-member sweep than one sized exactly, re-captured at each retirement
         # (a capture of ~20 launches, no allocation, after the stream is idle)
-        self.launch_tiles = table.num_tiles
+        self.launch_tiles = self.capacity
         self.active = DeviceArray.from_numpy(np.asarray([self.num_tiles], dtype=np.int32))
         self._graphs: dict = {}
         self._args = N.Rk4Args()
+
+    def set_active_members(self, members) -> None:
+        """Restrict launches to the tiles (or strips) of ``members``."""
+        tab = self.table
+        self.set_active_tiles(tab.tiles_of(members) if self.strip_h is None else tab.strip_pairs(self.strip_h, members))
 
     def set_active_tiles(self, pairs: np.ndarray) -> None:
         """Restrict launches to the listed (member, local tile) pairs: the pairs are written into
@@ -104,7 +118,8 @@ class Rk4Engine:
         a.reservoir = N.RESERVOIR_KIND[self.reservoir]
         a.num_tiles = self.launch_tiles
         a.active_tiles = self.active.ptr
-        a.tile_y = tab.tile_h
+        a.tile_y = tab.tile_h if self.strip_h is None else self.strip_h
+        a.strips = 0 if self.strip_h is None else 1
         a.members = tab.members
         a.components = self.components
         a.res_components = self.res_components

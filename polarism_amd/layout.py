@@ -21,6 +21,7 @@ from .device import DeviceArray
 from .reservoir import reservoir_rates
 
 TILE_W = 64
+STRIP_W = 56      # output columns of a row strip: a 64-lane wavefront less a 4-column halo each side
 
 
 @dataclass
@@ -121,6 +122,18 @@ class MemberTable:
         """(member, local) pairs of the given members, in table order."""
         keep = np.isin(self.tile_map_host[:, 0], np.asarray(list(members), dtype=np.int32))
         return self.tile_map_host[keep]
+
+    def strip_pairs(self, strip_h: int, members=None) -> np.ndarray:
+        """(member, local strip) pairs of the row-strip whole-step launch: member m is cut into
+        ceil(nx / STRIP_W) x ceil(ny / strip_h) strips of STRIP_W output columns, numbered row
+        by row; ``members`` (default all) selects whose strips, in table order."""
+        keep = range(self.members) if members is None else sorted(int(m) for m in members)
+        out = []
+        for m in keep:
+            s = self.specs[m]
+            n = (-(-s.nx // STRIP_W)) * (-(-s.ny // int(strip_h)))
+            out.append(np.stack([np.full(n, m, dtype=np.int32), np.arange(n, dtype=np.int32)], axis=1))
+        return np.concatenate(out) if out else np.zeros((0, 2), dtype=np.int32)
 
     def member_points(self, m: int) -> int:
         s = self.specs[m]

@@ -70,8 +70,9 @@ class PackedSweep:
     def __init__(self, members: list[SweepMember], *, reservoir="single", precision="single", dt=0.05,
                  check_every=1.0, rtol=1e-6, min_time=0.0, max_time=1000.0, tile_h="auto", form="auto",
                  initial_amplitude=1e-3, retire=True, graphs=True, progress=None, progress_every_s=30.0):
-        """``form`` is "step" (whole-step kernel), "stage" (four stage kernels) or "auto";
-        ``tile_h`` a tile height or "auto", which asks the autotuner (cached per size mix).
+        """``form`` is "step" (whole-step kernel on LDS tiles), "strip" (whole-step kernel on
+        register row strips), "stage" (four stage kernels) or "auto"; ``tile_h`` a tile (strip)
+        height or "auto", which asks the autotuner (cached per size mix).
         An explicit tile height with ``form="auto"`` takes the whole-step kernel where that
         height allows it (16, 32, 64)."""
         init()
@@ -107,16 +108,19 @@ class PackedSweep:
             if form in ("auto", choice.form):
                 form, tile_h = choice.form, choice.tile_h
             else:
-                tile_h = 32 if form == "step" else 16
+                tile_h = {"step": 32, "strip": 24}.get(form, 16)
         elif form == "auto":
             form = "step" if tile_h in (16, 32, 64) else "stage"
+        if form not in ("step", "strip", "stage"):
+            raise ValueError(f"unknown launch form {form!r}")
         self.form, self.tile_h = form, int(tile_h)
-        self.table = MemberTable(specs, self.tile_h)
+        self.table = MemberTable(specs, 16 if form == "strip" else self.tile_h)
         r = real_dtype(precision)
         env = np.concatenate([np.asarray(m.envelope, dtype=np.float64).reshape(-1) for m in members])
         self.pump = DeviceArray.from_numpy(env.astype(r))
         self.engine = Rk4Engine(self.table, self.pump, precision, reservoir, self.stream,
-                                whole_step=(form == "step"))
+                                whole_step=form in ("step", "strip"),
+                                strip_h=self.tile_h if form == "strip" else None)
         self.engine.set_dt(self.dt)
         coef = np.tile(np.asarray([m.P0 for m in members], dtype=np.float64), (4, 1)).astype(r)
         self.coef = DeviceArray.from_numpy(coef)
@@ -124,6 +128,10 @@ class PackedSweep:
         n = self.table.total_points
         self.old = DeviceArray.zeros((n,), r)
         self.partials = DeviceArray.zeros((3 * self.table.num_tiles,), np.float64)
+        # the convergence check reduces over the member table's tiles of the running members
+        # (its own map: a strip launch map is cut differently)
+        self.check_map = DeviceArray.from_numpy(self.table.tile_map_host.reshape(-1))
+        self.check_tiles = self.table.num_tiles
         self.sums = DeviceArray.zeros((3 * len(members),), np.float64)
 
     def _load_initial(self):
@@ -148,8 +156,8 @@ class PackedSweep:
     def _density_check(self):
         e = self.engine
         prec = 0 if self.precision == "single" else 1
-        N.call("pgpe_density_delta", e.psi[0].ptr, self.old.ptr, prec, 1, self.table.table_ptr, e.tile_map.ptr,
-               e.num_tiles, self.table.members, self.partials.ptr, self.sums.ptr, 1, self.stream.handle)
+        N.call("pgpe_density_delta", e.psi[0].ptr, self.old.ptr, prec, 1, self.table.table_ptr, self.check_map.ptr,
+               self.check_tiles, self.table.members, self.partials.ptr, self.sums.ptr, 1, self.stream.handle)
         return self.sums.get(self.stream).reshape(-1, 3)
 
     def run(self) -> SweepResult:
@@ -189,7 +197,12 @@ class PackedSweep:
                         wall_at_stop[newly] = last_conv
                     if newly.any() and self.retire:
                         active &= ~newly
-                        self.engine.set_active_tiles(self.table.tiles_of(np.flatnonzero(active)))
+                        running = np.flatnonzero(active)
+                        self.engine.set_active_members(running)
+                        pairs = self.table.tiles_of(running).reshape(-1)
+                        if pairs.size:
+                            self.check_map.view_range(0, (pairs.size,)).upload(pairs, self.stream)
+                        self.check_tiles = pairs.size // 2
                     elif newly.any():
                         active &= ~newly
             prev_norm = norms

@@ -14,6 +14,20 @@ def test_candidates_respect_kernel_limits():
     assert LaunchChoice("step", 64) in candidates("single", 1)
     assert LaunchChoice("step", 64) not in candidates("double", 1)
     assert all(c.form == "stage" for c in candidates("single", 2))
+    assert LaunchChoice("strip", 32) in candidates("single", 1)
+    assert LaunchChoice("strip", 32) not in candidates("double", 1)
+    assert LaunchChoice("strip", 24) in candidates("double", 1)
+
+
+def test_choice_geometry():
+    """Row strips run the whole-step kernel on a member table of 16-row tiles (the reductions'
+    tiles) with their own strip height; the other forms use their height for the table."""
+    s = LaunchChoice("strip", 24)
+    assert s.whole_step and s.table_h == 16 and s.strip_h == 24
+    t = LaunchChoice("step", 32)
+    assert t.whole_step and t.table_h == 32 and t.strip_h is None
+    g = LaunchChoice("stage", 8)
+    assert not g.whole_step and g.table_h == 8 and g.strip_h is None
 
 
 def test_choice_is_fastest_and_persisted(tmp_path):

@@ -1,4 +1,5 @@
-"""Whole-step RK4 kernel (pgpe_rk4_step) against the four stage launches, bit for bit."""
+"""Whole-step RK4 kernel (pgpe_rk4_step), on LDS tiles and on register row strips, against the
+four stage launches, bit for bit."""
 import numpy as np
 import pytest
 
@@ -8,7 +9,7 @@ SIZES = [(64, 64), (96, 80), (200, 72), (128, 128), (8, 40)]
 
 
 def _engines(sizes, reservoir, precision, tile_h, noise_strength=0.0, laplacian="five-point",
-             boundary="periodic"):
+             boundary="periodic", strip_h=None):
     from polarism_amd.config import PhysicsConstants
     from polarism_amd.device import DeviceArray, init
     from polarism_amd.engine import Rk4Engine, real_dtype
@@ -31,7 +32,7 @@ def _engines(sizes, reservoir, precision, tile_h, noise_strength=0.0, laplacian=
     stage_tab = MemberTable(specs, 16 if tile_h == 64 else tile_h)
     step_tab = MemberTable(specs, tile_h)
     e_stage = Rk4Engine(stage_tab, pump, precision, reservoir)
-    e_step = Rk4Engine(step_tab, pump, precision, reservoir, whole_step=True)
+    e_step = Rk4Engine(step_tab, pump, precision, reservoir, whole_step=True, strip_h=strip_h)
     total = stage_tab.total_points
     psi = (0.3 * (rng.standard_normal(total) + 1j * rng.standard_normal(total))).astype(e_stage.cdtype)
     res = [rng.uniform(0.0, 3.0, total).astype(r) for _ in range(e_stage.nres)]
@@ -103,3 +104,54 @@ def test_whole_step_rejects_aliased_reservoir():
     with pytest.raises(RuntimeError, match="alias"):
         e_step.launch_step(e_step.psi[0], e_step.psi[1], e_step.coef_ptrs(coef, 0), False, 0,
                            res_in=e_step.res, res_out=e_step.res)
+
+
+# ---------------------------------------------------------------- row strips
+@pytest.mark.parametrize("reservoir", ["none", "single", "double", "quadratic-double"])
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_strips_equal_stage_launches_bitwise(reservoir, precision):
+    """Sizes include a grid narrower than one strip (8 columns: the lanes past it hold wrapped
+    columns) and widths that leave a partial last strip (200 = 3 x 56 + 32)."""
+    e_stage, e_strip, coef = _engines(SIZES, reservoir, precision, 16, strip_h=16)
+    assert e_strip.capacity == sum(-(-nx // 56) * -(-ny // 16) for nx, ny in SIZES)
+    for e in (e_stage, e_strip):
+        e.run_steps(5, coef, True, False, 0, graphs=False)
+    _compare(e_stage, e_strip)
+
+
+@pytest.mark.parametrize("strip_h,precision", [(24, "single"), (32, "single"), (24, "double")])
+def test_strip_heights_and_graph_chunks(strip_h, precision):
+    """Graph-captured chunks plus a trailing single step, Wigner noise fused in."""
+    e_stage, e_strip, coef = _engines(SIZES, "single", precision, 16, noise_strength=1.0, strip_h=strip_h)
+    for e in (e_stage, e_strip):
+        e.run_steps(7, coef, True, True, 3, graphs=True)
+    _compare(e_stage, e_strip)
+
+
+@pytest.mark.parametrize("laplacian", ["five-point", "isotropic-9pt"])
+@pytest.mark.parametrize("boundary", ["periodic", "closed-interval"])
+def test_strip_stencils_and_boundaries(laplacian, boundary):
+    e_stage, e_strip, coef = _engines(SIZES, "double", "double", 16, laplacian=laplacian, boundary=boundary,
+                                      strip_h=24)
+    for e in (e_stage, e_strip):
+        e.run_steps(6, coef, True, False, 0, graphs=True)
+    _compare(e_stage, e_strip, rtol=0.0 if boundary == "periodic" else 1e-13)
+
+
+def test_strips_with_retired_members():
+    """Retiring members rewrites the strip map in place; the retired members stay untouched."""
+    e_stage, e_strip, coef = _engines(SIZES, "double", "single", 16, strip_h=24)
+    for e in (e_stage, e_strip):
+        e.run_steps(2, coef, True, False, 0, graphs=True)
+        e.set_active_members([0, 2, 4])
+        e.run_steps(4, coef, True, False, 2, graphs=True)
+    _compare(e_stage, e_strip)
+
+
+def test_strip_options_validated():
+    from polarism_amd.engine import Rk4Engine
+    e_stage, _, _ = _engines(SIZES[:1], "single", "double", 16)
+    with pytest.raises(ValueError, match="strip_h"):
+        Rk4Engine(e_stage.table, e_stage.pump, "double", "single", whole_step=True, strip_h=32)
+    with pytest.raises(ValueError, match="whole-step"):
+        Rk4Engine(e_stage.table, e_stage.pump, "double", "single", strip_h=16)

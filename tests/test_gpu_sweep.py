@@ -77,16 +77,17 @@ def test_whole_step_sweep_equals_stage_sweep_bitwise():
     members = _members(SIZES[:5], reservoir="double")
     from polarism_amd.sweep import PackedSweep
     runs = []
-    for form, tile_h in (("stage", 16), ("step", 32)):
+    for form, tile_h in (("stage", 16), ("step", 32), ("strip", 24)):
         sw = PackedSweep(members, reservoir="double", dt=0.05, check_every=0.5, rtol=1e-3, max_time=3.0,
                          tile_h=tile_h, form=form)
         runs.append(sw.run())
-    a, b = runs
-    np.testing.assert_array_equal(a.stop_times, b.stop_times)
-    for i in range(len(members)):
-        np.testing.assert_array_equal(a.final_psi[i], b.final_psi[i])
-        for x, y in zip(a.final_res[i], b.final_res[i]):
-            np.testing.assert_array_equal(x, y)
+    a = runs[0]
+    for b in runs[1:]:
+        np.testing.assert_array_equal(a.stop_times, b.stop_times)
+        for i in range(len(members)):
+            np.testing.assert_array_equal(a.final_psi[i], b.final_psi[i])
+            for x, y in zip(a.final_res[i], b.final_res[i]):
+                np.testing.assert_array_equal(x, y)
 
 
 def test_autotuner_picks_a_valid_form_and_caches_it(tmp_path):
