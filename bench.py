@@ -85,7 +85,7 @@ def summarise(res, extra_s=0.0):
             "effective_GBps_172B_model": round(res.point_steps * 172.0 / res.wall_s / 1e9, 1)}
 
 
-def _member_engine(n, tile_h, whole_step, reservoir="single"):
+def _member_engine(n, tile_h, whole_step, reservoir="single", strip_h=None):
     from polarism_amd.config import PhysicsConstants
     from polarism_amd.device import DeviceArray
     from polarism_amd.engine import Rk4Engine
@@ -93,7 +93,7 @@ def _member_engine(n, tile_h, whole_step, reservoir="single"):
     p = PhysicsConstants()
     tab = MemberTable([MemberSpec(n, n, 0.5, 0.5, p, reservoir=reservoir, dt=0.05)], tile_h)
     pump = DeviceArray.from_numpy(np.full(n * n, 0.01, dtype=np.float32))
-    e = Rk4Engine(tab, pump, "single", reservoir, whole_step=whole_step)
+    e = Rk4Engine(tab, pump, "single", reservoir, whole_step=whole_step, strip_h=strip_h)
     e.set_dt(0.05)
     rng = np.random.default_rng(0)
     e.psi[0].upload((1e-3 * (rng.standard_normal(n * n) + 1j * rng.standard_normal(n * n))).astype(np.complex64))
@@ -120,7 +120,8 @@ def _time_launches(e, launch, reps):
 
 def sweep_step_kernel(reps=20):
     """Average duration of the whole-step kernel on the headline workload's member table (all 256
-    members of 256^2 active, complex64, single reservoir, the autotuner's tile height), timed with
+    members of 256^2 active, complex64, single reservoir, the autotuner's whole-step form: register
+    row strips or LDS tiles, and its height), timed with
     HIP events on the engine's stream around ``reps`` direct launches.  Algorithmic bytes per
     point-step: psi0 (8), n0 (4), P (4) in, psi1 (8), n1 (4) out = 28 B."""
     from polarism_amd.autotune import tune_rk4
@@ -135,10 +136,11 @@ def sweep_step_kernel(reps=20):
                                 dt=SWEEP["dt"]))
         off += m.nx * m.ny
     choice = tune_rk4(specs, "single", "single")
-    th = choice.tile_h if choice.whole_step else 32
-    tab = MemberTable(specs, th)
+    if not choice.whole_step:
+        choice = type(choice)("step", 32)
+    tab = MemberTable(specs, choice.table_h)
     pump = DeviceArray.from_numpy(np.concatenate([m.envelope.reshape(-1) for m in members]).astype(np.float32))
-    e = Rk4Engine(tab, pump, "single", "single", whole_step=True)
+    e = Rk4Engine(tab, pump, "single", "single", whole_step=True, strip_h=choice.strip_h)
     e.set_dt(SWEEP["dt"])
     rng = np.random.default_rng(0)
     e.psi[0].upload((0.1 * (rng.standard_normal(off) + 1j * rng.standard_normal(off))).astype(np.complex64))
@@ -146,7 +148,13 @@ def sweep_step_kernel(reps=20):
     ptrs = e.coef_ptrs(coef, 0)
     r0, r1 = e.res, e._res_alt
     ms = _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0, res_in=r0, res_out=r1), reps)
-    return ms, off, th
+    if choice.form == "strip":
+        name = f"rk4_strip_kernel<float, 1, {(choice.tile_h + 8) // 4}>"
+        shape = f"row strips of 56 x {choice.tile_h} output points"
+    else:
+        name = f"rk4_step_kernel<float, 1, {choice.tile_h}>"
+        shape = f"LDS tiles of 64 x {choice.tile_h} output points"
+    return ms, off, name, shape, e.capacity
 
 
 def roofline_leg(n=8192, reps=20):
@@ -161,19 +169,19 @@ def roofline_leg(n=8192, reps=20):
         whole = _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0), reps)
         return mid, whole
 
-    def step_kernel_ms(th):
-        e, coef = _member_engine(n, th, True)
+    def step_kernel_ms(th, strip_h=None):
+        e, coef = _member_engine(n, th, True, strip_h=strip_h)
         ptrs = e.coef_ptrs(coef, 0)
         r0, r1 = e.res, e._res_alt
         return _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0, res_in=r0, res_out=r1),
                               reps)
 
-    ms, points, th = sweep_step_kernel(reps)
+    ms, points, name, shape, blocks = sweep_step_kernel(reps)
     per_launch = 28.0 * points
     achieved = per_launch / (ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"rk4_step_kernel<float, SINGLE, {th}>", "workload": "256 members of 256x256 (sweep table)",
+            "kernel": name, "workload": f"256 members of 256x256 (sweep table), {shape}, {blocks} workgroups",
             "avg_us": round(ms * 1e3, 2), "algorithmic_bytes_per_launch": int(per_launch),
             "bytes_per_point_step": 28, "point_steps_per_s": round(points / (ms * 1e-3), 1),
             "effective_GBps_172B_per_stage_model": round(172.0 * points / (ms * 1e-3) / 1e9, 1)}
@@ -188,11 +196,12 @@ def roofline_leg(n=8192, reps=20):
         "avg_us_by_tile_h": {str(k): round(v[0] * 1e3, 2) for k, v in stage.items()},
         "four_stage_step_us": round(step_ms * 1e3, 2)}
     ws_ = {}
-    for th_ in (32, 64):
-        t = step_kernel_ms(th_)
-        ws_[f"tile_h_{th_}"] = {"avg_us": round(t * 1e3, 2), "point_steps_per_s": round(pts / (t * 1e-3), 1),
-                                "frac_of_hbm_peak_28B": round(28.0 * pts / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                                "effective_GBps_172B_per_stage_model": round(172.0 * pts / (t * 1e-3) / 1e9, 1)}
+    for label, th_, sh in (("tile_h_32", 32, None), ("tile_h_64", 64, None), ("strip_h_24", 16, 24),
+                           ("strip_h_32", 16, 32)):
+        t = step_kernel_ms(th_, sh)
+        ws_[label] = {"avg_us": round(t * 1e3, 2), "point_steps_per_s": round(pts / (t * 1e-3), 1),
+                      "frac_of_hbm_peak_28B": round(28.0 * pts / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "effective_GBps_172B_per_stage_model": round(172.0 * pts / (t * 1e-3) / 1e9, 1)}
     roof["step_kernel_8192"] = ws_
     return roof
 
@@ -563,7 +572,7 @@ def sensitivity_sample(members, cold, passed, count=8, window=200.0):
     out = {"members": [int(i) for i in idx], "window_ps": window}
     for name, prec, dt in (("fp64", "double", SWEEP["dt"]), ("half_dt", "single", SWEEP["dt"] / 2)):
         sw = PackedSweep(sub, reservoir="single", precision=prec, dt=dt, check_every=SWEEP["check_every"],
-                         rtol=SWEEP["rtol"], max_time=window, tile_h=16, form="step")
+                         rtol=SWEEP["rtol"], max_time=window)
         res = sw.run()
         errs = [density_error(res.final_psi[k], cold.final_psi[i]) for k, i in enumerate(idx)]
         out[name] = {"converged": int(np.isfinite(res.stop_times).sum()),

@@ -187,7 +187,9 @@ typedef struct pgpe_rk4_args {
                                         table; 1 = row strips of 56 output columns x tile_y rows
                                         (16, 24, or 32 in single precision), local index
                                         (row * ceil(nx / 56) + column), from the strip map */
-    int32_t pad1;
+    int32_t no_eta;                  /* pgpe_rk4_step with strips only: 1 promises that no member
+                                        of the table has a kinetic-relaxation term (eta = 0), so
+                                        the kernel leaves it out; 0 = general */
 } pgpe_rk4_args;
 
 int pgpe_rk4_stage(const pgpe_rk4_args* args, void* stream);

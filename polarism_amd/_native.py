@@ -62,7 +62,7 @@ class Rk4Args(C.Structure):
         ("kres_out", vp * 2), ("res_out", vp * 2),
         ("pump", vp), ("coef", vp),
         ("step", C.c_uint64), ("step_base", vp),
-        ("potential", vp), ("active_tiles", vp), ("strips", C.c_int32), ("pad1", C.c_int32),
+        ("potential", vp), ("active_tiles", vp), ("strips", C.c_int32), ("no_eta", C.c_int32),
     ]
 
 

@@ -117,6 +117,17 @@ __device__ __forceinline__ void st_stream_c(c2<T>* p, const c2<T>& v) {
 
 template <int RES> struct ResFields { static constexpr int n = (RES == 0) ? 0 : (RES == 1 ? 1 : 2); };
 
+// A value known to be equal in every lane, read from the first active lane into scalar registers.
+__device__ __forceinline__ float uniform_value(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double uniform_value(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // Member constants of the scalar right-hand side in the working precision.
 template <typename T>
 struct ScalarConsts {
@@ -125,6 +136,12 @@ struct ScalarConsts {
         : inv_dx2((T)mb.inv_dx2), inv_dy2((T)mb.inv_dy2), kin((T)mb.kin), inv_hbar((T)mb.inv_hbar),
           gC((T)mb.g_C), gR((T)mb.g_R), gI((T)mb.g_I), R((T)mb.R), gammaC((T)mb.gamma_C), eta_c((T)mb.eta),
           hb2m((T)mb.hbar_over_2m) {}
+    // The constants are converted from the member row's doubles by vector instructions, so they
+    // land in vector registers; a kernel short of them moves the (uniform) values to scalar ones.
+    __device__ __forceinline__ void to_scalar_registers() {
+        T* f[] = {&inv_dx2, &inv_dy2, &kin, &inv_hbar, &gC, &gR, &gI, &R, &gammaC, &eta_c, &hb2m};
+        for (T* v : f) *v = uniform_value(*v);
+    }
 };
 
 // Neighbour index along one axis: periodic wrap, or the closed interval's
@@ -177,7 +194,7 @@ __device__ __forceinline__ c2<T> lap9(const c2<T>& ps, const c2<T>& pn, const c2
 // an optional complex potential V in (V + gC|psi|^2 + gR nA + gI nI) psi.
 // Both the per-stage and the whole-step kernels call this, so they perform
 // the same operations in the same order.
-template <typename T, int RES>
+template <typename T, int RES, bool ETA = true>
 __device__ __forceinline__ void scalar_rhs(const pgpe_member& mb, const ScalarConsts<T>& k, const c2<T>& ps,
                                            const c2<T>& lap, const T n[2], T P, const c2<T>* V, c2<T>& d,
                                            T dn[2]) {
@@ -192,7 +209,8 @@ __device__ __forceinline__ void scalar_rhs(const pgpe_member& mb, const ScalarCo
     const T gain = (k.R * nA - k.gammaC) * (T)0.5;
     T Xr, Xi;
     if (V == nullptr) {
-        const T eff = (k.gC * rho + k.gR * nA) + k.gI * nI;
+        // terms of reservoir fields the model does not have are left out (they are +-0)
+        const T eff = NF > 1 ? (k.gC * rho + k.gR * nA) + k.gI * nI : NF > 0 ? k.gC * rho + k.gR * nA : k.gC * rho;
         Xr = k.kin * lap.x + eff * ps.x;
         Xi = k.kin * lap.y + eff * ps.y;
     } else {
@@ -203,7 +221,7 @@ __device__ __forceinline__ void scalar_rhs(const pgpe_member& mb, const ScalarCo
     }
     d.x = k.inv_hbar * Xi + gain * ps.x;
     d.y = -(k.inv_hbar * Xr) + gain * ps.y;
-    if (k.eta_c != (T)0) {
+    if (ETA && k.eta_c != (T)0) {
         const T s = (k.eta_c * nA) * k.hb2m;
         d.x = d.x + s * lap.x;
         d.y = d.y + s * lap.y;
@@ -717,12 +735,30 @@ __device__ __forceinline__ double lane_prev(double v) {
     const uint32_t lo = (uint32_t)lane_prev_i((int)(uint32_t)u), hi = (uint32_t)lane_prev_i((int)(uint32_t)(u >> 32));
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
+// The Laplacians of lap5 / lap9 on lane-held rows, bit for bit: 2 psi is exact, so pe - 2 psi
+// rounds once whether it is evaluated as an FMA (as the compiler contracts lap5) or as the add
+// pe + (-2 psi), and the add lets the compiler fold each wave shift into its consumer
+// (v_add_f32_dpp) instead of a separate move.  The final sum is written as the FMA that lap5 /
+// lap9 contract to.
+template <typename T>
+__device__ __forceinline__ T strip_lap5(T ps, T pn, T pso, T inv_dx2, T inv_dy2) {
+    const T m2 = (T)-2 * ps;
+    const T ex = lane_prev(ps) + (lane_next(ps) + m2);
+    const T ey = (pn + m2) + pso;
+    return __builtin_fma(ey, inv_dy2, ex * inv_dx2);
+}
+__device__ __forceinline__ float strip_lap5(float ps, float pn, float pso, float inv_dx2, float inv_dy2) {
+    const float m2 = -2.0f * ps;
+    const float ex = lane_prev(ps) + (lane_next(ps) + m2);
+    const float ey = (pn + m2) + pso;
+    return __builtin_fmaf(ey, inv_dy2, ex * inv_dx2);
+}
 template <typename T>
 __device__ __forceinline__ c2<T> lane_next_c(const c2<T>& v) { return make_c<T>(lane_next(v.x), lane_next(v.y)); }
 template <typename T>
 __device__ __forceinline__ c2<T> lane_prev_c(const c2<T>& v) { return make_c<T>(lane_prev(v.x), lane_prev(v.y)); }
 
-template <typename T, int RES, int R>
+template <typename T, int RES, int R, bool ETA>
 __global__ void __launch_bounds__(256) rk4_strip_kernel(const Rk4StepParams<T> p) {
 #pragma clang fp contract(on)
     constexpr int NF = ResFields<RES>::n;
@@ -743,12 +779,13 @@ __global__ void __launch_bounds__(256) rk4_strip_kernel(const Rk4StepParams<T> p
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
     const int64_t off = mb.off;
-    const ScalarConsts<T> kc(mb);
+    ScalarConsts<T> kc(mb);
+    kc.to_scalar_registers();
     const bool nine = mb.laplacian == PGPE_LAP_NINE;
-    const T inv_6h2 = kc.inv_dx2 / (T)6;
+    const T inv_6h2 = uniform_value(kc.inv_dx2 / (T)6);
     T coef[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) coef[s] = p.coef[s * p.members + m];
+    for (int s = 0; s < 4; ++s) coef[s] = uniform_value(p.coef[s * p.members + m]);
 
     const int gx = map_index(x0 - kStepHalo + lane, nx, mb.boundary);
     const bool periodic = mb.boundary != PGPE_BC_CLOSED;
@@ -789,19 +826,19 @@ __global__ void __launch_bounds__(256) rk4_strip_kernel(const Rk4StepParams<T> p
             const c2<T> ps = cur[j];
             const c2<T> pn = j + 1 < R ? cur[j + 1] : above;
             const c2<T> pso = south;
-            const c2<T> pe = lane_next_c<T>(ps), pw = lane_prev_c<T>(ps);
             c2<T> lap;
             if (nine) {
-                lap = lap9<T>(ps, pn, pso, pe, pw, lane_prev_c<T>(pso), lane_next_c<T>(pso), lane_prev_c<T>(pn),
-                              lane_next_c<T>(pn), inv_6h2);
+                lap = lap9<T>(ps, pn, pso, lane_next_c<T>(ps), lane_prev_c<T>(ps), lane_prev_c<T>(pso),
+                              lane_next_c<T>(pso), lane_prev_c<T>(pn), lane_next_c<T>(pn), inv_6h2);
             } else {
-                lap = lap5<T>(ps, pn, pso, pe, pw, kc.inv_dx2, kc.inv_dy2);
+                lap.x = strip_lap5(ps.x, pn.x, pso.x, kc.inv_dx2, kc.inv_dy2);
+                lap.y = strip_lap5(ps.y, pn.y, pso.y, kc.inv_dx2, kc.inv_dy2);
             }
             south = ps;
             const T P = env[j] * coef[s];
             c2<T> d;
             T dn[2];
-            scalar_rhs<T, RES>(mb, kc, ps, lap, ns[j], P, nullptr, d, dn);
+            scalar_rhs<T, RES, ETA>(mb, kc, ps, lap, ns[j], P, nullptr, d, dn);
             if (s == 0) {
                 acc[j] = d;
 #pragma unroll
@@ -850,7 +887,10 @@ __global__ void __launch_bounds__(256) rk4_strip_kernel(const Rk4StepParams<T> p
 
 template <typename T, int RES, int R>
 static int launch_strip_r(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
-    hipLaunchKernelGGL((rk4_strip_kernel<T, RES, R>), dim3(xcd_grid(a->num_tiles)), dim3(256), 0, s, p);
+    if (a->no_eta)
+        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, R, false>), dim3(xcd_grid(a->num_tiles)), dim3(256), 0, s, p);
+    else
+        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, R, true>), dim3(xcd_grid(a->num_tiles)), dim3(256), 0, s, p);
     PGPE_LAUNCHED("rk4_strip_kernel");
     return PGPE_OK;
 }

@@ -120,6 +120,7 @@ class Rk4Engine:
         a.active_tiles = self.active.ptr
         a.tile_y = tab.tile_h if self.strip_h is None else self.strip_h
         a.strips = 0 if self.strip_h is None else 1
+        a.no_eta = 1 if all(r.eta == 0.0 for r in tab.rows) else 0
         a.members = tab.members
         a.components = self.components
         a.res_components = self.res_components
