@@ -104,18 +104,20 @@ def _member_engine(n, tile_h, whole_step, reservoir="single", strip_h=None):
 
 
 def _time_launches(e, launch, reps):
+    """Average duration of ``reps`` back-to-back launches, from HIP events on the launch stream.
+    One launch is queued ahead of the start event, so the host's launch latency (the ctypes
+    call) overlaps a running kernel instead of opening a gap inside the timed region."""
     from polarism_amd.device import Event
     launch()
     e.stream.synchronize()
     start, stop = Event(), Event()
-    total = 0.0
+    launch()
+    start.record(e.stream)
     for _ in range(reps):
-        start.record(e.stream)
         launch()
-        stop.record(e.stream)
-        e.stream.synchronize()
-        total += start.elapsed_ms(stop)
-    return total / reps
+    stop.record(e.stream)
+    e.stream.synchronize()
+    return start.elapsed_ms(stop) / reps
 
 
 def sweep_step_kernel(reps=20):
@@ -181,6 +183,8 @@ def roofline_leg(n=8192, reps=20):
     achieved = per_launch / (ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "traffic_note": "no counters in this run; FETCH_SIZE / WRITE_SIZE of the same launches are in "
+                            "profiles/rk4_strip_kernel_sweep256.md",
             "kernel": name, "workload": f"256 members of 256x256 (sweep table), {shape}, {blocks} workgroups",
             "avg_us": round(ms * 1e3, 2), "algorithmic_bytes_per_launch": int(per_launch),
             "bytes_per_point_step": 28, "point_steps_per_s": round(points / (ms * 1e-3), 1),
@@ -246,10 +250,10 @@ def uneven_leg(steps=2000, seed=4):
         envs.append(np.exp(-0.5 * (x[None, :] ** 2 + x[:, None] ** 2) / sig ** 2).ravel())
         off += nn * nn
     choice = tune_rk4(specs, "single", "double")
-    tab = MemberTable(specs, choice.tile_h)
+    tab = MemberTable(specs, choice.table_h)
     pump = DeviceArray.from_numpy(np.concatenate(envs).astype(np.float32))
     coef = DeviceArray.from_numpy(np.tile(rng.uniform(0.5, 3.0, 256) * pth, (4, 1)).astype(np.float32))
-    e = Rk4Engine(tab, pump, "single", "double", whole_step=choice.whole_step)
+    e = Rk4Engine(tab, pump, "single", "double", whole_step=choice.whole_step, strip_h=choice.strip_h)
     e.set_dt(1e-3)
     e.psi[0].upload((1e-3 * (rng.standard_normal(off) + 1j * rng.standard_normal(off))).astype(np.complex64))
     e.run_steps(1000, coef, True, False, 0)
@@ -268,6 +272,37 @@ def uneven_leg(steps=2000, seed=4):
             "wall_ms": round(ms, 1), "point_steps_per_s": round(pt_steps / (ms * 1e-3), 1),
             "point_stage_updates_per_s": round(4 * pt_steps / (ms * 1e-3), 1),
             "effective_GBps_224B_model": round(224.0 * pt_steps / (ms * 1e-3) / 1e9, 1)}
+
+
+def quickstart_leg(steps=2000):
+    """The reference quickstart on the GPU: rk4-fdm-hip, 128^2 periodic, lx = ly = 100 um,
+    continuous Gaussian pump P0 0.02 / sigma 12 um, single reservoir, complex128, dt 1e-3 ps,
+    2000 steps (2 ps) after one untimed pass that captures the graphs."""
+    from polarism_amd import Config, SimulationController
+    from polarism_amd.config import GridParameters
+    from polarism_amd.device import Event
+    cfg = Config()
+    cfg.grid = GridParameters(nx=128, ny=128, lx=100.0, ly=100.0)
+    cfg.laser.laser_type = "continuous-gaussian"
+    cfg.laser.P0 = 0.02
+    cfg.laser.sigma_space = 12.0
+    cfg.reservoir.reservoir_type = "single"
+    cfg.solver.method = "rk4-fdm-hip"
+    cfg.solver.dt = 1e-3
+    cfg.solver.total_time = steps * 1e-3
+    cfg.solver.precision = "double"
+    ctl = SimulationController(cfg)
+    sol = ctl.solver
+    sol.advance(ctl.model, ctl.state, steps)
+    stream = sol.stream
+    stream.synchronize()
+    start, stop = Event(), Event()
+    start.record(stream)
+    sol.advance(ctl.model, ctl.state, steps)
+    stop.record(stream)
+    stream.synchronize()
+    ms = start.elapsed_ms(stop)
+    return {"steps": steps, "steps_per_s": round(steps / (ms * 1e-3), 1), "step_us": round(ms * 1e3 / steps, 2)}
 
 
 def fft_leg(steps=1000):
@@ -323,19 +358,86 @@ def _oracle_rate(steps):
     return steps * g.nx * g.ny / (time.perf_counter() - t0)
 
 
-def cpu_worker(steps, procs):
-    """Runs in a child process that never touches the GPU: one core, then a pool of ``procs``."""
-    import multiprocessing as mp
-    single = _oracle_rate(steps)
-    with mp.get_context("fork").Pool(procs) as pool:
+def _median_of(fn, repeats=3):
+    vals = [fn() for _ in range(repeats)]
+    return float(np.median(vals)), vals
+
+
+def _oracle_configs():
+    """The oracle (the reference's NumPy algorithms) on one core, each a median of three timings:
+    the quickstart rk4-fdm (128^2 complex128, single reservoir), the split-step run with Wigner
+    noise (512^2 complex64, CPU operator order) and the spinor Lawson DP5 trial (1024^2 complex64,
+    double reservoir per component, ring pump)."""
+    from oracle import gpe_oracle as O
+    out = {}
+
+    g = O.Grid(128, 128, 100.0, 100.0)
+    p = O.Physics()
+    env = O.gaussian_profile(g, 12.0)
+    psi0 = O.initial_psi(g.shape, 1e-3, 0, "double").reshape(1, 1, *g.shape)
+
+    def c1(steps=100):
+        psi, res = psi0, (np.zeros((1, 1, *g.shape)),)
         t0 = time.perf_counter()
-        rates = pool.map(_oracle_rate, [steps] * procs)
-        wall = time.perf_counter() - t0
-    pooled = procs * steps * SWEEP["n"] ** 2 / wall
-    print(json.dumps({"single": single, "pooled": pooled, "per_proc": rates, "procs": procs}))
+        for k in range(steps):
+            psi, res = O.rk4_fdm_step("single", p, g, psi, res, lambda t: env * 0.02, k * 1e-3, 1e-3)
+        return steps / (time.perf_counter() - t0)
+    med, vals = _median_of(c1)
+    out["quickstart_rk4_128_fp64"] = {"steps_per_s": round(med, 2), "repeats": [round(v, 2) for v in vals]}
+
+    g2 = O.Grid(512, 512, 200.0, 200.0)
+    env2 = O.gaussian_profile(g2, 20.0).astype(np.float32)
+    psi2 = O.initial_psi(g2.shape, 1e-3, 1234, "single").reshape(1, 1, *g2.shape)
+    nz = dict(seeds=[1234], gamma=[p.gamma_C], scattering=[p.R], strength=[1.0], volume=g2.dx * g2.dy,
+              precision="single")
+
+    def c2(steps=5):
+        psi, res = psi2, (np.zeros((1, 1, *g2.shape), np.float32),)
+        t0 = time.perf_counter()
+        for k in range(steps):
+            psi, res = O.split_step_cpu("single", p, g2, psi, res, lambda t: env2 * np.float32(0.0415),
+                                        k * 5e-3, 5e-3, k, nz)
+        return steps / (time.perf_counter() - t0)
+    med, vals = _median_of(c2)
+    out["split_step_512_fp32_noise"] = {"steps_per_s": round(med, 3), "repeats": [round(v, 3) for v in vals]}
+
+    g3 = O.Grid(1024, 1024, 300.0, 300.0)
+    p3 = O.Physics(te_tm=0.05, zeeman=0.02, alpha_2=-0.1 * O.Physics().g_C)
+    env3 = O.ring_profile(g3, 15.0, 3.0).astype(np.float32)
+    weights = np.asarray([0.75, 0.25], dtype=np.float32).reshape(1, 2, 1, 1)
+    prob = O.LawsonProblem("double", p3, g3, precision="single", tableau="dp5", adaptive=True, rtol=1e-5,
+                           atol=1e-8, pump_weights=weights, reservoir_components=2)
+    psi3 = np.stack((O.initial_psi(g3.shape, 1e-3, 3, "single"), O.initial_psi(g3.shape, 1e-3, 4, "single")))[None]
+    psi3k = np.fft.fft2(psi3, axes=(-2, -1)).astype(np.complex64)
+    res3 = tuple(np.zeros((1, 2) + g3.shape, np.float32) for _ in range(2))
+
+    def c3(trials=2):
+        t0 = time.perf_counter()
+        for _ in range(trials):
+            prob.step(psi3k, res3, lambda t: env3 * np.float32(0.1), 0.0, 0.05)
+        return trials / (time.perf_counter() - t0)
+    med, vals = _median_of(c3)
+    out["lawson_dp5_spinor_1024_fp32"] = {"trials_per_s": round(med, 4), "repeats": [round(v, 4) for v in vals]}
+    return out
 
 
-def cpu_baseline(point_steps, converged, steps=400):
+def cpu_worker(steps, procs):
+    """Runs in a child process that never touches the GPU: the sweep's rk4-fdm member step on one
+    core and on a pool of ``procs`` (one member per process), each a median of three timings, then
+    the other configurations on one core."""
+    import multiprocessing as mp
+    single, singles = _median_of(lambda: _oracle_rate(steps))
+    pooled_runs = []
+    with mp.get_context("fork").Pool(procs) as pool:
+        for _ in range(3):
+            t0 = time.perf_counter()
+            pool.map(_oracle_rate, [steps] * procs)
+            pooled_runs.append(procs * steps * SWEEP["n"] ** 2 / (time.perf_counter() - t0))
+    print(json.dumps({"single": single, "singles": singles, "pooled": float(np.median(pooled_runs)),
+                      "pooled_runs": pooled_runs, "procs": procs, "configs": _oracle_configs()}))
+
+
+def cpu_baseline(point_steps, converged, steps=200):
     """The oracle rk4-fdm (the reference's NumPy algorithm) on the host cores, in a child process:
     one member per core, scaled to the plain sweep's point-steps."""
     import subprocess
@@ -355,9 +457,13 @@ def cpu_baseline(point_steps, converged, steps=400):
     return {"value": round(converged / (point_steps / pooled), 6), "unit": "members/s", "cores": procs,
             "kind": "port",
             "sample": f"oracle rk4-fdm (NumPy restatement of rk4_fdm_solver.py), complex64 256x256, {steps} steps "
-                      f"per process on {procs} processes ({pooled / 1e6:.1f} Mpt-step/s pooled, "
-                      f"{single / 1e6:.2f} on one core), scaled to the plain sweep's {point_steps:.3e} point-steps",
+                      f"per process on {procs} processes ({pooled / 1e6:.1f} Mpt-step/s pooled, median of 3; "
+                      f"{single / 1e6:.2f} on one core, median of 3), scaled to the plain sweep's "
+                      f"{point_steps:.3e} point-steps",
+            "pooled_mpt_step_per_s_runs": [round(v / 1e6, 2) for v in r["pooled_runs"]],
+            "single_core_mpt_step_per_s_runs": [round(v / 1e6, 3) for v in r["singles"]],
             "single_core_members_per_s": round(converged / (point_steps / single), 6),
+            "configs_one_core": r["configs"],
             "cpu_model": cpu_model, "affinity_cores": len(os.sched_getaffinity(0))}
 
 
@@ -589,7 +695,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--members", type=int, default=SWEEP["members"])
-    ap.add_argument("--legs", default="sweep,copy,roofline,uneven,fft,ffthbm,lawson,fno,cpu")
+    ap.add_argument("--legs", default="sweep,copy,roofline,uneven,quickstart,fft,ffthbm,lawson,fno,cpu")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--dump", default=None, help="write per-member stop times and density errors to this .npz")
     ap.add_argument("--max-time", type=float, default=SWEEP["max_time"],
@@ -653,6 +759,8 @@ def main():
         result["roofline"] = roofline_leg()
     if "uneven" in legs:
         result["uneven_grid_sweep"] = uneven_leg()
+    if "quickstart" in legs:
+        result["quickstart_rk4_fp64"] = quickstart_leg()
     if "fft" in legs:
         result["split_step_fft"] = fft_leg()
     if "ffthbm" in legs:
@@ -670,6 +778,18 @@ def main():
     if "cpu" in legs and "sweep" in result:
         s = result["sweep"]["plain"]
         result["cpu_baseline"] = cpu_baseline(s["point_steps"], max(s["converged"], 1))
+        cpu, ratios = result["cpu_baseline"]["configs_one_core"], {}
+        if "quickstart_rk4_fp64" in result:
+            ratios["quickstart_rk4_128_fp64"] = round(result["quickstart_rk4_fp64"]["steps_per_s"]
+                                                      / cpu["quickstart_rk4_128_fp64"]["steps_per_s"], 1)
+        if "split_step_fft" in result:
+            ratios["split_step_512_fp32_noise"] = round(result["split_step_fft"]["split-step-fft-hip"]["steps_per_s"]
+                                                        / cpu["split_step_512_fp32_noise"]["steps_per_s"], 1)
+        if "lawson_spinor_dp5" in result:
+            ratios["lawson_dp5_spinor_1024_fp32"] = round(
+                1e3 / result["lawson_spinor_dp5"]["device_controller"]["ms_per_trial"]
+                / cpu["lawson_dp5_spinor_1024_fp32"]["trials_per_s"], 1)
+        result["cpu_baseline"]["gpu_over_one_core"] = ratios
     print(json.dumps(result), flush=True)
 
 
