@@ -24,27 +24,39 @@ Workload (one "step" = one pass over the whole held-out batch, plain and warm):
   the steady density.
 
 Roofline leg: the headline's dominant kernel, the whole-step RK4 kernel (all
-four stages of a step on halo tiles, one launch per step; FP32, single
-reservoir) on the sweep's 256-member table, timed with HIP events on the
-engine's stream.  Its algorithmic bytes are psi0, n0, P in and psi1, n1 out =
-28 B per point-step.  Beside it: the middle RK4 stage kernel on one 8192 x 8192
-member (537 MB per complex field, beyond the 256 MB Infinity Cache; psi0,
-k_prev, n0, kn_prev, P in, k, kn out = 40 B per point) and the whole-step
-kernel on that member.  HBM traffic counters come from separate rocprofv3
---pmc runs (not part of this program), so ``traffic`` is null here.
+four stages of a step in one launch; FP32, single reservoir) in the form the
+autotuner picks for the sweep (register row strips of 56 output columns, or
+LDS halo tiles) on the sweep's 256-member table, timed with HIP events on the
+engine's stream over back-to-back launches.  Its algorithmic bytes are psi0,
+n0, P in and psi1, n1 out = 28 B per point-step.  Beside it: the middle RK4
+stage kernel on one 8192 x 8192 member (537 MB per complex field, beyond the
+256 MB Infinity Cache; psi0, k_prev, n0, kn_prev, P in, k, kn out = 40 B per
+point) and both whole-step forms on that member.  HBM traffic counters come
+from separate rocprofv3 --pmc runs (not part of this program), so ``traffic``
+is null here.
 
 Extra legs (reported beside the headline, not part of value):
-  uneven: 256 members with grids drawn from {64..512}^2, double reservoir,
-          complex64, dt 1e-3 ps, 2000 steps in 1000-step graph chunks
-          (grid-point-stage updates/s).
-  fft:    split-step FFT (CPU operator order, 4 transforms per step) and the
-          K-outer order (2 per step), 512 x 512, single reservoir, Wigner
-          noise seed 1234, dt 5e-3 ps, complex64 (steps/s and FFT GB/s with
-          2 x 8 B per point per transform).
+  uneven:     256 members with grids drawn from {64..512}^2, double reservoir,
+              complex64, dt 1e-3 ps, 2000 steps in 1000-step graph chunks
+              (grid-point-stage updates/s).
+  quickstart: the reference quickstart (rk4-fdm, 128^2, complex128, 2000
+              steps of 1e-3 ps), steps/s.
+  fft:        split-step FFT (CPU operator order, 4 transforms per step) and
+              the K-outer order (2 per step), 512 x 512, single reservoir,
+              Wigner noise seed 1234, dt 5e-3 ps, complex64 (steps/s and FFT
+              GB/s with 2 x 8 B per point per transform).
+  ffthbm:     batched forward 2-D FFT of 16 complex64 2048^2 planes.
+  lawson:     spinor Lawson IFRK DP5, adaptive 1 ps ticks, 1024^2, double
+              reservoir, complex64 (device and host trial controller).
+  fno:        the surrogate's BF16 MFMA GEMM (8192^3) and its forward /
+              training step at batch 16 on 256^2 maps.
 
 cpu_baseline: the NumPy oracle's rk4-fdm step (complex64, one 256 x 256
-member) timed on one host core for a bounded number of steps; its
-point-step rate converts the plain run's point-steps into CPU time.
+member) on one host core and on a pool of up to 16 processes (one member
+each), each rate the median of three timings of a bounded number of steps;
+the pooled rate converts the plain run's point-steps into CPU time.  The
+quickstart, split-step and spinor Lawson configurations are timed with the
+oracle on one core (median of three) and set beside their GPU legs.
 """
 from __future__ import annotations
 

@@ -96,7 +96,8 @@ def test_split_step_with_noise_config_window_matches_oracle(precision, tol):
     The psi tolerance is the reference's CPU-GPU bar (1e-10 FP64, 1e-5 FP32,
     tests/crosscheck/test_cpu_gpu_agreement.py:12-13); the density error of a field is twice
     its relative field error to first order, so the FP32 density bar is 2e-5 (240 FP32
-    transforms leave ~7e-6 on psi against the oracle's own FP32 transforms)."""
+    transforms leave ~7e-6 on psi against the oracle's own FP32 transforms).  The norm (the sum
+    of the density) is held to the plain bar, 1e-5: its pointwise rounding errors average out."""
     cfg = _cfg(grid__nx=512, grid__ny=512, grid__lx=200.0, grid__ly=200.0, laser__sigma_space=20.0,
                laser__P0=2 * 0.02075, solver__precision=precision, solver__total_time=0.3,
                noise__stochastic="wigner", noise__seed=1234)
@@ -104,8 +105,10 @@ def test_split_step_with_noise_config_window_matches_oracle(precision, tol):
     psi_ref, res_ref = _oracle_split(cfg, 60, noise=True)
     e = O.rel_l2(ctl.state.field.get(), psi_ref)
     d = O.rel_l2(np.abs(ctl.state.field.get()) ** 2, np.abs(psi_ref) ** 2)
-    print(f"512^2 split-step + noise {precision}: psi {e:.3e} density {d:.3e}")
-    assert e <= tol and d <= 2 * tol
+    norm, norm_ref = float((np.abs(ctl.state.field.get()) ** 2).sum()), float((np.abs(psi_ref) ** 2).sum())
+    en = abs(norm - norm_ref) / norm_ref
+    print(f"512^2 split-step + noise {precision}: psi {e:.3e} density {d:.3e} norm {en:.3e}")
+    assert e <= tol and d <= 2 * tol and en <= tol
     assert O.rel_l2(ctl.reservoir.get_state()[0].get(), res_ref[0]) <= tol
 
 
