@@ -3,8 +3,8 @@
 For a batch of members it times every launch form the device supports --
 the four fused stage launches (1-point halo, tile heights 8 / 16 / 32) and
 the whole-step launch (4-point halo recomputed on chip, LDS tile heights 16 / 32
-and 64 in single precision, or register row strips 16 / 24 and 32 in single
-precision high) -- on the batch's own member table, and keeps the
+and 64 in single precision, or register row strips 16 / 24 and 32 / 40 in
+single precision high) -- on the batch's own member table, and keeps the
 fastest.  Every form gives the same result (bit for bit on periodic grids,
 to rounding on closed intervals; the kernels share their arithmetic), so the
 choice only changes speed.  Choices are stored in a
@@ -51,7 +51,7 @@ def candidates(precision: str, components: int) -> list[LaunchChoice]:
         out += [LaunchChoice("step", h) for h in (16, 32)]
         out += [LaunchChoice("strip", h) for h in (16, 24)]
         if precision == "single":
-            out += [LaunchChoice("step", 64), LaunchChoice("strip", 32)]
+            out += [LaunchChoice("step", 64), LaunchChoice("strip", 32), LaunchChoice("strip", 40)]
     return out
 
 

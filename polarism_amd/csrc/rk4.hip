@@ -710,9 +710,9 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
 // ------------------------------------------------------------------ whole step, row strips
 // The same four-stage temporal block as rk4_step_kernel with a layout that keeps the stencil's
 // neighbours in registers.  A strip is 64 columns (one per lane of a wavefront: 56 output
-// columns and 4 halo columns each side) by 4R rows (TH = 4R - 8 output rows and 4 halo rows each
-// side); wavefront w of the 256-thread workgroup owns the R consecutive rows [wR, wR + R) of
-// every column, so a point's y neighbours are the same thread's adjacent registers, its x
+// columns and 4 halo columns each side) by NW R rows (TH = NW R - 8 output rows and 4 halo rows
+// each side); wavefront w of the NW-wavefront workgroup owns the R consecutive rows [wR, wR + R)
+// of every column, so a point's y neighbours are the same thread's adjacent registers, its x
 // neighbours the adjacent lanes' (read with DPP wave shifts, no LDS), and only the first and
 // last row of each wavefront cross to the neighbouring wavefronts through LDS, once per stage.
 // Per-point arithmetic is rk4_step_kernel's (lap5 / lap9, scalar_rhs, the same accumulation),
@@ -758,14 +758,14 @@ __device__ __forceinline__ c2<T> lane_next_c(const c2<T>& v) { return make_c<T>(
 template <typename T>
 __device__ __forceinline__ c2<T> lane_prev_c(const c2<T>& v) { return make_c<T>(lane_prev(v.x), lane_prev(v.y)); }
 
-template <typename T, int RES, int R, bool ETA>
-__global__ void __launch_bounds__(256) rk4_strip_kernel(const Rk4StepParams<T> p) {
+template <typename T, int RES, int NW, int R, bool ETA, bool NOISE>
+__global__ void __launch_bounds__(NW * kWave) rk4_strip_kernel(const Rk4StepParams<T> p) {
 #pragma clang fp contract(on)
     constexpr int NF = ResFields<RES>::n;
-    constexpr int EH = 4 * R;
+    constexpr int EH = NW * R;
     constexpr int TH = EH - 2 * kStepHalo;
     // first and last row of each wavefront, double buffered over the stages
-    __shared__ c2<T> edge[2][4][2][kWave];
+    __shared__ c2<T> edge[2][NW][2][kWave];
 
     const int tile = xcd_tile(blockIdx.x, p.active_tiles ? *p.active_tiles : p.num_tiles);
     if (tile < 0) return;
@@ -816,11 +816,11 @@ __global__ void __launch_bounds__(256) rk4_strip_kernel(const Rk4StepParams<T> p
         edge[s & 1][w][0][lane] = cur[0];
         edge[s & 1][w][1][lane] = cur[R - 1];
         __syncthreads();
-        // wavefront 0 has no row below its first and wavefront 3 none above its last: they read
-        // their own edge rows instead, which only feed halo points that are stale by the end of
-        // the step (a plain clamped index: no select between LDS and register values)
+        // the first wavefront has no row below its first and the last none above its last: they
+        // read their own edge rows instead, which only feed halo points that are stale by the end
+        // of the step (a plain clamped index: no select between LDS and register values)
         c2<T> south = edge[s & 1][w > 0 ? w - 1 : 0][1][lane];
-        const c2<T> above = edge[s & 1][w < 3 ? w + 1 : 3][0][lane];
+        const c2<T> above = edge[s & 1][w < NW - 1 ? w + 1 : NW - 1][0][lane];
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             const c2<T> ps = cur[j];
@@ -872,7 +872,7 @@ __global__ void __launch_bounds__(256) rk4_strip_kernel(const Rk4StepParams<T> p
                     n_new[q] = v;
                     p.res_out[q][off + idx] = v;
                 }
-                if (p.noise) {
+                if (NOISE) {
                     uint64_t step = p.step;
                     if (p.step_base) step += *p.step_base;
                     wigner_site<T>(out.x, out.y, mb.seed, (uint32_t)idx, step, 1u, (T)mb.noise_strength,
@@ -885,25 +885,38 @@ __global__ void __launch_bounds__(256) rk4_strip_kernel(const Rk4StepParams<T> p
     }
 }
 
-template <typename T, int RES, int R>
+template <typename T, int RES, int NW, int R>
 static int launch_strip_r(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
-    if (a->no_eta)
-        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, R, false>), dim3(xcd_grid(a->num_tiles)), dim3(256), 0, s, p);
+    // the kinetic-relaxation term and the Wigner increment are compiled in only when used: each
+    // costs registers (the noise generator ~25 per lane), and fewer lift the occupancy
+    const dim3 grid(xcd_grid(a->num_tiles)), block(NW * kWave);
+    if (a->no_eta && !a->noise)
+        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, false, false>), grid, block, 0, s, p);
+    else if (a->no_eta)
+        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, false, true>), grid, block, 0, s, p);
+    else if (!a->noise)
+        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, true, false>), grid, block, 0, s, p);
     else
-        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, R, true>), dim3(xcd_grid(a->num_tiles)), dim3(256), 0, s, p);
+        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, true, true>), grid, block, 0, s, p);
     PGPE_LAUNCHED("rk4_strip_kernel");
     return PGPE_OK;
 }
 
 template <typename T, int RES>
 static int launch_strip_res(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
+    // strip height TH = NW R - 8: wavefronts per workgroup x rows per wavefront, less the halo
     switch (a->tile_y) {
-        case 16: return launch_strip_r<T, RES, 6>(a, p, s);
-        case 24: return launch_strip_r<T, RES, 8>(a, p, s);
+        case 16: return launch_strip_r<T, RES, 4, 6>(a, p, s);
+        case 24: return launch_strip_r<T, RES, 8, 4>(a, p, s);
         case 32:
-            if constexpr (sizeof(T) == 4) return launch_strip_r<T, RES, 10>(a, p, s);
-            return fail(PGPE_ERR_ARG, "pgpe_rk4_step: strip height 32 is single precision only");
-        default: return fail(PGPE_ERR_ARG, "pgpe_rk4_step: strip height (tile_y) must be 16, 24 or 32");
+        case 40:
+            // double precision needs more than the 256 registers a 512-thread workgroup allows
+            if constexpr (sizeof(T) == 4) {
+                if (a->tile_y == 32) return launch_strip_r<T, RES, 8, 5>(a, p, s);
+                return launch_strip_r<T, RES, 8, 6>(a, p, s);
+            }
+            return fail(PGPE_ERR_ARG, "pgpe_rk4_step: strip heights 32 and 40 are single precision only");
+        default: return fail(PGPE_ERR_ARG, "pgpe_rk4_step: strip height (tile_y) must be 16, 24, 32 or 40");
     }
 }
 

@@ -57,8 +57,8 @@ class Rk4Engine:
         if self.strip_h is not None:
             if not self.whole_step:
                 raise ValueError("row strips run the whole-step kernel (whole_step=True)")
-            if self.strip_h not in (16, 24, 32) or (self.strip_h == 32 and precision != "single"):
-                raise ValueError("strip_h must be 16 or 24 (or 32 in single precision)")
+            if self.strip_h not in (16, 24, 32, 40) or (self.strip_h > 24 and precision != "single"):
+                raise ValueError("strip_h must be 16 or 24 (or 32 and 40 in single precision)")
         self.psi = [DeviceArray.empty((n,), c), DeviceArray.empty((n,), c)]
         stage_bufs = 0 if self.whole_step else 3
         self.k = [DeviceArray.empty((n,), c) for _ in range(stage_bufs)]

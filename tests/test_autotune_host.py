@@ -16,6 +16,8 @@ def test_candidates_respect_kernel_limits():
     assert all(c.form == "stage" for c in candidates("single", 2))
     assert LaunchChoice("strip", 32) in candidates("single", 1)
     assert LaunchChoice("strip", 32) not in candidates("double", 1)
+    assert LaunchChoice("strip", 40) in candidates("single", 1)
+    assert LaunchChoice("strip", 40) not in candidates("double", 1)
     assert LaunchChoice("strip", 24) in candidates("double", 1)
 
 

@@ -119,7 +119,7 @@ def test_strips_equal_stage_launches_bitwise(reservoir, precision):
     _compare(e_stage, e_strip)
 
 
-@pytest.mark.parametrize("strip_h,precision", [(24, "single"), (32, "single"), (24, "double")])
+@pytest.mark.parametrize("strip_h,precision", [(24, "single"), (32, "single"), (40, "single"), (24, "double")])
 def test_strip_heights_and_graph_chunks(strip_h, precision):
     """Graph-captured chunks plus a trailing single step, Wigner noise fused in."""
     e_stage, e_strip, coef = _engines(SIZES, "single", precision, 16, noise_strength=1.0, strip_h=strip_h)
@@ -133,6 +133,7 @@ def test_strip_heights_and_graph_chunks(strip_h, precision):
 def test_strip_stencils_and_boundaries(laplacian, boundary):
     e_stage, e_strip, coef = _engines(SIZES, "double", "double", 16, laplacian=laplacian, boundary=boundary,
                                       strip_h=24)
+    assert e_strip.capacity == sum(-(-nx // 56) * -(-ny // 24) for nx, ny in SIZES)
     for e in (e_stage, e_strip):
         e.run_steps(6, coef, True, False, 0, graphs=True)
     _compare(e_stage, e_strip, rtol=0.0 if boundary == "periodic" else 1e-13)
@@ -151,7 +152,8 @@ def test_strips_with_retired_members():
 def test_strip_options_validated():
     from polarism_amd.engine import Rk4Engine
     e_stage, _, _ = _engines(SIZES[:1], "single", "double", 16)
-    with pytest.raises(ValueError, match="strip_h"):
-        Rk4Engine(e_stage.table, e_stage.pump, "double", "single", whole_step=True, strip_h=32)
+    for bad in (32, 40, 20):
+        with pytest.raises(ValueError, match="strip_h"):
+            Rk4Engine(e_stage.table, e_stage.pump, "double", "single", whole_step=True, strip_h=bad)
     with pytest.raises(ValueError, match="whole-step"):
         Rk4Engine(e_stage.table, e_stage.pump, "double", "single", strip_h=16)
