@@ -249,6 +249,17 @@ static void twiddles(int n, std::vector<c2<T>>& out) {
     }
 }
 
+// FP32 tails of the n-th roots: lo = root - (float)root, rounded.
+static void twiddle_tails(int n, std::vector<float2>& out) {
+    out.resize(n);
+    for (int m = 0; m < n; ++m) {
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+        const long double c = cosl(a), s = sinl(a);
+        const float ch = (float)(double)c, sh = (float)(double)s;
+        out[m] = make_c<float>((float)(double)(c - (long double)ch), (float)(double)(s - (long double)sh));
+    }
+}
+
 // k = 2 pi fftfreq(n, d): integer index times 1/(n d), then times 2 pi (periodic_grid.py:66-67).
 static void wavenumbers(int n, double d, std::vector<double>& k) {
     k.resize(n);
@@ -269,7 +280,7 @@ static int launch_rows_v(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hi
     const int rows_per = E / plan->nx;
     const int total_rows = p.planes * p.ny;
     const dim3 grid((total_rows + rows_per - 1) / rows_per);
-    const size_t shm = (size_t)(rows_per * seq_stride<T>(plan->nx) + plan->nx) * sizeof(c2<T>);
+    const size_t shm = (size_t)(rows_per * seq_stride<T>(plan->nx) + tw_slots<T>(plan->nx)) * sizeof(c2<T>);
     auto kern = fft_rows_kernel<T, V, MODE, RES, NT>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -300,7 +311,7 @@ static int launch_cols_v(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hi
     p.tw = (const c2<T>*)plan->tw_y[radix_class<V>()];
     const int cols_per = E / plan->ny;
     const int groups = (plan->nx + cols_per - 1) / cols_per;
-    const size_t shm = (size_t)(cols_per * seq_stride<T>(plan->ny) + plan->ny) * sizeof(c2<T>);
+    const size_t shm = (size_t)(cols_per * seq_stride<T>(plan->ny) + tw_slots<T>(plan->ny)) * sizeof(c2<T>);
     auto kern = fft_cols_kernel<T, V, MODE, NT>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -320,7 +331,7 @@ static int launch_cols_v(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hi
 template <typename T>
 static int column_shape(const pgpe_fft_plan* plan, int planes) {
     const int n = plan->ny;
-    auto lds = [&](int cols) { return (size_t)(cols * seq_stride<T>(n) + n) * sizeof(c2<T>); };
+    auto lds = [&](int cols) { return (size_t)(cols * seq_stride<T>(n) + tw_slots<T>(n)) * sizeof(c2<T>); };
     if (n % 3 == 0) return 256 * 12 >= n && lds(256 * 12 / n) <= 163840 ? 7 : -1;
     const int cols_total = plan->nx * planes;
     int want = sizeof(T) == 4 ? 8 : 4;
@@ -452,7 +463,7 @@ int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pg
         d.npass = (int)lists[k].size();
         d.passes = plan->passes + offs[k];
         void** tw = k < 3 ? &plan->tw_x[k] : &plan->tw_y[k - 3];
-        const size_t slots = n > 1 ? n - 1 : 1;
+        const size_t slots = precision ? (size_t)tw_slots<double>(n) : (size_t)tw_slots<float>(n);
         e = hipMalloc(tw, csz * slots);
         if (e != hipSuccess) break;
         if (precision) {
@@ -461,10 +472,16 @@ int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pg
             pass_twiddles(n, lists[k], root, tab);
             e = hipMemcpy(*tw, tab.data(), csz * slots, hipMemcpyHostToDevice);
         } else {
-            std::vector<float2> root, tab;
+            // heads in slots [0, n - 1), a zero pad, then the tails of slots [0, kTwLo - 1)
+            std::vector<float2> root, tails, tab, tail_tab;
             twiddles<float>(n, root);
+            twiddle_tails(n, tails);
             pass_twiddles(n, lists[k], root, tab);
-            e = hipMemcpy(*tw, tab.data(), csz * slots, hipMemcpyHostToDevice);
+            pass_twiddles(n, lists[k], tails, tail_tab);
+            std::vector<float2> all_slots(slots, make_c<float>(0.0f, 0.0f));
+            for (int j = 0; j < n - 1; ++j) all_slots[j] = tab[j];
+            for (int j = 0; j < n - 1 && j < kTwLo - 1; ++j) all_slots[n + j] = tail_tab[j];
+            e = hipMemcpy(*tw, all_slots.data(), csz * slots, hipMemcpyHostToDevice);
         }
     }
     if (e == hipSuccess) e = hipMalloc((void**)&plan->kx, sizeof(double) * nx);

@@ -11,7 +11,8 @@
 // 2048-point transform: 16, 16, 8), then one radix-3 pass when 3 divides n.
 // Twiddles come from host-computed n-th roots of unity (long double cos/sin
 // rounded once), so the FP64 transform keeps ~1e-16 relative accuracy; the
-// constants inside the radix-8/16 DFTs are cos(pi/8), sin(pi/8) and 1/sqrt(2).
+// constants inside the radix-3/8/16 DFTs are sqrt(3)/2, cos(pi/8), sin(pi/8) and
+// 1/sqrt(2), which FP32 takes as a head and a tail (see kmul).
 //
 // Layout: element k of a sequence sits at k + k/16 (one pad slot per 16
 // elements: a stride-16 scatter of the first radix-16 pass then spreads over
@@ -72,6 +73,40 @@ constexpr int stage_chunk() {
     return V < cap ? V : cap;
 }
 
+// An irrational constant K as head + tail: hi = K rounded to T, lo = (K - hi) rounded (0 in
+// FP64).  FP32 multiplies by both, so a product by K is unbiased.  With the head alone,
+// |1/sqrt(2) (1 - i)|^2 = 2 hi^2 = 1 - 3.4e-8 and cos^2 + sin^2 of pi/8 = 1 - 5.7e-8: every
+// radix-8 pass would shrink the energy by ~1e-8, a loss that compounds over repeated
+// transforms (240 2-D transforms of a 512^2 field lost 1.1e-5 of the norm).
+template <typename T>
+struct KSplit {
+    T hi, lo;
+};
+
+template <typename T>
+__device__ __forceinline__ constexpr KSplit<T> ksplit(double k) {
+    return KSplit<T>{(T)k, (T)(k - (double)(T)k)};
+}
+
+// x K
+template <typename T>
+__device__ __forceinline__ T kmul(T x, KSplit<T> k) {
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(x, k.hi, x * k.lo);
+    else return x * k.hi;
+}
+
+// a x + b y
+template <typename T>
+__device__ __forceinline__ T kdot(KSplit<T> a, T x, KSplit<T> b, T y) {
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(a.hi, x, __builtin_fmaf(b.hi, y, a.lo * x + b.lo * y));
+    else return a.hi * x + b.hi * y;
+}
+
+template <typename T>
+__device__ __forceinline__ KSplit<T> kneg(KSplit<T> k) {
+    return KSplit<T>{-k.hi, -k.lo};
+}
+
 template <typename T>
 __device__ __forceinline__ c2<T> cmul(c2<T> a, c2<T> b) {
     return make_c<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -100,11 +135,11 @@ __device__ __forceinline__ void dft_small(c2<T>* v, bool inverse) {
         v[2] = make_c<T>(s0.x - s2.x, s0.y - s2.y);
         v[3] = make_c<T>(s1.x - s3.x, s1.y - s3.y);
     } else {  // R == 3
-        const T h = (T)0.86602540378443864676372317075294;  // sqrt(3)/2
+        constexpr KSplit<T> h = ksplit<T>(0.86602540378443864676372317075294);  // sqrt(3)/2
         const c2<T> a = v[0], b = v[1], c = v[2];
         const c2<T> t1 = make_c<T>(b.x + c.x, b.y + c.y);
         const c2<T> t2 = make_c<T>(a.x - (T)0.5 * t1.x, a.y - (T)0.5 * t1.y);
-        const c2<T> d = make_c<T>(h * (b.x - c.x), h * (b.y - c.y));
+        const c2<T> d = make_c<T>(kmul<T>(b.x - c.x, h), kmul<T>(b.y - c.y, h));
         const c2<T> t3 = rot90<T>(d, inverse);  // -+i sqrt(3)/2 (b - c)
         v[0] = make_c<T>(a.x + t1.x, a.y + t1.y);
         v[1] = make_c<T>(t2.x + t3.x, t2.y + t3.y);
@@ -144,25 +179,27 @@ __host__ __device__ __forceinline__ constexpr int seq_stride(int n) {
 // x * exp(-+2 pi i / 8): (1 -+ i) / sqrt(2).
 template <typename T>
 __device__ __forceinline__ c2<T> mul_w8(c2<T> a, bool inverse) {
-    const T h = (T)0.70710678118654752440084436210485;
-    return inverse ? make_c<T>(h * (a.x - a.y), h * (a.y + a.x)) : make_c<T>(h * (a.x + a.y), h * (a.y - a.x));
+    constexpr KSplit<T> h = ksplit<T>(0.70710678118654752440084436210485);
+    return inverse ? make_c<T>(kmul<T>(a.x - a.y, h), kmul<T>(a.y + a.x, h))
+                   : make_c<T>(kmul<T>(a.x + a.y, h), kmul<T>(a.y - a.x, h));
 }
 
 // x * exp(-+2 pi i m / 16) for the products m = n2 k1 of a 4 x 4 split.
 template <typename T>
 __device__ __forceinline__ c2<T> mul_w16(c2<T> a, int m, bool inverse) {
-    const T c = (T)0.92387953251128675612818318939679;  // cos(pi / 8)
-    const T s = (T)0.38268343236508977172845998403040;  // sin(pi / 8)
-    const T si = inverse ? s : -s;
-    const T ci = inverse ? c : -c;
+    constexpr KSplit<T> c = ksplit<T>(0.92387953251128675612818318939679);  // cos(pi / 8)
+    constexpr KSplit<T> s = ksplit<T>(0.38268343236508977172845998403040);  // sin(pi / 8)
+    const KSplit<T> si = inverse ? s : kneg<T>(s);
+    const KSplit<T> ci = inverse ? c : kneg<T>(c);
     switch (m) {
         case 0: return a;
-        case 1: return make_c<T>(c * a.x - si * a.y, c * a.y + si * a.x);     // c -+ i s
+        case 1: return make_c<T>(kdot<T>(c, a.x, kneg<T>(si), a.y), kdot<T>(c, a.y, si, a.x));  // c -+ i s
         case 2: return mul_w8<T>(a, inverse);
-        case 3: return make_c<T>(s * a.x - ci * a.y, s * a.y + ci * a.x);     // s -+ i c
+        case 3: return make_c<T>(kdot<T>(s, a.x, kneg<T>(ci), a.y), kdot<T>(s, a.y, ci, a.x));  // s -+ i c
         case 4: return rot90<T>(a, inverse);
         case 6: return rot90<T>(mul_w8<T>(a, inverse), inverse);
-        default: return make_c<T>(-(c * a.x - si * a.y), -(c * a.y + si * a.x));  // m = 9: -(c -+ i s)
+        default:  // m = 9: -(c -+ i s)
+            return make_c<T>(-kdot<T>(c, a.x, kneg<T>(si), a.y), -kdot<T>(c, a.y, si, a.x));
     }
 }
 
@@ -227,15 +264,38 @@ __device__ __forceinline__ void dft_radix(c2<T>* v, bool inverse) {
     else dft_small<T, R>(v, inverse);
 }
 
-// Stages a plan's twiddles into LDS (the n - 1 entries of the per-pass table, see the
-// plan builder): pass p (radix R, ns = product of the earlier radices) finds w^(q jm n / (ns R))
-// at ns - 1 + jm (R - 1) + q - 1 for jm < ns, 1 <= q < R, so a butterfly reads its R - 1
+// FP32 twiddle tails.  A twiddle rounded to FP32 has |w|^2 - 1 of up to 1e-7, and over the
+// few distinct roots of the early passes these errors do not average out (the 16th roots lose
+// 2.3e-8 of the energy per element and pass).  That makes every FP32 2-D transform shrink the
+// energy by ~8e-8, which compounds over repeated transforms.  FP32 tables therefore carry a
+// tail lo = w - hi for the slots below kTwLo - 1 (the passes with ns R <= kTwLo), and
+// those passes multiply by hi + lo.  The later passes use 512th or finer roots, whose rounding
+// errors average to ~1e-9 per pass.
+constexpr int kTwLo = 256;
+
+// Table slots of an n-point plan (device and LDS alike): FP64, the n - 1 twiddles; FP32, the
+// n - 1 heads, one pad slot, then kTwLo tails (tail of slot j at n + j, zero past n - 2).
+template <typename T>
+__host__ __device__ __forceinline__ constexpr int tw_slots(int n) {
+    return sizeof(T) == 4 ? n + kTwLo : (n > 1 ? n - 1 : 1);
+}
+
+// Stages a plan's twiddles into LDS (the per-pass table, see the plan builder): pass p
+// (radix R, ns = product of the earlier radices) finds w^(q jm n / (ns R)) at
+// ns - 1 + jm (R - 1) + q - 1 for jm < ns, 1 <= q < R, so a butterfly reads its R - 1
 // factors from consecutive slots and neighbouring butterflies from neighbouring runs.
 // Callers put a barrier between this and lds_fft.
 template <typename T, int NT = kFftThreads>
 __device__ __forceinline__ void stage_twiddles(c2<T>* s_tw, const c2<T>* __restrict__ tw, int n) {
-    staged<c2<T>, (sizeof(T) == 4 ? 16 : 8), NT>(n - 1, [&](int i) { return tw[i]; },
+    staged<c2<T>, (sizeof(T) == 4 ? 16 : 8), NT>(tw_slots<T>(n), [&](int i) { return tw[i]; },
                                                  [&](int i, c2<T> w) { s_tw[i] = w; });
+}
+
+// x (hi + lo) for an FP32 twiddle with its tail.
+__device__ __forceinline__ c2<float> cmul_split(c2<float> x, c2<float> hi, c2<float> lo) {
+    const float re = __builtin_fmaf(x.x, hi.x, __builtin_fmaf(-x.y, hi.y, __builtin_fmaf(x.x, lo.x, -(x.y * lo.y))));
+    const float im = __builtin_fmaf(x.x, hi.y, __builtin_fmaf(x.y, hi.x, __builtin_fmaf(x.x, lo.y, x.y * lo.x)));
+    return make_c<float>(re, im);
 }
 
 // One Stockham pass over `count` sequences ld apart by a group of NT threads; V = elements
@@ -247,6 +307,7 @@ __device__ __forceinline__ void stockham_pass(c2<T>* buf, int n, int ld, int cou
     const int nb = n / R;
     const int Ns = ps.ns;
     const int total = nb * count;
+    const bool tails = sizeof(T) == 4 && Ns * R <= kTwLo;  // this pass's FP32 slots carry tails
     const FastDiv by_nb = FastDiv::from_magic(ps.nb_m), by_ns = FastDiv::from_magic(ps.ns_m);
     c2<T> v[B][R];
     int base[B], k0[B];
@@ -264,9 +325,20 @@ __device__ __forceinline__ void stockham_pass(c2<T>* buf, int n, int ld, int cou
             for (int q = 0; q < R; ++q) {
                 c2<T> x = buf[base[i] + pad_at(j + q * nb)];
                 if (q > 0 && jm > 0) {
-                    c2<T> w = tw[jm * (R - 1) + q - 1];
+                    const int at = jm * (R - 1) + q - 1;
+                    c2<T> w = tw[at];
                     if (inverse) w.y = -w.y;
-                    x = cmul<T>(x, w);
+                    if constexpr (sizeof(T) == 4) {
+                        if (tails) {
+                            c2<T> lo = tw[n + at];
+                            if (inverse) lo.y = -lo.y;
+                            x = cmul_split(x, w, lo);
+                        } else {
+                            x = cmul<T>(x, w);
+                        }
+                    } else {
+                        x = cmul<T>(x, w);
+                    }
                 }
                 v[i][q] = x;
             }

@@ -308,7 +308,7 @@ static int launch_rows_spinor_v(const pgpe_fft_plan* plan, const SplitParams<T>&
     p.tw = (const c2<T>*)plan->tw_x[radix_class<V>()];
     const int pairs_per = E / (2 * plan->nx);
     const int total = p.members * p.ny;
-    const size_t shm = (size_t)(2 * pairs_per * seq_stride<T>(plan->nx) + plan->nx) * sizeof(c2<T>);
+    const size_t shm = (size_t)(2 * pairs_per * seq_stride<T>(plan->nx) + tw_slots<T>(plan->nx)) * sizeof(c2<T>);
     if (pairs_per < 1 || shm > 163840) return fail(PGPE_ERR_UNSUPPORTED, "split-step: spinor rows too long");
     auto kern = fft_rows_spinor_kernel<T, V, MODE, RES, NT>;
     if (shm > 65536) {
@@ -343,7 +343,7 @@ static int launch_cols_spinor_v(const pgpe_fft_plan* plan, const SplitParams<T>&
     p.tw = (const c2<T>*)plan->tw_y[radix_class<V>()];
     const int cols_per = E / (2 * plan->ny);
     const int groups = (plan->nx + cols_per - 1) / cols_per;
-    const size_t shm = (size_t)(2 * cols_per * seq_stride<T>(plan->ny) + plan->ny) * sizeof(c2<T>);
+    const size_t shm = (size_t)(2 * cols_per * seq_stride<T>(plan->ny) + tw_slots<T>(plan->ny)) * sizeof(c2<T>);
     auto kern = fft_cols_spinor_kernel<T, V, NT>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -359,7 +359,7 @@ static int launch_cols_spinor_v(const pgpe_fft_plan* plan, const SplitParams<T>&
 template <typename T>
 static int launch_cols_spinor(const pgpe_fft_plan* plan, const SplitParams<T>& p, hipStream_t s) {
     const int n = plan->ny;
-    auto lds = [&](int cols) { return (size_t)(2 * cols * seq_stride<T>(n) + n) * sizeof(c2<T>); };
+    auto lds = [&](int cols) { return (size_t)(2 * cols * seq_stride<T>(n) + tw_slots<T>(n)) * sizeof(c2<T>); };
     if (n % 3 == 0) {
         if (256 * 12 < 2 * n || lds(256 * 12 / (2 * n)) > 163840)
             return fail(PGPE_ERR_UNSUPPORTED, "split-step: spinor columns longer than 1536");

@@ -26,10 +26,10 @@ namespace pgpe {
 
 constexpr int kMaxStages = 12;
 
-// LDS of a group: `seqs` padded sequences of length n plus the n - 1 staged twiddles.
+// LDS of a group: `seqs` padded sequences of length n plus the staged twiddle table.
 template <typename T>
 static size_t lds_bytes(int seqs, int n) {
-    return (size_t)(seqs * seq_stride<T>(n) + n) * sizeof(c2<T>);
+    return (size_t)(seqs * seq_stride<T>(n) + tw_slots<T>(n)) * sizeof(c2<T>);
 }
 
 enum LawsonColMode { LC_FWD = 0, LC_COMBINE = 1, LC_FINAL = 2 };

@@ -93,11 +93,11 @@ def test_split_step_with_noise_config_window_matches_oracle(precision, tol):
     """The split-step-with-noise configuration at full size (512^2, lx = ly = 200 um, Gaussian
     sigma 20 um, P0 = 2 P_th, Wigner noise seed 1234, dt 5e-3 ps) over a 60-step window.
 
-    The psi tolerance is the reference's CPU-GPU bar (1e-10 FP64, 1e-5 FP32,
-    tests/crosscheck/test_cpu_gpu_agreement.py:12-13); the density error of a field is twice
-    its relative field error to first order, so the FP32 density bar is 2e-5 (240 FP32
-    transforms leave ~7e-6 on psi against the oracle's own FP32 transforms).  The norm (the sum
-    of the density) is held to the plain bar, 1e-5: its pointwise rounding errors average out."""
+    psi, density, norm and reservoir are held to the reference's CPU-GPU bar (1e-10 FP64,
+    1e-5 FP32, tests/crosscheck/test_cpu_gpu_agreement.py:12-13).  The 240 FP32 transforms of
+    this window leave ~2e-6 on psi and density and ~3e-7 on the norm; the norm holds because
+    the FP32 twiddles carry tails (without them each 2-D transform lost ~8e-8 of the energy,
+    1.1e-5 over the window)."""
     cfg = _cfg(grid__nx=512, grid__ny=512, grid__lx=200.0, grid__ly=200.0, laser__sigma_space=20.0,
                laser__P0=2 * 0.02075, solver__precision=precision, solver__total_time=0.3,
                noise__stochastic="wigner", noise__seed=1234)
@@ -108,7 +108,7 @@ def test_split_step_with_noise_config_window_matches_oracle(precision, tol):
     norm, norm_ref = float((np.abs(ctl.state.field.get()) ** 2).sum()), float((np.abs(psi_ref) ** 2).sum())
     en = abs(norm - norm_ref) / norm_ref
     print(f"512^2 split-step + noise {precision}: psi {e:.3e} density {d:.3e} norm {en:.3e}")
-    assert e <= tol and d <= 2 * tol and en <= tol
+    assert e <= tol and d <= tol and en <= tol
     assert O.rel_l2(ctl.reservoir.get_state()[0].get(), res_ref[0]) <= tol
 
 
