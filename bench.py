@@ -76,6 +76,11 @@ SWEEP = dict(members=256, n=256, L=128.0, dt=0.05, check_every=1.0, rtol=1e-6, m
 WEIGHTS = os.path.join(ROOT, "polarism_amd", "surrogate", "weights", "fno_steady_256.npz")
 
 
+def progress(msg):
+    """One line on stderr when a step or a leg has finished (the JSON result alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def run_sweep(members, tile_h="auto", graphs=True):
     from polarism_amd.sweep import PackedSweep
     sw = PackedSweep(members, reservoir="single", precision="single", dt=SWEEP["dt"],
@@ -163,7 +168,9 @@ def sweep_step_kernel(reps=20):
     r0, r1 = e.res, e._res_alt
     ms = _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0, res_in=r0, res_out=r1), reps)
     if choice.form == "strip":
-        name = f"rk4_strip_kernel<float, 1, {(choice.tile_h + 8) // 4}>"
+        # strip height TH = NW R - 8 (wavefronts per workgroup x rows per wavefront, less the halo)
+        nw, r = {16: (4, 6), 24: (8, 4), 32: (8, 5), 40: (8, 6)}[choice.tile_h]
+        name = f"rk4_strip_kernel<float, 1, {nw}, {r}, false, false>"
         shape = f"row strips of 56 x {choice.tile_h} output points"
     else:
         name = f"rk4_step_kernel<float, 1, {choice.tile_h}>"
@@ -439,14 +446,18 @@ def cpu_worker(steps, procs):
     the other configurations on one core."""
     import multiprocessing as mp
     single, singles = _median_of(lambda: _oracle_rate(steps))
+    progress("cpu: one-core rk4 step timed")
     pooled_runs = []
     with mp.get_context("fork").Pool(procs) as pool:
-        for _ in range(3):
+        for k in range(3):
             t0 = time.perf_counter()
             pool.map(_oracle_rate, [steps] * procs)
             pooled_runs.append(procs * steps * SWEEP["n"] ** 2 / (time.perf_counter() - t0))
+            progress(f"cpu: pooled run {k + 1}/3 timed")
+    configs = _oracle_configs()
+    progress("cpu: configurations timed")
     print(json.dumps({"single": single, "singles": singles, "pooled": float(np.median(pooled_runs)),
-                      "pooled_runs": pooled_runs, "procs": procs, "configs": _oracle_configs()}))
+                      "pooled_runs": pooled_runs, "procs": procs, "configs": configs}))
 
 
 def cpu_baseline(point_steps, converged, steps=200):
@@ -454,10 +465,11 @@ def cpu_baseline(point_steps, converged, steps=200):
     one member per core, scaled to the plain sweep's point-steps."""
     import subprocess
     procs = max(1, min(16, len(os.sched_getaffinity(0))))
+    # the worker's progress lines go straight to this process's stderr
     out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", str(steps), str(procs)],
-                         capture_output=True, text=True, timeout=600, cwd=ROOT)
+                         stdout=subprocess.PIPE, text=True, timeout=600, cwd=ROOT)
     if out.returncode != 0:
-        raise RuntimeError(f"cpu worker failed: {out.stderr[-2000:]}")
+        raise RuntimeError(f"cpu worker failed with status {out.returncode}")
     r = json.loads(out.stdout.strip().splitlines()[-1])
     pooled, single = r["pooled"], r["single"]
     cpu_model = ""
@@ -733,18 +745,23 @@ def main():
     members = ws.make_members(args.members, SWEEP["seed"])
     have_weights = os.path.exists(args.weights)
     if "sweep" in legs:
-        for _ in range(args.warmup):
+        for k in range(args.warmup):
             run_sweep(ws.make_members(8, SWEEP["seed"] + 1), graphs=not args.no_graphs)
+            progress(f"warmup {k + 1}/{args.warmup} done")
         plain_runs, warm_runs, infer_s = [], [], []
         t_all = time.perf_counter()
-        for _ in range(args.steps):
+        for k in range(args.steps):
+            t0 = time.perf_counter()
             plain_runs.append(run_sweep(members, graphs=not args.no_graphs))
+            line = f"step {k + 1}/{args.steps}: plain {time.perf_counter() - t0:.2f} s"
             if have_weights:
                 params, cfg, norm = ws.load_weights(args.weights)
                 t0 = time.perf_counter()
                 wm = ws.warm_start(members, None, norm, states=ws.predict_states(params, members, cfg, norm))
                 infer_s.append(time.perf_counter() - t0)
                 warm_runs.append(run_sweep(wm, graphs=not args.no_graphs))
+                line += f", warm {time.perf_counter() - t0:.2f} s"
+            progress(line)
         wall = time.perf_counter() - t_all
         plain = summarise(plain_runs[-1])
         sweep = {"plain": plain, "warm": None}
@@ -767,20 +784,28 @@ def main():
         result["sweep"] = sweep
     if "copy" in legs:
         result["device_copy"] = copy_leg()
+        progress("copy leg done")
     if "roofline" in legs:
         result["roofline"] = roofline_leg()
+        progress("roofline leg done")
     if "uneven" in legs:
         result["uneven_grid_sweep"] = uneven_leg()
+        progress("uneven leg done")
     if "quickstart" in legs:
         result["quickstart_rk4_fp64"] = quickstart_leg()
+        progress("quickstart leg done")
     if "fft" in legs:
         result["split_step_fft"] = fft_leg()
+        progress("fft leg done")
     if "ffthbm" in legs:
         result["fft_batched_2048"] = fft_hbm_leg()
+        progress("ffthbm leg done")
     if "lawson" in legs:
         result["lawson_spinor_dp5"] = lawson_leg()
+        progress("lawson leg done")
     if "fno" in legs:
         result["surrogate_fno"] = fno_leg()
+        progress("fno leg done")
     result["config"] = {"workload": "steady-state pump sweep, 256 members of 256x256, rk4-fdm-hip complex64, "
                                     "single reservoir, dt 0.05 ps; warm start (FNO) vs plain cold start",
                         "members": len(members), "grid": "256x256", "lx_um": SWEEP["L"], "dt_ps": SWEEP["dt"],

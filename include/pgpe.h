@@ -159,7 +159,9 @@ typedef struct pgpe_rk4_args {
     int32_t members;
     int32_t components;     /* 1 scalar; 2 spinor (psi+, psi-) with TE-TM, Zeeman and alpha_2 */
     int32_t res_components; /* spinor reservoir: 1 shared (rho = |psi+|^2 + |psi-|^2), 2 per spin */
-    int32_t pad0;
+    int32_t five_point;     /* pgpe_rk4_step with strips only: 1 promises that every member of the
+                               table uses the 5-point Laplacian, so the kernel leaves the 9-point
+                               branch out; 0 = general */
     double dt;
     const pgpe_member* table;        /* device */
     const int32_t* tile_map;         /* device, num_tiles (member, local tile) int32 pairs */
@@ -186,7 +188,9 @@ typedef struct pgpe_rk4_args {
     int32_t strips;                  /* pgpe_rk4_step only: 0 = 64 x tile_y tiles of the member
                                         table; 1 = row strips of 56 output columns x tile_y rows
                                         (16, 24, or 32 and 40 in single precision), local index
-                                        (row * ceil(nx / 56) + column), from the strip map */
+                                        (row * ceil(nx / 56) + column), from the strip map; every
+                                        member must hold at most 2^28 points (the kernel
+                                        addresses a member with 32-bit byte offsets) */
     int32_t no_eta;                  /* pgpe_rk4_step with strips only: 1 promises that no member
                                         of the table has a kinetic-relaxation term (eta = 0), so
                                         the kernel leaves it out; 0 = general */

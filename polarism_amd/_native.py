@@ -53,7 +53,8 @@ class Rk4Args(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("reservoir", C.c_int32), ("stage", C.c_int32),
         ("num_tiles", C.c_int32), ("tile_y", C.c_int32), ("noise", C.c_int32),
-        ("members", C.c_int32), ("components", C.c_int32), ("res_components", C.c_int32), ("pad0", C.c_int32),
+        ("members", C.c_int32), ("components", C.c_int32), ("res_components", C.c_int32),
+        ("five_point", C.c_int32),
         ("dt", C.c_double),
         ("table", vp), ("tile_map", vp),
         ("psi0", vp), ("kprev", vp), ("k1", vp), ("k2", vp),

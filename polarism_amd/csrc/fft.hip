@@ -239,6 +239,107 @@ __global__ void __launch_bounds__(NT) fft_cols_kernel(const SplitParams<T> p) {
         [&](int, Pt q) { p.out[base + q.at] = q.v; });
 }
 
+// ---------------------------------------------------------------- tall columns, split in two
+// A forward or inverse column transform of length n = 2h (a power of two) as one radix-2
+// decimation-in-frequency step in registers and two h-point LDS transforms:
+//   a[y] = x[y] + x[y + h],  b[y] = (x[y] - x[y + h]) w^y,  w = exp(-+2 pi i / n),
+//   X[2k] = DFT_h(a)[k],     X[2k + 1] = DFT_h(b)[k].
+// Every thread loads its V pairs (all 2V loads in flight), keeps its b values in registers while
+// the group transforms and stores the even outputs, then transforms b into the odd outputs.
+// The LDS holds C half columns (C = 8 complex64 or 4 complex128: 64-byte row segments, as the
+// one-pass kernel), half of what the one-pass kernel needs, so two groups share a CU and one
+// group's loads and stores overlap the other's passes.  FP32 roots carry tails (the slot
+// y + h of `root`), as the plan's early-pass twiddles do.
+template <typename T, int V, int MODE, int NT, int CLS>
+__global__ void __launch_bounds__(NT, 2 * NT / 256) fft_cols_half_kernel(const SplitParams<T> p,
+                                                                        const c2<T>* __restrict__ root) {
+    extern __shared__ unsigned char smem_raw[];
+    c2<T>* buf = reinterpret_cast<c2<T>*>(smem_raw);
+    constexpr int E = NT * V;
+    const int h = p.ny >> 1;
+    const int cols_per = E / h;
+    const int ld = seq_stride<T>(h);
+    c2<T>* s_tw = buf + cols_per * ld;
+    const int groups_per_plane = (p.nx + cols_per - 1) / cols_per;
+    const int g = xcd_tile(blockIdx.x, groups_per_plane * p.planes);
+    if (g < 0) return;
+    const int plane = g / groups_per_plane;
+    const int x0 = (g - plane * groups_per_plane) * cols_per;
+    const int count = min(cols_per, p.nx - x0);
+    const int64_t base = (int64_t)plane * p.ny * p.nx + x0;
+    const int64_t half = (int64_t)h * p.nx;
+    const int elems = count * h;
+    const FastDiv by_count((uint32_t)count);
+    constexpr bool inverse = MODE == COL_INV;
+    stage_twiddles<T, NT>(s_tw, p.tw, h);
+
+    // the pairs in batches of PB: 2 PB loads in flight per thread, within the register budget
+    constexpr int PB = V < 8 ? V : 8;
+    c2<T> b[V];
+#pragma unroll
+    for (int k0 = 0; k0 < V; k0 += PB) {
+        c2<T> u[PB], v[PB];
+#pragma unroll
+        for (int k = 0; k < PB; ++k) {
+            const int i = (int)threadIdx.x + (k0 + k) * NT;
+            if (i < elems) {
+                const int y = by_count.div(i);
+                const int64_t at = base + (int64_t)y * p.nx + (i - y * count);
+                u[k] = p.in[at];
+                v[k] = p.in[at + half];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PB; ++k) {
+            const int i = (int)threadIdx.x + (k0 + k) * NT;
+            if (i < elems) {
+                const int y = by_count.div(i);
+                const int c = i - y * count;
+                const c2<T> a = make_c<T>(u[k].x + v[k].x, u[k].y + v[k].y);
+                const c2<T> d = make_c<T>(u[k].x - v[k].x, u[k].y - v[k].y);
+                c2<T> w = root[y];
+                if (inverse) w.y = -w.y;
+                if constexpr (sizeof(T) == 4) {
+                    c2<T> lo = root[h + y];
+                    if (inverse) lo.y = -lo.y;
+                    b[k0 + k] = cmul_split(d, w, lo);
+                } else {
+                    b[k0 + k] = cmul<T>(d, w);
+                }
+                buf[c * ld + pad_at(y)] = a;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int odd = 0; odd < 2; ++odd) {
+        if (odd) {
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                const int i = (int)threadIdx.x + k * NT;
+                if (i < elems) {
+                    const int y = by_count.div(i);
+                    buf[(i - y * count) * ld + pad_at(y)] = b[k];
+                }
+            }
+            __syncthreads();
+        }
+        lds_fft<T, V, NT, CLS>(buf, p.dims, ld, count, s_tw, inverse);
+        // outputs k of this half go to rows 2k + odd (PB LDS reads in flight, then PB stores)
+        staged<c2<T>, PB, NT>(
+            elems,
+            [&](int i) {
+                const int k = by_count.div(i);
+                return buf[(i - k * count) * ld + pad_at(k)];
+            },
+            [&](int i, c2<T> v) {
+                const int k = by_count.div(i);
+                p.out[base + (int64_t)(2 * k + odd) * p.nx + (i - k * count)] = v;
+            });
+        if (!odd) __syncthreads();
+    }
+}
+
 
 template <typename T>
 static void twiddles(int n, std::vector<c2<T>>& out) {
@@ -347,8 +448,61 @@ static int column_shape(const pgpe_fft_plan* plan, int planes) {
     return best;
 }
 
+template <typename T, int V, int MODE, int NT, int CLS>
+static int launch_cols_half_v(const pgpe_fft_plan* plan, const SplitParams<T>& p0, hipStream_t s) {
+    constexpr int E = NT * V;
+    SplitParams<T> p = p0;
+    const int h = plan->ny / 2;
+    p.dims = plan->dims_yh[CLS];
+    p.tw = (const c2<T>*)plan->tw_yh[CLS];
+    const int cols_per = E / h;
+    const int groups = (plan->nx + cols_per - 1) / cols_per;
+    const size_t shm = (size_t)(cols_per * seq_stride<T>(h) + tw_slots<T>(h)) * sizeof(c2<T>);
+    auto kern = fft_cols_half_kernel<T, V, MODE, NT, CLS>;
+    if (shm > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(fft_cols_half_kernel)");
+    }
+    hipLaunchKernelGGL(kern, dim3(xcd_grid(groups * p.planes)), dim3(NT), shm, s, p, (const c2<T>*)plan->root_y);
+    PGPE_LAUNCHED("fft_cols_half_kernel");
+    return PGPE_OK;
+}
+
+// Columns split in two (fft_cols_half_kernel) where one-pass groups of 64-byte row segments would
+// leave one group per CU: complex64 columns of 2048 (8 per group) or 4096 (4 per group) points
+// and complex128 columns of 1024 (4 per group), on batches of >= 512 groups.  The complex64
+// groups hold 16 b values per thread under the 128 registers of four waves per SIMD, so their
+// half transforms take the radix-8 plan (radix-16 passes would spill).  Sets *done = false when
+// the one-pass kernel is to be used.
+template <typename T, int MODE>
+static int launch_cols_half(const pgpe_fft_plan* plan, const SplitParams<T>& p, hipStream_t s, bool* done) {
+    *done = false;
+    if constexpr (MODE != COL_FWD && MODE != COL_INV) {
+        return PGPE_OK;
+    } else {
+        if (!plan->root_y) return PGPE_OK;
+        const int n = plan->ny;
+        const int64_t cols = (int64_t)plan->nx * p.planes;
+        if constexpr (sizeof(T) == 4) {
+            if ((n == 2048 && cols / 8 >= 512) || (n == 4096 && cols / 4 >= 512)) {
+                *done = true;
+                return launch_cols_half_v<T, 16, MODE, 512, 1>(plan, p, s);
+            }
+        } else {
+            if (n == 1024 && cols / 4 >= 512) {
+                *done = true;
+                return launch_cols_half_v<T, 8, MODE, 256, 1>(plan, p, s);
+            }
+        }
+        return PGPE_OK;
+    }
+}
+
 template <typename T, int MODE>
 static int launch_cols(const pgpe_fft_plan* plan, const SplitParams<T>& p, hipStream_t s) {
+    bool done;
+    const int rc = launch_cols_half<T, MODE>(plan, p, s, &done);
+    if (done || rc) return rc;
     switch (column_shape<T>(plan, p.planes)) {
         case 0: return launch_cols_v<T, 4, MODE, 64>(plan, p, s);
         case 1: return launch_cols_v<T, 4, MODE, 128>(plan, p, s);
@@ -439,14 +593,31 @@ int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pg
     plan->vx = vx;
     plan->vy = vy;
     plan->row_shape = row_shape(nx);
-    // pass lists: x classes 0..2, then y classes 0..2, in one device block
-    std::vector<FftPass> lists[6], all;
+    {
+        // a row group holds threads x per_thread / nx padded rows and the twiddles in LDS: very
+        // short complex128 rows (6 points) pad past the 160 KB
+        int threads, per_thread;
+        row_shape_size(plan->row_shape, threads, per_thread);
+        const int rows = threads * per_thread / nx;
+        const size_t shm = precision ? (size_t)(rows * seq_stride<double>(nx) + tw_slots<double>(nx)) * sizeof(double2)
+                                     : (size_t)(rows * seq_stride<float>(nx) + tw_slots<float>(nx)) * sizeof(float2);
+        if (shm > 163840) {
+            delete plan;
+            return fail(PGPE_ERR_UNSUPPORTED, "pgpe_fft_plan_create: rows this short do not fit a row group in this precision");
+        }
+    }
+    // pass lists: x classes 0..2, y classes 0..2, then (tall columns) ny / 2 classes 0..2, in
+    // one device block
+    const int nh = (ny >= 4 && (ny & (ny - 1)) == 0) ? ny / 2 : 0;
+    const int nlists = nh ? 9 : 6;
+    std::vector<FftPass> lists[9], all;
     for (int c = 0; c < 3; ++c) {
         build_passes(nx, c, lists[c]);
         build_passes(ny, c, lists[3 + c]);
+        if (nh) build_passes(nh, c, lists[6 + c]);
     }
-    int offs[6];
-    for (int k = 0; k < 6; ++k) {
+    int offs[9];
+    for (int k = 0; k < nlists; ++k) {
         offs[k] = (int)all.size();
         all.insert(all.end(), lists[k].begin(), lists[k].end());
     }
@@ -456,13 +627,13 @@ int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pg
     const size_t csz = precision ? sizeof(double2) : sizeof(float2);
     hipError_t e = hipMalloc((void**)&plan->passes, sizeof(FftPass) * all.size());
     if (e == hipSuccess) e = hipMemcpy(plan->passes, all.data(), sizeof(FftPass) * all.size(), hipMemcpyHostToDevice);
-    for (int k = 0; k < 6 && e == hipSuccess; ++k) {
-        const int n = k < 3 ? nx : ny;
-        FftDims& d = k < 3 ? plan->dims_x[k] : plan->dims_y[k - 3];
+    for (int k = 0; k < nlists && e == hipSuccess; ++k) {
+        const int n = k < 3 ? nx : (k < 6 ? ny : nh);
+        FftDims& d = k < 3 ? plan->dims_x[k] : (k < 6 ? plan->dims_y[k - 3] : plan->dims_yh[k - 6]);
         d.n = n;
         d.npass = (int)lists[k].size();
         d.passes = plan->passes + offs[k];
-        void** tw = k < 3 ? &plan->tw_x[k] : &plan->tw_y[k - 3];
+        void** tw = k < 3 ? &plan->tw_x[k] : (k < 6 ? &plan->tw_y[k - 3] : &plan->tw_yh[k - 6]);
         const size_t slots = precision ? (size_t)tw_slots<double>(n) : (size_t)tw_slots<float>(n);
         e = hipMalloc(tw, csz * slots);
         if (e != hipSuccess) break;
@@ -484,6 +655,23 @@ int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pg
             e = hipMemcpy(*tw, all_slots.data(), csz * slots, hipMemcpyHostToDevice);
         }
     }
+    if (nh && e == hipSuccess) {
+        // roots of the splitting step: ny / 2 heads (FP32: then ny / 2 tails)
+        if (precision) {
+            std::vector<double2> root;
+            twiddles<double>(ny, root);
+            e = hipMalloc(&plan->root_y, sizeof(double2) * nh);
+            if (e == hipSuccess) e = hipMemcpy(plan->root_y, root.data(), sizeof(double2) * nh, hipMemcpyHostToDevice);
+        } else {
+            std::vector<float2> root, tails;
+            twiddles<float>(ny, root);
+            twiddle_tails(ny, tails);
+            std::vector<float2> both(root.begin(), root.begin() + nh);
+            both.insert(both.end(), tails.begin(), tails.begin() + nh);
+            e = hipMalloc(&plan->root_y, sizeof(float2) * 2 * nh);
+            if (e == hipSuccess) e = hipMemcpy(plan->root_y, both.data(), sizeof(float2) * 2 * nh, hipMemcpyHostToDevice);
+        }
+    }
     if (e == hipSuccess) e = hipMalloc((void**)&plan->kx, sizeof(double) * nx);
     if (e == hipSuccess) e = hipMalloc((void**)&plan->ky, sizeof(double) * ny);
     if (e == hipSuccess) e = hipMemcpy(plan->kx, kx.data(), sizeof(double) * nx, hipMemcpyHostToDevice);
@@ -499,7 +687,8 @@ int pgpe_fft_plan_create(int nx, int ny, double dx, double dy, int precision, pg
 int pgpe_fft_plan_destroy(pgpe_fft_plan* plan) {
     if (!plan) return PGPE_OK;
     hipError_t e = hipSuccess, f;
-    void* ptrs[] = {plan->tw_x[0], plan->tw_x[1], plan->tw_x[2], plan->tw_y[0], plan->tw_y[1], plan->tw_y[2],
+    void* ptrs[] = {plan->tw_x[0],  plan->tw_x[1],  plan->tw_x[2],  plan->tw_y[0], plan->tw_y[1],
+                    plan->tw_y[2],  plan->tw_yh[0], plan->tw_yh[1], plan->tw_yh[2], plan->root_y,
                     (void*)plan->passes, (void*)plan->kx, (void*)plan->ky};
     for (void* ptr : ptrs)
         if (ptr && (f = hipFree(ptr)) != hipSuccess && e == hipSuccess) e = f;

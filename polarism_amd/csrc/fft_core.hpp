@@ -359,16 +359,18 @@ __device__ __forceinline__ void stockham_pass(c2<T>* buf, int n, int ld, int cou
 
 // Sequence s occupies buf[s * ld + pad_at(k)], k < n, ld >= pad_at(n - 1) + 1; tw holds the
 // staged twiddles of the plan d, whose radices divide V (the launchers pick the plan by V).
-template <typename T, int V, int NT = kFftThreads>
+// CLS is the plan's radix class (radix_class below by default); passes of larger radices are not
+// compiled in, so a kernel short of registers can run a class-1 plan at V = 16.
+template <typename T, int V, int NT = kFftThreads, int CLS = ((V % 16 == 0 && V < 64) ? 2 : (V % 8 == 0 ? 1 : 0))>
 __device__ __forceinline__ void lds_fft(c2<T>* buf, const FftDims& d, int ld, int count,
                                         const c2<T>* __restrict__ tw, bool inverse) {
     for (int p = 0; p < d.npass; ++p) {
         const FftPass ps = d.passes[p];
         const c2<T>* t = tw + (ps.ns - 1);
         if (ps.radix == 16) {
-            if constexpr (V % 16 == 0 && V < 64) stockham_pass<T, 16, V, NT>(buf, d.n, ld, count, ps, t, inverse);
+            if constexpr (CLS >= 2 && V % 16 == 0 && V < 64) stockham_pass<T, 16, V, NT>(buf, d.n, ld, count, ps, t, inverse);
         } else if (ps.radix == 8) {
-            if constexpr (V % 8 == 0) stockham_pass<T, 8, V, NT>(buf, d.n, ld, count, ps, t, inverse);
+            if constexpr (CLS >= 1 && V % 8 == 0) stockham_pass<T, 8, V, NT>(buf, d.n, ld, count, ps, t, inverse);
         } else if (ps.radix == 4) {
             if constexpr (V % 4 == 0) stockham_pass<T, 4, V, NT>(buf, d.n, ld, count, ps, t, inverse);
         } else if (ps.radix == 2) {

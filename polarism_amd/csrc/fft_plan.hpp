@@ -14,9 +14,15 @@ struct pgpe_fft_plan {
     pgpe::FftDims dims_x[3], dims_y[3];
     void* tw_x[3];
     void* tw_y[3];
-    pgpe::FftPass* passes;  // device block holding the six pass lists
+    pgpe::FftPass* passes;  // device block holding the pass lists
     double* kx;             // 2 pi fftfreq(nx, dx)
     double* ky;
+    // Tall columns split in two (ny a power of two, ny >= 4): the pass lists and per-pass
+    // twiddles of the ny / 2-point transform per radix class, and the ny-th roots
+    // exp(-2 pi i y / ny), y < ny / 2, of the splitting step (FP32: the heads, then their tails).
+    pgpe::FftDims dims_yh[3];
+    void* tw_yh[3];
+    void* root_y;
     int vx, vy;   // elements per thread of the Lawson row / column kernels (16, or 12 with a radix 3)
     int row_shape;  // threads x elements per thread of the FFT / split-step row kernels
                     // (pgpe::RowShape); column groups are sized per launch
@@ -81,6 +87,13 @@ inline int row_shape(int n) {
     if (n <= 1024) return ROWS_256x4;
     if (n <= 2048) return ROWS_256x8;
     return ROWS_256x16;
+}
+
+// Threads x elements per thread of a row shape.
+inline void row_shape_size(int shape, int& threads, int& per_thread) {
+    static const int sizes[][2] = {{64, 4}, {128, 4}, {256, 4}, {256, 8}, {256, 12}, {256, 16}};
+    threads = sizes[shape][0];
+    per_thread = sizes[shape][1];
 }
 
 }  // namespace pgpe

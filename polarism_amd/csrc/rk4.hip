@@ -16,11 +16,12 @@
 // formed once in LDS from coalesced loads of psi0 and k_{s-1}, so HBM sees each
 // array once per stage; the 5-point Laplacian then reads LDS.  Wavefront w
 // handles rows w, w+4, ... so every global access of a wavefront is one
-// contiguous 64-point row segment.  Arithmetic follows the reference's
-// operation order; the library is built with -ffp-contract=off, and the RK4
-// point arithmetic (stencils, right-hand side, stage combinations) opts into
-// fused multiply-adds per source expression (clang fp contract(on)), so the
-// stage and whole-step kernels fuse the same operations.
+// contiguous 64-point row segment.  The right-hand side folds the kinetic
+// term into stencil weights on neighbour sums (scalar_rhs), which agrees with the
+// reference's operation order to rounding; the library is built with
+// -ffp-contract=off, and the RK4 point arithmetic (stencil sums, right-hand side,
+// stage combinations) opts into fused multiply-adds per source expression (clang
+// fp contract(on)), so the stage and whole-step kernels fuse the same operations.
 #include "common.hpp"
 
 namespace pgpe {
@@ -128,18 +129,43 @@ __device__ __forceinline__ double uniform_value(double v) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// Member constants of the scalar right-hand side in the working precision.
+// Member constants of the scalar right-hand side in the working precision, with the kinetic
+// term folded into stencil weights.  From the 5-point neighbour sums sx = psi_e + psi_w and
+// sy = psi_n + psi_s,
+//   (1/hbar)(-hbar^2/2m) lap psi = cy sy + cx sx - c0 psi,
+//   cx = kin / (hbar dx^2), cy = kin / (hbar dy^2), c0 = 2 (cx + cy)   (kin = -hbar^2/2m);
+// from the isotropic 9-point sums s4 = sy + sx and sd (the four diagonals, dx = dy = h),
+//   c4 s4 + c1 sd - c20 psi,  c1 = kin / (6 hbar h^2), c4 = 4 c1, c20 = 20 c1.
+// The couplings enter as g / hbar and the gain as R/2 nA - gammaC/2.  Every folded constant is
+// formed in FP64 from the member row and rounded once.
 template <typename T>
 struct ScalarConsts {
-    T inv_dx2, inv_dy2, kin, inv_hbar, gC, gR, gI, R, gammaC, eta_c, hb2m;
-    __device__ __forceinline__ explicit ScalarConsts(const pgpe_member& mb)
-        : inv_dx2((T)mb.inv_dx2), inv_dy2((T)mb.inv_dy2), kin((T)mb.kin), inv_hbar((T)mb.inv_hbar),
-          gC((T)mb.g_C), gR((T)mb.g_R), gI((T)mb.g_I), R((T)mb.R), gammaC((T)mb.gamma_C), eta_c((T)mb.eta),
-          hb2m((T)mb.hbar_over_2m) {}
+    T cx, cy, c0, c1, c4, c20;
+    T gCh, gRh, gIh, hR, hG, inv_hbar;
+    T eta_h;      // eta hbar / 2m
+    T lap_scale;  // hbar / kin: lap psi = (K - centre psi) hbar / kin
+    __device__ __forceinline__ explicit ScalarConsts(const pgpe_member& mb) {
+        const double kh = mb.kin * mb.inv_hbar;
+        const double k9 = kh * mb.inv_dx2 / 6.0;
+        cx = (T)(kh * mb.inv_dx2);
+        cy = (T)(kh * mb.inv_dy2);
+        c0 = (T)(2.0 * kh * (mb.inv_dx2 + mb.inv_dy2));
+        c1 = (T)k9;
+        c4 = (T)(4.0 * k9);
+        c20 = (T)(20.0 * k9);
+        gCh = (T)(mb.g_C * mb.inv_hbar);
+        gRh = (T)(mb.g_R * mb.inv_hbar);
+        gIh = (T)(mb.g_I * mb.inv_hbar);
+        hR = (T)(0.5 * mb.R);
+        hG = (T)(0.5 * mb.gamma_C);
+        inv_hbar = (T)mb.inv_hbar;
+        eta_h = (T)(mb.eta * mb.hbar_over_2m);
+        lap_scale = (T)(1.0 / kh);
+    }
     // The constants are converted from the member row's doubles by vector instructions, so they
     // land in vector registers; a kernel short of them moves the (uniform) values to scalar ones.
     __device__ __forceinline__ void to_scalar_registers() {
-        T* f[] = {&inv_dx2, &inv_dy2, &kin, &inv_hbar, &gC, &gR, &gI, &R, &gammaC, &eta_c, &hb2m};
+        T* f[] = {&cx, &cy, &c0, &c1, &c4, &c20, &gCh, &gRh, &gIh, &hR, &hG, &inv_hbar, &eta_h, &lap_scale};
         for (T* v : f) *v = uniform_value(*v);
     }
 };
@@ -163,40 +189,42 @@ __device__ __forceinline__ int map_index(int g, int n, int boundary) {
     return r < 0 ? r + n : r;
 }
 
-// 5-point Laplacian (psi[y+1] - 2 psi + psi[y-1]) / dy^2 + (psi[x+1] - 2 psi + psi[x-1]) / dx^2
-// (model.py:220-224) from the point and its north / south / east / west neighbours.
+// Kinetic term (1/hbar)(-hbar^2/2m) lap psi without its centre part, from the 5-point sums
+// sx = psi_e + psi_w, sy = psi_n + psi_s (the stencil of model.py:220-224, periodic or mirrored):
+// K = cy sy + cx sx.
 template <typename T>
-__device__ __forceinline__ c2<T> lap5(const c2<T>& ps, const c2<T>& pn, const c2<T>& pso, const c2<T>& pe,
-                                      const c2<T>& pw, T inv_dx2, T inv_dy2) {
+__device__ __forceinline__ c2<T> kin5(const ScalarConsts<T>& k, const c2<T>& sx, const c2<T>& sy) {
 #pragma clang fp contract(on)
-    c2<T> lap;
-    lap.x = ((pn.x - (T)2 * ps.x) + pso.x) * inv_dy2 + ((pe.x - (T)2 * ps.x) + pw.x) * inv_dx2;
-    lap.y = ((pn.y - (T)2 * ps.y) + pso.y) * inv_dy2 + ((pe.y - (T)2 * ps.y) + pw.y) * inv_dx2;
-    return lap;
+    return make_c<T>(k.cy * sy.x + k.cx * sx.x, k.cy * sy.y + k.cx * sx.y);
 }
 
-// Isotropic 9-point Laplacian (dx == dy = h) of rk4_cuda_solver.py:760-770:
-// (4 (n + s + e + w) + sw + se + nw + ne - 20 psi) / (6 h^2), where the
-// diagonal order is (y-1, x-1), (y-1, x+1), (y+1, x-1), (y+1, x+1).
+// The same for the isotropic 9-point stencil of rk4_cuda_solver.py:760-770,
+// (4 (n + s + e + w) + sw + se + nw + ne - 20 psi) / (6 h^2): K = c4 s4 + c1 sd with
+// s4 = (n + s) + (e + w) and sd = ((sw + se) + nw) + ne, diagonals in the order
+// (y-1, x-1), (y-1, x+1), (y+1, x-1), (y+1, x+1).
 template <typename T>
-__device__ __forceinline__ c2<T> lap9(const c2<T>& ps, const c2<T>& pn, const c2<T>& pso, const c2<T>& pe,
-                                      const c2<T>& pw, const c2<T>& psw, const c2<T>& pse, const c2<T>& pnw,
-                                      const c2<T>& pne, T inv_6h2) {
+__device__ __forceinline__ c2<T> kin9(const ScalarConsts<T>& k, const c2<T>& s4, const c2<T>& sd) {
 #pragma clang fp contract(on)
-    c2<T> lap;
-    lap.x = ((((((T)4 * (((pn.x + pso.x) + pe.x) + pw.x) + psw.x) + pse.x) + pnw.x) + pne.x) - (T)20 * ps.x) * inv_6h2;
-    lap.y = ((((((T)4 * (((pn.y + pso.y) + pe.y) + pw.y) + psw.y) + pse.y) + pnw.y) + pne.y) - (T)20 * ps.y) * inv_6h2;
-    return lap;
+    return make_c<T>(k.c4 * s4.x + k.c1 * sd.x, k.c4 * s4.y + k.c1 * sd.y);
 }
 
-// Scalar condensate and reservoir derivatives at one point from the stage
-// input ps, its Laplacian and the stage reservoir n (model.py:89-127), with
-// an optional complex potential V in (V + gC|psi|^2 + gR nA + gI nI) psi.
-// Both the per-stage and the whole-step kernels call this, so they perform
-// the same operations in the same order.
+template <typename T>
+__device__ __forceinline__ c2<T> csum(const c2<T>& a, const c2<T>& b) {
+    return make_c<T>(a.x + b.x, a.y + b.y);
+}
+
+// Scalar condensate and reservoir derivatives at one point (model.py:89-127):
+//   k = (-i/hbar)[-hbar^2/2m lap psi + (V + gC|psi|^2 + gR nA + gI nI) psi]
+//       + (R nA - gammaC)/2 psi + eta nA hbar/2m lap psi,
+// from the stage input ps, the kinetic sums K (kin5 / kin9) with their centre weight
+// (c0 / c20), the stage reservoir n and an optional complex potential V.  With
+// E = (gC rho + gR nA + gI nI + Re V) / hbar - centre and G = R nA / 2 - gammaC / 2 + Im V / hbar,
+//   k = (G psi_r + (E psi_i + K_i), G psi_i - (E psi_r + K_r)).
+// The per-stage and the whole-step kernels all call this, so they perform the same operations in
+// the same order.
 template <typename T, int RES, bool ETA = true>
 __device__ __forceinline__ void scalar_rhs(const pgpe_member& mb, const ScalarConsts<T>& k, const c2<T>& ps,
-                                           const c2<T>& lap, const T n[2], T P, const c2<T>* V, c2<T>& d,
+                                           const c2<T>& K, T centre, const T n[2], T P, const c2<T>* V, c2<T>& d,
                                            T dn[2]) {
 #pragma clang fp contract(on)
     constexpr int NF = ResFields<RES>::n;
@@ -206,25 +234,20 @@ __device__ __forceinline__ void scalar_rhs(const pgpe_member& mb, const ScalarCo
     reservoir_rhs<T, RES>(mb, rho, P, n, dn);
     const T nA = (NF > 0) ? n[0] : (T)0;
     const T nI = (NF > 1) ? n[1] : (T)0;
-    const T gain = (k.R * nA - k.gammaC) * (T)0.5;
-    T Xr, Xi;
-    if (V == nullptr) {
-        // terms of reservoir fields the model does not have are left out (they are +-0)
-        const T eff = NF > 1 ? (k.gC * rho + k.gR * nA) + k.gI * nI : NF > 0 ? k.gC * rho + k.gR * nA : k.gC * rho;
-        Xr = k.kin * lap.x + eff * ps.x;
-        Xi = k.kin * lap.y + eff * ps.y;
-    } else {
-        const T er = ((V->x + k.gC * rho) + k.gR * nA) + k.gI * nI;
-        const T ei = V->y;
-        Xr = k.kin * lap.x + (er * ps.x - ei * ps.y);
-        Xi = k.kin * lap.y + (er * ps.y + ei * ps.x);
+    // terms of reservoir fields the model does not have are left out
+    T E = NF > 1 ? k.gCh * rho + (k.gRh * nA + (k.gIh * nI - centre))
+                 : (NF > 0 ? k.gCh * rho + (k.gRh * nA - centre) : k.gCh * rho - centre);
+    T G = NF > 0 ? k.hR * nA - k.hG : -k.hG;
+    if (V != nullptr) {
+        E = E + V->x * k.inv_hbar;
+        G = G + V->y * k.inv_hbar;
     }
-    d.x = k.inv_hbar * Xi + gain * ps.x;
-    d.y = -(k.inv_hbar * Xr) + gain * ps.y;
-    if (ETA && k.eta_c != (T)0) {
-        const T s = (k.eta_c * nA) * k.hb2m;
-        d.x = d.x + s * lap.x;
-        d.y = d.y + s * lap.y;
+    d.x = G * ps.x + (E * ps.y + K.y);
+    d.y = G * ps.y - (E * ps.x + K.x);
+    if (ETA && k.eta_h != (T)0) {
+        const T s = k.eta_h * nA;
+        d.x = d.x + s * ((K.x - centre * ps.x) * k.lap_scale);
+        d.y = d.y + s * ((K.y - centre * ps.y) * k.lap_scale);
     }
 }
 
@@ -308,7 +331,7 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
     if (gx >= nx) return;
     const ScalarConsts<T> kc(mb);
     const bool nine = mb.laplacian == PGPE_LAP_NINE;
-    const T inv_6h2 = kc.inv_dx2 / (T)6;
+    const T centre = nine ? kc.c20 : kc.c0;
     const T coef = p.coef[m];
     uint64_t step = p.step;
     if (KIND == 2 && p.noise && p.step_base) step += *p.step_base;
@@ -333,15 +356,17 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
         }
         const T P = envq[q] * coef;
         const c2<T> ps = s_psi[c];
-        const c2<T> lap = nine ? lap9<T>(ps, s_psi[c + W], s_psi[c - W], s_psi[c + 1], s_psi[c - 1], s_psi[c - W - 1],
-                                         s_psi[c - W + 1], s_psi[c + W - 1], s_psi[c + W + 1], inv_6h2)
-                               : lap5<T>(ps, s_psi[c + W], s_psi[c - W], s_psi[c + 1], s_psi[c - 1], kc.inv_dx2,
-                                         kc.inv_dy2);
+        const c2<T> sx = csum<T>(s_psi[c + 1], s_psi[c - 1]);
+        const c2<T> sy = csum<T>(s_psi[c + W], s_psi[c - W]);
+        const c2<T> K = nine ? kin9<T>(kc, csum<T>(sy, sx),
+                                       csum<T>(csum<T>(csum<T>(s_psi[c - W - 1], s_psi[c - W + 1]), s_psi[c + W - 1]),
+                                               s_psi[c + W + 1]))
+                             : kin5<T>(kc, sx, sy);
         c2<T> vpot;
         if (p.potential) vpot = p.potential[mb.pot_off + idx];
         c2<T> d;
         T dn[2];
-        scalar_rhs<T, RES>(mb, kc, ps, lap, n, P, p.potential ? &vpot : nullptr, d, dn);
+        scalar_rhs<T, RES>(mb, kc, ps, K, centre, n, P, p.potential ? &vpot : nullptr, d, dn);
 
         if constexpr (KIND != 2) {
             st_stream_c<T>(&p.kout[off + idx], d);
@@ -602,7 +627,7 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
     const int64_t off = mb.off;
     const ScalarConsts<T> kc(mb);
     const bool nine = mb.laplacian == PGPE_LAP_NINE;
-    const T inv_6h2 = kc.inv_dx2 / (T)6;
+    const T centre = nine ? kc.c20 : kc.c0;
     T coef[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) coef[s] = p.coef[s * p.members + m];
@@ -642,21 +667,19 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
             if (f >= NP) continue;
             // neighbours clamped into the buffer; border points are stale anyway
             const c2<T> ps = cur[f];
-            const c2<T> pn = cur[f + EW];
-            const c2<T> pso = cur[f - EW];
-            const c2<T> pe = cur[f + 1];
-            const c2<T> pw = cur[f - 1];
-            c2<T> lap;
+            const c2<T> sx = csum<T>(cur[f + 1], cur[f - 1]);
+            const c2<T> sy = csum<T>(cur[f + EW], cur[f - EW]);
+            c2<T> K;
             if (nine) {
-                lap = lap9<T>(ps, pn, pso, pe, pw, cur[f - EW - 1], cur[f - EW + 1], cur[f + EW - 1],
-                              cur[f + EW + 1], inv_6h2);
+                K = kin9<T>(kc, csum<T>(sy, sx),
+                            csum<T>(csum<T>(csum<T>(cur[f - EW - 1], cur[f - EW + 1]), cur[f + EW - 1]), cur[f + EW + 1]));
             } else {
-                lap = lap5<T>(ps, pn, pso, pe, pw, kc.inv_dx2, kc.inv_dy2);
+                K = kin5<T>(kc, sx, sy);
             }
             const T P = env[j] * coef[s];
             c2<T> d;
             T dn[2];
-            scalar_rhs<T, RES>(mb, kc, ps, lap, ns[j], P, nullptr, d, dn);
+            scalar_rhs<T, RES>(mb, kc, ps, K, centre, ns[j], P, nullptr, d, dn);
             if (s == 0) {
                 acc[j] = d;
 #pragma unroll
@@ -715,9 +738,21 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
 // of every column, so a point's y neighbours are the same thread's adjacent registers, its x
 // neighbours the adjacent lanes' (read with DPP wave shifts, no LDS), and only the first and
 // last row of each wavefront cross to the neighbouring wavefronts through LDS, once per stage.
-// Per-point arithmetic is rk4_step_kernel's (lap5 / lap9, scalar_rhs, the same accumulation),
-// so the two whole-step kernels and the stage kernels agree bit for bit on periodic grids.
+// Per-point arithmetic is rk4_step_kernel's (the neighbour sums in the same order, kin5 / kin9,
+// scalar_rhs, the same accumulation), so the two whole-step kernels and the stage kernels agree
+// bit for bit on periodic grids.
 constexpr int kStripW = kWave - 2 * kStepHalo;   // 56 output columns per strip
+
+// Element e of an array at a uniform base, addressed by a 32-bit byte offset (the base in scalar
+// registers, one vector register per access).
+template <typename V>
+__device__ __forceinline__ V& at_offset(V* base, uint32_t e) {
+    return *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + e * (uint32_t)sizeof(V));
+}
+template <typename V>
+__device__ __forceinline__ const V& at_offset(const V* base, uint32_t e) {
+    return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + e * (uint32_t)sizeof(V));
+}
 
 // Value of the next (lane + 1) or previous (lane - 1) lane of the wavefront (DPP wave_shl:1 /
 // wave_shr:1); the last / first lane receives 0, a halo column whose value is never used.
@@ -735,35 +770,25 @@ __device__ __forceinline__ double lane_prev(double v) {
     const uint32_t lo = (uint32_t)lane_prev_i((int)(uint32_t)u), hi = (uint32_t)lane_prev_i((int)(uint32_t)(u >> 32));
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
-// The Laplacians of lap5 / lap9 on lane-held rows, bit for bit: 2 psi is exact, so pe - 2 psi
-// rounds once whether it is evaluated as an FMA (as the compiler contracts lap5) or as the add
-// pe + (-2 psi), and the add lets the compiler fold each wave shift into its consumer
-// (v_add_f32_dpp) instead of a separate move.  The final sum is written as the FMA that lap5 /
-// lap9 contract to.
-template <typename T>
-__device__ __forceinline__ T strip_lap5(T ps, T pn, T pso, T inv_dx2, T inv_dy2) {
-    const T m2 = (T)-2 * ps;
-    const T ex = lane_prev(ps) + (lane_next(ps) + m2);
-    const T ey = (pn + m2) + pso;
-    return __builtin_fma(ey, inv_dy2, ex * inv_dx2);
-}
-__device__ __forceinline__ float strip_lap5(float ps, float pn, float pso, float inv_dx2, float inv_dy2) {
-    const float m2 = -2.0f * ps;
-    const float ex = lane_prev(ps) + (lane_next(ps) + m2);
-    const float ey = (pn + m2) + pso;
-    return __builtin_fmaf(ey, inv_dy2, ex * inv_dx2);
-}
 template <typename T>
 __device__ __forceinline__ c2<T> lane_next_c(const c2<T>& v) { return make_c<T>(lane_next(v.x), lane_next(v.y)); }
 template <typename T>
 __device__ __forceinline__ c2<T> lane_prev_c(const c2<T>& v) { return make_c<T>(lane_prev(v.x), lane_prev(v.y)); }
 
-template <typename T, int RES, int NW, int R, bool ETA, bool NOISE>
+//
+// Rows stale at a stage are skipped: after stage s the outer s + 1 rows of the strip can no
+// longer feed an output point, so at stage s the first wavefront leaves out its rows j <= s and
+// the last its rows j >= R - 1 - s (20 of the 4 NW R row-stages; the choice is per wavefront, so
+// no lane diverges).  A skipped row still passes its stage input on as the next row's south
+// neighbour; that value is stale, and so is every point it reaches.  FIVE compiles the 5-point
+// Laplacian alone, for tables without 9-point members.
+template <typename T, int RES, int NW, int R, bool ETA, bool NOISE, bool FIVE>
 __global__ void __launch_bounds__(NW * kWave) rk4_strip_kernel(const Rk4StepParams<T> p) {
 #pragma clang fp contract(on)
     constexpr int NF = ResFields<RES>::n;
     constexpr int EH = NW * R;
     constexpr int TH = EH - 2 * kStepHalo;
+    static_assert(R >= kStepHalo, "a wavefront must hold at least the halo's rows");
     // first and last row of each wavefront, double buffered over the stages
     __shared__ c2<T> edge[2][NW][2][kWave];
 
@@ -777,20 +802,28 @@ __global__ void __launch_bounds__(NW * kWave) rk4_strip_kernel(const Rk4StepPara
     const int x0 = (local % strips_x) * kStripW;
     const int y0 = (local / strips_x) * TH;
     const int lane = threadIdx.x & (kWave - 1);
-    const int w = threadIdx.x / kWave;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));  // uniform per wavefront
     const int64_t off = mb.off;
     ScalarConsts<T> kc(mb);
     kc.to_scalar_registers();
-    const bool nine = mb.laplacian == PGPE_LAP_NINE;
-    const T inv_6h2 = uniform_value(kc.inv_dx2 / (T)6);
+    const bool nine = !FIVE && mb.laplacian == PGPE_LAP_NINE;
+    const T centre = nine ? kc.c20 : kc.c0;
     T coef[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) coef[s] = uniform_value(p.coef[s * p.members + m]);
 
+    // Addressing: each row's member-local element offset e = gy nx + gx (the wrapped or mirrored
+    // point), as a 32-bit byte offset against the member's uniform base pointers (the host keeps
+    // a strip member below 2^28 points, so the offsets fit); an output point's offset is its own
+    // index, since output rows and columns never wrap.
     const int gx = map_index(x0 - kStepHalo + lane, nx, mb.boundary);
     const bool periodic = mb.boundary != PGPE_BC_CLOSED;
+    const bool lane_out = lane >= kStepHalo && lane < kStepHalo + kStripW && x0 - kStepHalo + lane < nx;
+    const c2<T>* __restrict__ psi_m = p.psi0 + off;
+    const T* __restrict__ env_m = p.pump + mb.pump_off;
     c2<T> z[R], cur[R], acc[R];
     T env[R], n0[R][2], ns[R][2], accn[R][2];
+    uint32_t e[R];
     int gy = map_index(y0 - kStepHalo + w * R, ny, mb.boundary);
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -799,13 +832,13 @@ __global__ void __launch_bounds__(NW * kWave) rk4_strip_kernel(const Rk4StepPara
             if (periodic) gy = gy + 1 == ny ? 0 : gy + 1;
             else gy = map_index(y0 - kStepHalo + w * R + j, ny, mb.boundary);
         }
-        const int64_t g = (int64_t)gy * nx + gx;
-        z[j] = p.psi0[off + g];
+        e[j] = (uint32_t)(gy * nx + gx);
+        z[j] = at_offset(psi_m, e[j]);
         cur[j] = z[j];
-        env[j] = p.pump[mb.pump_off + g];
+        env[j] = at_offset(env_m, e[j]);
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
-            n0[j][q] = p.res0[q][off + g];
+            n0[j][q] = at_offset(p.res0[q] + off, e[j]);
             ns[j][q] = n0[j][q];
         }
     }
@@ -824,21 +857,28 @@ __global__ void __launch_bounds__(NW * kWave) rk4_strip_kernel(const Rk4StepPara
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             const c2<T> ps = cur[j];
+            if ((w == 0 && j <= s) || (w == NW - 1 && j >= R - 1 - s)) {
+                south = ps;
+                continue;
+            }
             const c2<T> pn = j + 1 < R ? cur[j + 1] : above;
             const c2<T> pso = south;
-            c2<T> lap;
+            // x neighbours from the adjacent lanes (DPP wave shifts), y neighbours from this lane
+            const c2<T> sx = csum<T>(lane_next_c<T>(ps), lane_prev_c<T>(ps));
+            const c2<T> sy = csum<T>(pn, pso);
+            c2<T> K;
             if (nine) {
-                lap = lap9<T>(ps, pn, pso, lane_next_c<T>(ps), lane_prev_c<T>(ps), lane_prev_c<T>(pso),
-                              lane_next_c<T>(pso), lane_prev_c<T>(pn), lane_next_c<T>(pn), inv_6h2);
+                K = kin9<T>(kc, csum<T>(sy, sx),
+                            csum<T>(csum<T>(csum<T>(lane_prev_c<T>(pso), lane_next_c<T>(pso)), lane_prev_c<T>(pn)),
+                                    lane_next_c<T>(pn)));
             } else {
-                lap.x = strip_lap5(ps.x, pn.x, pso.x, kc.inv_dx2, kc.inv_dy2);
-                lap.y = strip_lap5(ps.y, pn.y, pso.y, kc.inv_dx2, kc.inv_dy2);
+                K = kin5<T>(kc, sx, sy);
             }
             south = ps;
             const T P = env[j] * coef[s];
             c2<T> d;
             T dn[2];
-            scalar_rhs<T, RES, ETA>(mb, kc, ps, lap, ns[j], P, nullptr, d, dn);
+            scalar_rhs<T, RES, ETA>(mb, kc, ps, K, centre, ns[j], P, nullptr, d, dn);
             if (s == 0) {
                 acc[j] = d;
 #pragma unroll
@@ -855,12 +895,8 @@ __global__ void __launch_bounds__(NW * kWave) rk4_strip_kernel(const Rk4StepPara
 #pragma unroll
                 for (int q = 0; q < NF; ++q) ns[j][q] = n0[j][q] + a_next * dn[q];
             } else {
-                const int ly = w * R + j;
-                const int ox = x0 + lane - kStepHalo, oy = y0 + ly - kStepHalo;
-                if (lane < kStepHalo || lane >= kStepHalo + kStripW || ly < kStepHalo || ly >= kStepHalo + TH ||
-                    ox >= nx || oy >= ny)
-                    continue;
-                const int64_t idx = (int64_t)oy * nx + ox;
+                const int ly = w * R + j;  // uniform per wavefront
+                if (ly < kStepHalo || ly >= kStepHalo + TH || y0 + ly - kStepHalo >= ny || !lane_out) continue;
                 c2<T> out;
                 out.x = z[j].x + p.dt6 * (acc[j].x + d.x);
                 out.y = z[j].y + p.dt6 * (acc[j].y + d.y);
@@ -870,34 +906,40 @@ __global__ void __launch_bounds__(NW * kWave) rk4_strip_kernel(const Rk4StepPara
                     T v = n0[j][q] + p.dt6 * (accn[j][q] + dn[q]);
                     v = v > (T)0 ? v : (T)0;  // set_state clamp (single_reservoir.py:70-73)
                     n_new[q] = v;
-                    p.res_out[q][off + idx] = v;
+                    at_offset(p.res_out[q] + off, e[j]) = v;
                 }
                 if (NOISE) {
                     uint64_t step = p.step;
                     if (p.step_base) step += *p.step_base;
-                    wigner_site<T>(out.x, out.y, mb.seed, (uint32_t)idx, step, 1u, (T)mb.noise_strength,
+                    wigner_site<T>(out.x, out.y, mb.seed, e[j], step, 1u, (T)mb.noise_strength,
                                    (T)mb.noise_gamma, (T)mb.noise_R, NF > 0 ? n_new[0] : (T)0,
                                    (T)mb.noise_dt_over_4v);
                 }
-                p.psi_out[off + idx] = out;
+                at_offset(p.psi_out + off, e[j]) = out;
             }
         }
     }
 }
 
-template <typename T, int RES, int NW, int R>
-static int launch_strip_r(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
+template <typename T, int RES, int NW, int R, bool FIVE>
+static void launch_strip_f(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
     // the kinetic-relaxation term and the Wigner increment are compiled in only when used: each
     // costs registers (the noise generator ~25 per lane), and fewer lift the occupancy
     const dim3 grid(xcd_grid(a->num_tiles)), block(NW * kWave);
     if (a->no_eta && !a->noise)
-        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, false, false>), grid, block, 0, s, p);
+        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, false, false, FIVE>), grid, block, 0, s, p);
     else if (a->no_eta)
-        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, false, true>), grid, block, 0, s, p);
+        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, false, true, FIVE>), grid, block, 0, s, p);
     else if (!a->noise)
-        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, true, false>), grid, block, 0, s, p);
+        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, true, false, FIVE>), grid, block, 0, s, p);
     else
-        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, true, true>), grid, block, 0, s, p);
+        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, true, true, FIVE>), grid, block, 0, s, p);
+}
+
+template <typename T, int RES, int NW, int R>
+static int launch_strip_r(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
+    if (a->five_point) launch_strip_f<T, RES, NW, R, true>(a, p, s);
+    else launch_strip_f<T, RES, NW, R, false>(a, p, s);
     PGPE_LAUNCHED("rk4_strip_kernel");
     return PGPE_OK;
 }

@@ -18,6 +18,8 @@ from . import _native as N
 from .device import DeviceArray, Graph, Stream
 from .layout import MemberTable
 
+STRIP_MAX_POINTS = 1 << 28   # per member on row strips: 32-bit byte offsets of complex128 values
+
 
 def real_dtype(precision):
     return np.float32 if precision == "single" else np.float64
@@ -59,6 +61,9 @@ class Rk4Engine:
                 raise ValueError("row strips run the whole-step kernel (whole_step=True)")
             if self.strip_h not in (16, 24, 32, 40) or (self.strip_h > 24 and precision != "single"):
                 raise ValueError("strip_h must be 16 or 24 (or 32 and 40 in single precision)")
+            # the strip kernel addresses a member with 32-bit byte offsets
+            if any(s.nx * s.ny > STRIP_MAX_POINTS for s in table.specs):
+                raise ValueError(f"row strips take members of at most {STRIP_MAX_POINTS} points")
         self.psi = [DeviceArray.empty((n,), c), DeviceArray.empty((n,), c)]
         stage_bufs = 0 if self.whole_step else 3
         self.k = [DeviceArray.empty((n,), c) for _ in range(stage_bufs)]
@@ -121,6 +126,7 @@ class Rk4Engine:
         a.tile_y = tab.tile_h if self.strip_h is None else self.strip_h
         a.strips = 0 if self.strip_h is None else 1
         a.no_eta = 1 if all(r.eta == 0.0 for r in tab.rows) else 0
+        a.five_point = 1 if all(r.laplacian == N.LAPLACIAN_KIND["five-point"] for r in tab.rows) else 0
         a.members = tab.members
         a.components = self.components
         a.res_components = self.res_components

@@ -61,12 +61,20 @@ def test_plan_rejects_unsupported_size():
     init()
     with pytest.raises(RuntimeError, match="sizes"):
         _plan(100, 64, "double")
+    # 6-point complex128 rows do not fit a row group's LDS: rejected by the plan, not at launch
+    with pytest.raises(RuntimeError, match="short"):
+        _plan(6, 64, "double")
+    _plan(6, 64, "single")
 
 
-@pytest.mark.parametrize("ny,nx,B", [(2048, 16, 40), (2048, 64, 64), (1024, 32, 96)])
+@pytest.mark.parametrize("ny,nx,B", [(2048, 16, 40), (2048, 64, 64), (2048, 12, 400), (4096, 32, 64),
+                                     (1024, 32, 96), (1024, 3, 700)])
 @pytest.mark.parametrize("precision,tol", [("double", 1e-14), ("single", 2e-6)])
 def test_fft2_tall_columns_in_large_batches(ny, nx, B, precision, tol):
-    """Large batches take wide column groups (up to 64 elements per thread, 64-byte row segments)."""
+    """Large batches take wide column groups (up to 64 elements per thread, 64-byte row segments).
+    Batches of >= 512 groups of complex64 columns of 2048 / 4096 points or complex128 columns of
+    1024 points split each column in two halves (fft_cols_half_kernel), including groups that hold
+    fewer columns than the full group (nx = 12, 3)."""
     from polarism_amd import _native as N
     from polarism_amd.device import DeviceArray, init
     init()
@@ -82,8 +90,8 @@ def test_fft2_tall_columns_in_large_batches(ny, nx, B, precision, tol):
     assert err < tol, err
 
 
-@pytest.mark.parametrize("ny,nx", [(512, 512), (64, 256), (2048, 64)])
-def test_fp32_round_trips_keep_the_norm(ny, nx):
+@pytest.mark.parametrize("ny,nx,B", [(512, 512, 1), (64, 256, 1), (2048, 64, 1), (2048, 64, 64)])
+def test_fp32_round_trips_keep_the_norm(ny, nx, B):
     """120 forward + inverse FP32 round trips: the energy drifts by less than 1e-6 and the field
     by less than 1e-5 (rounding only).  FP32 twiddles rounded without tails lose ~8e-8 of the
     energy per 2-D transform on 512^2 (1.5e-5 over these round trips); numpy's complex64
@@ -92,15 +100,15 @@ def test_fp32_round_trips_keep_the_norm(ny, nx):
     from polarism_amd.device import DeviceArray, init
     init()
     rng = np.random.default_rng(5)
-    x = (rng.standard_normal((1, ny, nx)) + 1j * rng.standard_normal((1, ny, nx))).astype(np.complex64)
+    x = (rng.standard_normal((B, ny, nx)) + 1j * rng.standard_normal((B, ny, nx))).astype(np.complex64)
     plan = _plan(nx, ny, "single")
     d = DeviceArray.from_numpy(x)
     for _ in range(120):
-        N.call("pgpe_fft2", plan.handle, d.ptr, d.ptr, 1, 0, None)
-        N.call("pgpe_fft2", plan.handle, d.ptr, d.ptr, 1, 1, None)
+        N.call("pgpe_fft2", plan.handle, d.ptr, d.ptr, B, 0, None)
+        N.call("pgpe_fft2", plan.handle, d.ptr, d.ptr, B, 1, None)
     got = d.get().astype(np.complex128)
     x64 = x.astype(np.complex128)
     drift = np.sum(np.abs(got) ** 2) / np.sum(np.abs(x64) ** 2) - 1
     err = np.linalg.norm(got - x64) / np.linalg.norm(x64)
-    print(f"{ny}x{nx}: 120 FP32 round trips: energy drift {drift:.2e}, field error {err:.2e}")
+    print(f"{B} x {ny}x{nx}: 120 FP32 round trips: energy drift {drift:.2e}, field error {err:.2e}")
     assert abs(drift) < 1e-6 and err < 1e-5
