@@ -22,6 +22,8 @@
 // -ffp-contract=off, and the RK4 point arithmetic (stencil sums, right-hand side,
 // stage combinations) opts into fused multiply-adds per source expression (clang
 // fp contract(on)), so the stage and whole-step kernels fuse the same operations.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace pgpe {
@@ -782,27 +784,85 @@ __device__ __forceinline__ c2<T> lane_prev_c(const c2<T>& v) { return make_c<T>(
 // no lane diverges).  A skipped row still passes its stage input on as the next row's south
 // neighbour; that value is stale, and so is every point it reaches.  FIVE compiles the 5-point
 // Laplacian alone, for tables without 9-point members.
+// A read-only array the kernel never writes, addressed through the constant address space so that
+// uniform indices load through the scalar unit (member rows, launch maps, coefficients).
+template <typename V>
+__device__ __forceinline__ const __attribute__((address_space(4))) V* kconst(const V* ptr) {
+    return (const __attribute__((address_space(4))) V*)ptr;
+}
+
+// A copy of a read-only record (a member row) through the constant address space: dword by dword,
+// which the compiler merges into wide scalar loads when the address is uniform.
+template <typename V>
+__device__ __forceinline__ V load_const(const V* ptr) {
+    static_assert(sizeof(V) % 4 == 0, "records are whole dwords");
+    V out;
+    const __attribute__((address_space(4))) uint32_t* s = (const __attribute__((address_space(4))) uint32_t*)ptr;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(V) / 4); ++i) d[i] = s[i];
+    return out;
+}
+
+// Workgroup barrier for the strips' LDS exchange: this wavefront's LDS operations complete
+// (lgkmcnt(0)), then s_barrier, with compiler-only fences on both sides so no memory access moves
+// across it.  Its outstanding global loads, including LDS copies of a prefetch, stay in flight:
+// __syncthreads, or a workgroup fence even restricted to LDS, would wait for those as well.  A
+// strip wavefront reads no LDS another wavefront fills by copy, so nothing needs them here.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // s_waitcnt lgkmcnt(0)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Where a strip sits: member, member row, first output column and row.
+struct StripPos {
+    int m, x0, y0;
+};
+
+template <typename T, int TH>
+__device__ __forceinline__ StripPos strip_pos(const Rk4StepParams<T>& p, int t) {
+    StripPos s;
+    s.m = kconst(p.tile_map)[2 * t];
+    const int local = kconst(p.tile_map)[2 * t + 1];
+    const int strips_x = (kconst(p.table)[s.m].nx + kStripW - 1) / kStripW;
+    s.x0 = (local % strips_x) * kStripW;
+    s.y0 = (local / strips_x) * TH;
+    return s;
+}
+
+// Member-local element offsets of a wavefront's R rows in a strip, e = gy nx + gx of the wrapped or
+// mirrored point (32-bit: the host keeps a strip member below 2^28 points).  An output point's
+// offset is its own index, since output rows and columns never wrap.
+template <int R>
+__device__ __forceinline__ void strip_offsets(int nx, int ny, int boundary, int x0, int y0, int w, int lane,
+                                              uint32_t e[R]) {
+    const int gx = map_index(x0 - kStepHalo + lane, nx, boundary);
+    const bool periodic = boundary != PGPE_BC_CLOSED;
+    int gy = map_index(y0 - kStepHalo + w * R, ny, boundary);
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        if (j > 0) {
+            // the next row: periodic rows step with a wrap; mirrored rows need the full map
+            if (periodic) gy = gy + 1 == ny ? 0 : gy + 1;
+            else gy = map_index(y0 - kStepHalo + w * R + j, ny, boundary);
+        }
+        e[j] = (uint32_t)(gy * nx + gx);
+    }
+}
+
+// The four stages of one strip from psi0 (z), the pump envelope and n0 of the wavefront's R rows
+// held in registers, and the stores of its output points.  Both strip kernels run this.
 template <typename T, int RES, int NW, int R, bool ETA, bool NOISE, bool FIVE>
-__global__ void __launch_bounds__(NW * kWave) rk4_strip_kernel(const Rk4StepParams<T> p) {
+__device__ __forceinline__ void strip_stages(const Rk4StepParams<T>& p, c2<T> (&edge)[2][NW][2][kWave],
+                                             const pgpe_member& mb, int m, int x0, int y0, int w, int lane,
+                                             const uint32_t (&e)[R], const c2<T> (&z)[R], const T (&env)[R],
+                                             const T (&n0)[R][2]) {
 #pragma clang fp contract(on)
     constexpr int NF = ResFields<RES>::n;
-    constexpr int EH = NW * R;
-    constexpr int TH = EH - 2 * kStepHalo;
-    static_assert(R >= kStepHalo, "a wavefront must hold at least the halo's rows");
-    // first and last row of each wavefront, double buffered over the stages
-    __shared__ c2<T> edge[2][NW][2][kWave];
-
-    const int tile = xcd_tile(blockIdx.x, p.active_tiles ? *p.active_tiles : p.num_tiles);
-    if (tile < 0) return;
-    const int m = p.tile_map[2 * tile];
-    const pgpe_member mb = p.table[m];
+    constexpr int TH = NW * R - 2 * kStepHalo;
     const int nx = mb.nx, ny = mb.ny;
-    const int local = p.tile_map[2 * tile + 1];
-    const int strips_x = (nx + kStripW - 1) / kStripW;
-    const int x0 = (local % strips_x) * kStripW;
-    const int y0 = (local / strips_x) * TH;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));  // uniform per wavefront
     const int64_t off = mb.off;
     ScalarConsts<T> kc(mb);
     kc.to_scalar_registers();
@@ -810,37 +870,15 @@ __global__ void __launch_bounds__(NW * kWave) rk4_strip_kernel(const Rk4StepPara
     const T centre = nine ? kc.c20 : kc.c0;
     T coef[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) coef[s] = uniform_value(p.coef[s * p.members + m]);
-
-    // Addressing: each row's member-local element offset e = gy nx + gx (the wrapped or mirrored
-    // point), as a 32-bit byte offset against the member's uniform base pointers (the host keeps
-    // a strip member below 2^28 points, so the offsets fit); an output point's offset is its own
-    // index, since output rows and columns never wrap.
-    const int gx = map_index(x0 - kStepHalo + lane, nx, mb.boundary);
-    const bool periodic = mb.boundary != PGPE_BC_CLOSED;
+    for (int s = 0; s < 4; ++s) coef[s] = kconst(p.coef)[s * p.members + m];
     const bool lane_out = lane >= kStepHalo && lane < kStepHalo + kStripW && x0 - kStepHalo + lane < nx;
-    const c2<T>* __restrict__ psi_m = p.psi0 + off;
-    const T* __restrict__ env_m = p.pump + mb.pump_off;
-    c2<T> z[R], cur[R], acc[R];
-    T env[R], n0[R][2], ns[R][2], accn[R][2];
-    uint32_t e[R];
-    int gy = map_index(y0 - kStepHalo + w * R, ny, mb.boundary);
+    c2<T> cur[R], acc[R];
+    T ns[R][2], accn[R][2];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        if (j > 0) {
-            // the next row: periodic rows step with a wrap; mirrored rows need the full map
-            if (periodic) gy = gy + 1 == ny ? 0 : gy + 1;
-            else gy = map_index(y0 - kStepHalo + w * R + j, ny, mb.boundary);
-        }
-        e[j] = (uint32_t)(gy * nx + gx);
-        z[j] = at_offset(psi_m, e[j]);
         cur[j] = z[j];
-        env[j] = at_offset(env_m, e[j]);
 #pragma unroll
-        for (int q = 0; q < NF; ++q) {
-            n0[j][q] = at_offset(p.res0[q] + off, e[j]);
-            ns[j][q] = n0[j][q];
-        }
+        for (int q = 0; q < NF; ++q) ns[j][q] = n0[j][q];
     }
 
 #pragma unroll
@@ -848,7 +886,7 @@ __global__ void __launch_bounds__(NW * kWave) rk4_strip_kernel(const Rk4StepPara
         const T a_next = s < 2 ? p.a_half : p.a_full;
         edge[s & 1][w][0][lane] = cur[0];
         edge[s & 1][w][1][lane] = cur[R - 1];
-        __syncthreads();
+        lds_barrier();
         // the first wavefront has no row below its first and the last none above its last: they
         // read their own edge rows instead, which only feed halo points that are stale by the end
         // of the step (a plain clamped index: no select between LDS and register values)
@@ -910,7 +948,7 @@ __global__ void __launch_bounds__(NW * kWave) rk4_strip_kernel(const Rk4StepPara
                 }
                 if (NOISE) {
                     uint64_t step = p.step;
-                    if (p.step_base) step += *p.step_base;
+                    if (p.step_base) step += *kconst(p.step_base);
                     wigner_site<T>(out.x, out.y, mb.seed, e[j], step, 1u, (T)mb.noise_strength,
                                    (T)mb.noise_gamma, (T)mb.noise_R, NF > 0 ? n_new[0] : (T)0,
                                    (T)mb.noise_dt_over_4v);
@@ -921,19 +959,146 @@ __global__ void __launch_bounds__(NW * kWave) rk4_strip_kernel(const Rk4StepPara
     }
 }
 
+// One strip per workgroup, psi0 / n0 / P loaded straight into registers (double precision).
+template <typename T, int RES, int NW, int R, bool ETA, bool NOISE, bool FIVE>
+__global__ void __launch_bounds__(NW * kWave) rk4_strip_kernel(const Rk4StepParams<T> p) {
+    constexpr int NF = ResFields<RES>::n;
+    constexpr int TH = NW * R - 2 * kStepHalo;
+    static_assert(R >= kStepHalo, "a wavefront must hold at least the halo's rows");
+    // first and last row of each wavefront, double buffered over the stages
+    __shared__ c2<T> edge[2][NW][2][kWave];
+
+    const int tile = xcd_tile(blockIdx.x, p.active_tiles ? *kconst(p.active_tiles) : p.num_tiles);
+    if (tile < 0) return;
+    const StripPos sp = strip_pos<T, TH>(p, tile);
+    const pgpe_member mb = load_const(p.table + sp.m);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));  // uniform per wavefront
+    uint32_t e[R];
+    strip_offsets<R>(mb.nx, mb.ny, mb.boundary, sp.x0, sp.y0, w, lane, e);
+    const c2<T>* __restrict__ psi_m = p.psi0 + mb.off;
+    const T* __restrict__ env_m = p.pump + mb.pump_off;
+    c2<T> z[R];
+    T env[R], n0[R][2];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        z[j] = at_offset(psi_m, e[j]);
+        env[j] = at_offset(env_m, e[j]);
+#pragma unroll
+        for (int q = 0; q < NF; ++q) n0[j][q] = at_offset(p.res0[q] + mb.off, e[j]);
+    }
+    strip_stages<T, RES, NW, R, ETA, NOISE, FIVE>(p, edge, mb, sp.m, sp.x0, sp.y0, w, lane, e, z, env, n0);
+}
+
+// Single precision: persistent workgroups over the strips of their XCD's share of the launch map,
+// with the next strip's psi0 / P / n0 loaded while the current one is computed.  Workgroup b runs
+// on XCD b mod 8 and takes strips x per + b / 8, + G / 8, ... of that XCD's contiguous range
+// [x per, (x + 1) per), per = ceil(count / 8), so neighbouring strips still meet in one L2.  Each
+// wavefront issues the loads of its R rows of the next strip into a second set of registers
+// before the four stages of the current one and moves them over when that strip starts, so the
+// loads run under the arithmetic instead of ahead of it (the stage barriers wait for LDS only:
+// lds_barrier).  The launch holds as many workgroups per CU as are resident at once (the
+// occupancy of the instantiation: its registers decide), so every workgroup starts at once and
+// the strips divide evenly.
+
+template <int RES, int NW, int R, bool ETA, bool NOISE, bool FIVE>
+__global__ void __launch_bounds__(NW * kWave) rk4_strip_pf_kernel(const Rk4StepParams<float> p) {
+    using T = float;
+    constexpr int NF = ResFields<RES>::n;
+    constexpr int TH = NW * R - 2 * kStepHalo;
+    static_assert(R >= kStepHalo, "a wavefront must hold at least the halo's rows");
+    __shared__ c2<T> edge[2][NW][2][kWave];
+
+    const int count = p.active_tiles ? *kconst(p.active_tiles) : p.num_tiles;
+    const int xcd = blockIdx.x % kXcds;
+    const int gpx = gridDim.x / kXcds;
+    const int per = (count + kXcds - 1) / kXcds;
+    const int t_end = min((xcd + 1) * per, count);
+    int t = xcd * per + (int)(blockIdx.x / kXcds);
+    if (t >= t_end) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));  // uniform per wavefront
+
+    // this wavefront's rows of strip tt, loaded into the prefetch registers
+    c2<T> zq[R];
+    T envq[R], n0q[R][2];
+    auto fetch = [&](int tt) {
+        const StripPos sp = strip_pos<T, TH>(p, tt);
+        const auto& mb = kconst(p.table)[sp.m];
+        uint32_t e[R];
+        strip_offsets<R>(mb.nx, mb.ny, mb.boundary, sp.x0, sp.y0, w, lane, e);
+        const c2<T>* psi_m = p.psi0 + mb.off;
+        const T* env_m = p.pump + mb.pump_off;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            zq[j] = at_offset(psi_m, e[j]);
+            envq[j] = at_offset(env_m, e[j]);
+#pragma unroll
+            for (int q = 0; q < NF; ++q) n0q[j][q] = at_offset(p.res0[q] + mb.off, e[j]);
+        }
+    };
+
+    fetch(t);
+    for (; t < t_end; t += gpx) {
+        const StripPos sp = strip_pos<T, TH>(p, t);
+        const pgpe_member mb = load_const(p.table + sp.m);
+        uint32_t e[R];
+        strip_offsets<R>(mb.nx, mb.ny, mb.boundary, sp.x0, sp.y0, w, lane, e);
+        c2<T> z[R];
+        T env[R], n0[R][2];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            z[j] = zq[j];
+            env[j] = envq[j];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) n0[j][q] = n0q[j][q];
+        }
+        if (t + gpx < t_end) fetch(t + gpx);
+        strip_stages<T, RES, NW, R, ETA, NOISE, FIVE>(p, edge, mb, sp.m, sp.x0, sp.y0, w, lane, e, z, env, n0);
+    }
+}
+
+// CUs of the current device, asked once per device.
+static int device_cus() {
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
+template <typename T, int RES, int NW, int R, bool ETA, bool NOISE, bool FIVE>
+static void launch_strip_k(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
+    const dim3 block(NW * kWave);
+    if constexpr (sizeof(T) == 4) {
+        // persistent: the resident workgroups of every CU (at most one per strip), a multiple of
+        // the XCD count
+        auto kern = rk4_strip_pf_kernel<RES, NW, R, ETA, NOISE, FIVE>;
+        static const int per_cu = [&] {
+            int n = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, NW * kWave, 0) != hipSuccess || n < 1) n = 1;
+            return n;
+        }();
+        const int grid = std::min(xcd_grid(a->num_tiles), xcd_grid(per_cu * device_cus()));
+        hipLaunchKernelGGL(kern, dim3(grid), block, 0, s, p);
+    } else {
+        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, ETA, NOISE, FIVE>), dim3(xcd_grid(a->num_tiles)), block,
+                           0, s, p);
+    }
+}
+
 template <typename T, int RES, int NW, int R, bool FIVE>
 static void launch_strip_f(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
     // the kinetic-relaxation term and the Wigner increment are compiled in only when used: each
     // costs registers (the noise generator ~25 per lane), and fewer lift the occupancy
-    const dim3 grid(xcd_grid(a->num_tiles)), block(NW * kWave);
-    if (a->no_eta && !a->noise)
-        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, false, false, FIVE>), grid, block, 0, s, p);
-    else if (a->no_eta)
-        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, false, true, FIVE>), grid, block, 0, s, p);
-    else if (!a->noise)
-        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, true, false, FIVE>), grid, block, 0, s, p);
-    else
-        hipLaunchKernelGGL((rk4_strip_kernel<T, RES, NW, R, true, true, FIVE>), grid, block, 0, s, p);
+    if (a->no_eta && !a->noise) launch_strip_k<T, RES, NW, R, false, false, FIVE>(a, p, s);
+    else if (a->no_eta) launch_strip_k<T, RES, NW, R, false, true, FIVE>(a, p, s);
+    else if (!a->noise) launch_strip_k<T, RES, NW, R, true, false, FIVE>(a, p, s);
+    else launch_strip_k<T, RES, NW, R, true, true, FIVE>(a, p, s);
 }
 
 template <typename T, int RES, int NW, int R>
