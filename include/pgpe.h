@@ -188,7 +188,7 @@ typedef struct pgpe_rk4_args {
     int32_t strips;                  /* pgpe_rk4_step only: 0 = 64 x tile_y tiles of the member
                                         table; 1 = row strips of 56 output columns x tile_y rows
                                         (16, 24, or 32 and 40 in single precision), local index
-                                        (row * ceil(nx / 56) + column), from the strip map; every
+                                        (row << 16) | column, from the strip map; every
                                         member must hold at most 2^28 points (the kernel
                                         addresses a member with 32-bit byte offsets) */
     int32_t no_eta;                  /* pgpe_rk4_step with strips only: 1 promises that no member

@@ -162,7 +162,7 @@ struct ScalarConsts {
         hG = (T)(0.5 * mb.gamma_C);
         inv_hbar = (T)mb.inv_hbar;
         eta_h = (T)(mb.eta * mb.hbar_over_2m);
-        lap_scale = (T)(1.0 / kh);
+        lap_scale = mb.eta != 0.0 ? (T)(1.0 / kh) : (T)0;  // used by the eta term only
     }
     // The constants are converted from the member row's doubles by vector instructions, so they
     // land in vector registers; a kernel short of them moves the (uniform) values to scalar ones.
@@ -821,14 +821,14 @@ struct StripPos {
     int m, x0, y0;
 };
 
+// A strip map entry is (member, (row << 16) | column).
 template <typename T, int TH>
 __device__ __forceinline__ StripPos strip_pos(const Rk4StepParams<T>& p, int t) {
     StripPos s;
     s.m = kconst(p.tile_map)[2 * t];
     const int local = kconst(p.tile_map)[2 * t + 1];
-    const int strips_x = (kconst(p.table)[s.m].nx + kStripW - 1) / kStripW;
-    s.x0 = (local % strips_x) * kStripW;
-    s.y0 = (local / strips_x) * TH;
+    s.x0 = (local & 0xFFFF) * kStripW;
+    s.y0 = (local >> 16) * TH;
     return s;
 }
 
@@ -1019,35 +1019,37 @@ __global__ void __launch_bounds__(NW * kWave) rk4_strip_pf_kernel(const Rk4StepP
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));  // uniform per wavefront
 
-    // this wavefront's rows of strip tt, loaded into the prefetch registers
+    // this wavefront's rows of strip tt, loaded into the prefetch registers, with the strip's
+    // position and row offsets kept for when it is computed
     c2<T> zq[R];
     T envq[R], n0q[R][2];
+    StripPos sq;
+    uint32_t eq[R];
     auto fetch = [&](int tt) {
-        const StripPos sp = strip_pos<T, TH>(p, tt);
-        const auto& mb = kconst(p.table)[sp.m];
-        uint32_t e[R];
-        strip_offsets<R>(mb.nx, mb.ny, mb.boundary, sp.x0, sp.y0, w, lane, e);
+        sq = strip_pos<T, TH>(p, tt);
+        const auto& mb = kconst(p.table)[sq.m];
+        strip_offsets<R>(mb.nx, mb.ny, mb.boundary, sq.x0, sq.y0, w, lane, eq);
         const c2<T>* psi_m = p.psi0 + mb.off;
         const T* env_m = p.pump + mb.pump_off;
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-            zq[j] = at_offset(psi_m, e[j]);
-            envq[j] = at_offset(env_m, e[j]);
+            zq[j] = at_offset(psi_m, eq[j]);
+            envq[j] = at_offset(env_m, eq[j]);
 #pragma unroll
-            for (int q = 0; q < NF; ++q) n0q[j][q] = at_offset(p.res0[q] + mb.off, e[j]);
+            for (int q = 0; q < NF; ++q) n0q[j][q] = at_offset(p.res0[q] + mb.off, eq[j]);
         }
     };
 
     fetch(t);
     for (; t < t_end; t += gpx) {
-        const StripPos sp = strip_pos<T, TH>(p, t);
+        const StripPos sp = sq;
         const pgpe_member mb = load_const(p.table + sp.m);
         uint32_t e[R];
-        strip_offsets<R>(mb.nx, mb.ny, mb.boundary, sp.x0, sp.y0, w, lane, e);
         c2<T> z[R];
         T env[R], n0[R][2];
 #pragma unroll
         for (int j = 0; j < R; ++j) {
+            e[j] = eq[j];
             z[j] = zq[j];
             env[j] = envq[j];
 #pragma unroll

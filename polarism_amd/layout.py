@@ -124,15 +124,18 @@ class MemberTable:
         return self.tile_map_host[keep]
 
     def strip_pairs(self, strip_h: int, members=None) -> np.ndarray:
-        """(member, local strip) pairs of the row-strip whole-step launch: member m is cut into
-        ceil(nx / STRIP_W) x ceil(ny / strip_h) strips of STRIP_W output columns, numbered row
-        by row; ``members`` (default all) selects whose strips, in table order."""
+        """(member, strip) pairs of the row-strip whole-step launch: member m is cut into
+        ceil(nx / STRIP_W) x ceil(ny / strip_h) strips of STRIP_W output columns, listed row by
+        row, each as (row << 16) | column; ``members`` (default all) selects whose strips, in
+        table order."""
         keep = range(self.members) if members is None else sorted(int(m) for m in members)
         out = []
         for m in keep:
             s = self.specs[m]
-            n = (-(-s.nx // STRIP_W)) * (-(-s.ny // int(strip_h)))
-            out.append(np.stack([np.full(n, m, dtype=np.int32), np.arange(n, dtype=np.int32)], axis=1))
+            cols, rows = -(-s.nx // STRIP_W), -(-s.ny // int(strip_h))
+            r, c = np.divmod(np.arange(rows * cols, dtype=np.int64), cols)
+            local = ((r << 16) | c).astype(np.int32)
+            out.append(np.stack([np.full(rows * cols, m, dtype=np.int32), local], axis=1))
         return np.concatenate(out) if out else np.zeros((0, 2), dtype=np.int32)
 
     def member_points(self, m: int) -> int:
