@@ -203,7 +203,7 @@ def roofline_leg(n=8192, reps=20):
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "traffic_note": "no counters in this run; FETCH_SIZE / WRITE_SIZE of the same launches are in "
-                            "profiles/rk4_strip_kernel_sweep256.md",
+                            "profiles/rk4_strip_pf_kernel_sweep256.md",
             "kernel": name, "workload": f"256 members of 256x256 (sweep table), {shape}, {blocks} workgroups",
             "avg_us": round(ms * 1e3, 2), "algorithmic_bytes_per_launch": int(per_launch),
             "bytes_per_point_step": 28, "point_steps_per_s": round(points / (ms * 1e-3), 1),
