@@ -78,14 +78,14 @@ inline void pass_twiddles(int n, const std::vector<FftPass>& passes, const std::
 enum RowShape { ROWS_64x4, ROWS_128x4, ROWS_256x4, ROWS_256x8, ROWS_256x12, ROWS_256x16 };
 
 // One row per group at 4 elements per thread on 64, 128 or 256 threads (n <= 1024: more,
-// smaller workgroups on small grids, several rows per group below 256), then 8 or 16
-// elements per thread; lengths with a radix 3 take 256 x 12.
+// smaller workgroups on small grids, several rows per group below 256), then 16 elements per
+// thread (two 2048-point rows per group: the staged twiddles serve both, and the radix-16 plan
+// takes three passes); lengths with a radix 3 take 256 x 12.
 inline int row_shape(int n) {
     if (n % 3 == 0) return ROWS_256x12;
     if (n <= 256) return ROWS_64x4;
     if (n <= 512) return ROWS_128x4;
     if (n <= 1024) return ROWS_256x4;
-    if (n <= 2048) return ROWS_256x8;
     return ROWS_256x16;
 }
 
