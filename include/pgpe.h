@@ -301,14 +301,25 @@ typedef struct pgpe_split_args {
     void* tmp;              /* scratch of the same size */
     void* res[2];           /* in/out reservoir fields, members x res_components x (ny, nx) */
     const void* pump;       /* envelopes (real) */
-    const void* coef;       /* 2 * members reals: pump coefficient at t+dt/4, then at t+3dt/4 */
+    const void* coef;       /* 2 * members reals: pump coefficient at t+dt/4, then at t+3dt/4;
+                               3 * members with PGPE_CHAIN_OUT (row 2: the next step's t+dt/4) */
     uint64_t step;          /* noise counter step = (*step_base if set) + step */
     const uint64_t* step_base;
     int32_t res_components; /* spinor: 2 = one reservoir per component driven by its density and
                                its weighted pump, 1 = one shared reservoir driven by the total
                                density (split_step_fft_solver.py:81-94); 0 reads as 1 */
-    int32_t pad1;
+    int32_t chain;          /* 0, or bits of pgpe_split_chain for consecutive steps of the scalar
+                               CPU order (ordering 0): the field stays in tmp between them, and
+                               psi holds the state only after a step without PGPE_CHAIN_OUT */
 } pgpe_split_args;
+
+/* Chained split steps: one row pass instead of two between consecutive steps (the last pass of
+ * a step and the first of the next, same operations in the same order: bit-identical). */
+enum pgpe_split_chain {
+    PGPE_CHAIN_IN = 1,   /* the field enters in tmp, as a PGPE_CHAIN_OUT step left it */
+    PGPE_CHAIN_OUT = 2   /* leave the field in tmp, row-transformed after the next step's first
+                            reservoir half-step (coefficient row 2) */
+};
 
 int pgpe_split_step(const pgpe_split_args* args, void* stream);
 

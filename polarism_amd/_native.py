@@ -75,8 +75,11 @@ class SplitArgs(C.Structure):
         ("dt", C.c_double),
         ("table", vp), ("plan", vp), ("psi", vp), ("tmp", vp), ("res", vp * 2),
         ("pump", vp), ("coef", vp), ("step", C.c_uint64), ("step_base", vp),
-        ("res_components", C.c_int32), ("pad1", C.c_int32),
+        ("res_components", C.c_int32), ("chain", C.c_int32),
     ]
+
+
+CHAIN_IN, CHAIN_OUT = 1, 2   # pgpe_split_chain
 
 
 MAX_STAGES = 12

@@ -14,6 +14,10 @@
 //     rows  : inverse along x, exact nonlinear step, Wigner noise, forward along x
 //     cols  : as above
 //     rows  : inverse along x, reservoir half-step (P at t+3dt/4)
+//   Chained steps of the CPU order (inside a graph chunk) replace the last row pass of a step
+//   and the first of the next by one: inverse along x, this step's reservoir half-step (P at
+//   t+3dt/4), the next step's (P at t+dt+dt/4), forward along x.  The operations and their
+//   order are those of the two passes, so the chain is bit-identical to separate steps.
 //   K-outer (split_step_fft_cuda_solver.py:106-184 order):
 //     rows  : forward;  cols : propagator;
 //     rows  : inverse, R(dt/2) N(dt) R(dt/2), noise, forward;
@@ -81,8 +85,43 @@ __global__ void __launch_bounds__(NT) fft_rows_kernel(const SplitParams<T> p) {
         staged<c2<T>, CH, NT>(elems, [&](int i) { return p.in[g0 + i]; }, [&](int i, c2<T> v) { buf[slot(i)] = v; });
     }
     __syncthreads();
-    const bool first_inverse = (MODE == ROW_INV || MODE == ROW_SS_MID || MODE == ROW_SS_LAST || MODE == ROW_KO_MID);
+    const bool first_inverse = (MODE == ROW_INV || MODE == ROW_SS_MID || MODE == ROW_SS_LAST || MODE == ROW_KO_MID ||
+                                MODE == ROW_SS_JOIN);
     lds_fft<T, V, NT>(buf, p.dims, ld, count, s_tw, first_inverse);
+
+    if constexpr (MODE == ROW_SS_JOIN) {
+        // inverse done: scale, this step's last reservoir half-step (coefficient row 1), the next
+        // step's first (row 2), then forward again in place
+        struct Pt { c2<T> v; T n[2]; T P1, P2; };
+        staged<Pt, CH, NT>(
+            elems,
+            [&](int i) {
+                Pt q;
+                q.v = buf[slot(i)];
+                q.n[0] = q.n[1] = (T)0;
+                load_res<T, RES>(p, g0 + i, q.n);
+                const int r = by_n.div(i);
+                const int row = row0 + r;
+                const int m = row / p.ny;
+                const int64_t cell = (int64_t)(row - m * p.ny) * n + (i - r * n);
+                const T env = p.pump[p.table[m].pump_off + cell];
+                q.P1 = env * p.coef[p.members + m];
+                q.P2 = env * p.coef[2 * p.members + m];
+                return q;
+            },
+            [&](int i, Pt q) {
+                c2<T> v = q.v;
+                v.x *= p.scale;
+                v.y *= p.scale;
+                const int m = (row0 + by_n.div(i)) / p.ny;
+                res_half_step<T, RES>(p.table[m], v.x * v.x + v.y * v.y, q.P1, p.dt / (T)2, q.n);
+                res_half_step<T, RES>(p.table[m], v.x * v.x + v.y * v.y, q.P2, p.dt / (T)2, q.n);
+                if (res_here) store_res<T, RES>(p, g0 + i, q.n);
+                buf[slot(i)] = v;
+            });
+        __syncthreads();
+        lds_fft<T, V, NT>(buf, p.dims, ld, count, s_tw, false);
+    }
 
     if constexpr (MODE == ROW_SS_MID || MODE == ROW_KO_MID) {
         // inverse done: scale, local physics, then forward again in place
@@ -548,12 +587,14 @@ static int split_typed(const pgpe_split_args* a, hipStream_t s) {
     c2<T>* psi = (c2<T>*)a->psi;
     c2<T>* tmp = (c2<T>*)a->tmp;
     int rc;
-    // pass 1
-    p.in = psi;
-    p.out = tmp;
-    if (a->ordering == 0) rc = launch_rows<T, ROW_SS_FIRST, RES>(plan, p, s);
-    else rc = launch_rows<T, ROW_FWD, RES>(plan, p, s);
-    if (rc) return rc;
+    // pass 1 (skipped when the field enters in tmp from a chained step)
+    if (!(a->chain & PGPE_CHAIN_IN)) {
+        p.in = psi;
+        p.out = tmp;
+        if (a->ordering == 0) rc = launch_rows<T, ROW_SS_FIRST, RES>(plan, p, s);
+        else rc = launch_rows<T, ROW_FWD, RES>(plan, p, s);
+        if (rc) return rc;
+    }
     // pass 2: propagator along y
     p.in = tmp;
     p.out = tmp;
@@ -564,7 +605,14 @@ static int split_typed(const pgpe_split_args* a, hipStream_t s) {
     if (rc) return rc;
     // pass 4
     if ((rc = launch_cols<T, COL_PROP>(plan, p, s))) return rc;
-    // pass 5: inverse rows back into psi (+ second reservoir half-step for the CPU order)
+    // pass 5: inverse rows back into psi (+ second reservoir half-step for the CPU order), or, for
+    // a chained step, the join with the next step's first pass, in place in tmp
+    if (a->chain & PGPE_CHAIN_OUT) {
+        p.in = tmp;
+        p.out = tmp;
+        return launch_rows<T, ROW_SS_JOIN, RES>(plan, p, s);
+    }
+    p.in = tmp;
     p.out = psi;
     if (a->ordering == 0) return launch_rows<T, ROW_SS_LAST, RES>(plan, p, s);
     return launch_rows<T, ROW_INV, RES>(plan, p, s);
@@ -733,6 +781,10 @@ int pgpe_split_step(const pgpe_split_args* a, void* stream) {
     if (a->ordering != 0 && a->ordering != 1) return fail(PGPE_ERR_ARG, "pgpe_split_step: ordering must be 0 or 1");
     if (a->components < 0 || a->components > 2 || a->res_components < 0 || a->res_components > 2)
         return fail(PGPE_ERR_ARG, "pgpe_split_step: components and res_components must be 1 or 2");
+    if (a->chain & ~(PGPE_CHAIN_IN | PGPE_CHAIN_OUT))
+        return fail(PGPE_ERR_ARG, "pgpe_split_step: unknown chain bits");
+    if (a->chain && (a->ordering != 0 || a->components == 2))
+        return fail(PGPE_ERR_UNSUPPORTED, "pgpe_split_step: chained steps are the scalar CPU operator order only");
     const pgpe_fft_plan* plan = (const pgpe_fft_plan*)a->plan;
     if (plan->precision != a->precision) return fail(PGPE_ERR_ARG, "pgpe_split_step: plan precision mismatch");
     const int nf = a->reservoir == PGPE_RES_NONE ? 0 : (a->reservoir == PGPE_RES_SINGLE ? 1 : 2);

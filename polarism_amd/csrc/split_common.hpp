@@ -10,7 +10,15 @@
 
 namespace pgpe {
 
-enum RowMode { ROW_FWD = 0, ROW_INV = 1, ROW_SS_FIRST = 2, ROW_SS_MID = 3, ROW_SS_LAST = 4, ROW_KO_MID = 5 };
+enum RowMode {
+    ROW_FWD = 0,
+    ROW_INV = 1,
+    ROW_SS_FIRST = 2,
+    ROW_SS_MID = 3,
+    ROW_SS_LAST = 4,
+    ROW_KO_MID = 5,
+    ROW_SS_JOIN = 6  // CPU order, between chained steps: SS_LAST's work, then the next step's SS_FIRST
+};
 enum ColMode { COL_FWD = 0, COL_INV = 1, COL_PROP = 2 };
 
 template <typename T>

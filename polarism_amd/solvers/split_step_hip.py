@@ -18,6 +18,10 @@ reservoir after the second half-step.
 Each step is five fused launches (``pgpe_split_step``): rows, columns with
 the kinetic propagator between a forward and an inverse transform, rows with
 the local physics between an inverse and a forward transform, columns, rows.
+Consecutive steps of the scalar CPU order are chained: the last row pass of a
+step and the first of the next become one (the field stays in the scratch
+buffer between them), four launches per step inside a chunk, bit-identical to
+separate steps.
 """
 from __future__ import annotations
 
@@ -82,7 +86,8 @@ class _SplitStepBase(HipSolverBase):
         if const:
             if self._const is None:
                 v = self.pump_coefficients(model, [0.0], batch)[0]
-                self._const = DeviceArray.from_numpy(np.tile(v, (2, 1)).astype(self.rdtype))
+                # three rows: a chained step also reads the next step's first coefficient
+                self._const = DeviceArray.from_numpy(np.tile(v, (3, 1)).astype(self.rdtype))
             return self._const
         times = []
         for i in range(count):
@@ -94,7 +99,14 @@ class _SplitStepBase(HipSolverBase):
         self._coef_chunk.view_range(0, (vals.size,)).upload(vals, self.stream)
         return self._coef_chunk
 
-    def _launch(self, tab, coef_ptr, noise, step, step_base):
+    def _chain(self, k, chunk):
+        """Chain bits of step k of a chunk: the scalar CPU order keeps the field in the scratch
+        buffer between consecutive steps; the chunk's last step writes the state."""
+        if self.ordering != 0 or self.components != 1:
+            return 0
+        return (N.CHAIN_IN if k > 0 else 0) | (N.CHAIN_OUT if k < chunk - 1 else 0)
+
+    def _launch(self, tab, coef_ptr, noise, step, step_base, chain=0):
         a = self._args
         a.precision = self.prec_code
         a.reservoir = N.RESERVOIR_KIND[self.reservoir_kind]
@@ -114,6 +126,7 @@ class _SplitStepBase(HipSolverBase):
         a.coef = coef_ptr
         a.step = int(step)
         a.step_base = step_base
+        a.chain = chain
         N.check(N.lib().pgpe_split_step(C.byref(a), self.stream.handle), "pgpe_split_step")
 
     def _run(self, model, state, n_steps):
@@ -142,7 +155,7 @@ class _SplitStepBase(HipSolverBase):
                     def body(chunk=chunk, coef=coef):
                         for k in range(chunk):
                             ptr = coef.ptr if const else coef.ptr + k * 2 * batch * item
-                            self._launch(tab, ptr, noise, k, self._step_base.ptr)
+                            self._launch(tab, ptr, noise, k, self._step_base.ptr, self._chain(k, chunk))
                     self.stream.synchronize()
                     g = Graph.capture(self.stream, body)
                     if len(self._graphs) >= 4:
@@ -153,7 +166,7 @@ class _SplitStepBase(HipSolverBase):
             else:
                 for k in range(chunk):
                     ptr = coef.ptr if const else coef.ptr + k * 2 * batch * item
-                    self._launch(tab, ptr, noise, first + k, None)
+                    self._launch(tab, ptr, noise, first + k, None, self._chain(k, chunk))
             done += chunk
         self.stream.synchronize()
         state.step += n_steps
@@ -182,7 +195,7 @@ class SplitStepFFTHipSolver(_SplitStepBase):
     capabilities = SolverCapabilities(
         **_CAPS, description="Strang split-step FFT on gfx950 in the reference CPU order "
                              "(R K N noise K R): LDS Stockham transforms with the propagator and the "
-                             "local physics fused in, five launches per step.")
+                             "local physics fused in, five launches per step, four when chained.")
     ordering = 0
 
 
