@@ -127,30 +127,42 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmParams p) {
         }
         __syncthreads();
     }
-    // epilogue
+    // epilogue, one 32 x 32 block at a time: the block's old C and residual values are all loaded
+    // before its first store (16 loads in flight per lane instead of one round trip per value).
+    // Every output element belongs to one lane, so this stays exact when the residual is C itself.
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 2; ++j) {
+            const int gn = n0 + wn + 32 * j + (lane & 31);
+            const int gm0 = m0 + wm + 32 * i + 4 * (lane >> 5);
+            float old[16], res[16];
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int gm = m0 + wm + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const int gn = n0 + wn + 32 * j + (lane & 31);
+                const int gm = gm0 + 8 * (r >> 2) + (r & 3);
+                old[r] = 0.0f;
+                res[r] = 0.0f;
+                if (gm >= p.M || gn >= p.N) continue;
+                const int64_t off = bz * p.scb + gm * p.scm + gn * p.scn;
+                if (p.beta != 0.0f)
+                    old[r] = p.out_bf16 ? __bfloat162float(((const __hip_bfloat16*)p.C)[off]) : ((const float*)p.C)[off];
+                if (p.residual) res[r] = p.residual[bz * p.srb + gm * p.scm + gn * p.scn];
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int gm = gm0 + 8 * (r >> 2) + (r & 3);
                 if (gm >= p.M || gn >= p.N) continue;
                 const int64_t off = bz * p.scb + gm * p.scm + gn * p.scn;
                 float v = p.alpha * acc[i][j][r];
-                if (p.beta != 0.0f) {
-                    const float old = p.out_bf16 ? __bfloat162float(((const __hip_bfloat16*)p.C)[off])
-                                                 : ((const float*)p.C)[off];
-                    v += p.beta * old;
-                }
+                if (p.beta != 0.0f) v += p.beta * old[r];
                 if (p.bias_mode == 1) v += p.bias[gm];
                 else if (p.bias_mode == 2) v += p.bias[gn];
-                if (p.residual) v += p.residual[bz * p.srb + gm * p.scm + gn * p.scn];
+                if (p.residual) v += res[r];
                 if (p.gelu) v = gelu_erf(v);
                 if (p.out_bf16) ((__hip_bfloat16*)p.C)[off] = __float2bfloat16(v);
                 else ((float*)p.C)[off] = v;
             }
+        }
 }
 
 // ---------------------------------------------------------------- elementwise

@@ -56,6 +56,28 @@ def test_gemm_epilogue_bias_residual_gelu_and_beta():
     assert torch.allclose(Cb.float(), A.float() @ B.float(), atol=3e-2, rtol=1e-2)
 
 
+def test_gemm_epilogue_in_place_residual_and_bf16_beta():
+    """The epilogue reads a block's old C and residual values before its stores: a residual that is
+    C itself (an in-place residual add) and a bf16 C read with beta stay exact per element."""
+    from polarism_amd.device import init
+    from polarism_amd.surrogate.kernels import gemm
+    init()
+    torch.manual_seed(4)
+    M, N, K = 64, 1000, 64
+    A = torch.randn(M, K, device="cuda").to(torch.bfloat16)
+    B = torch.randn(K, N, device="cuda").to(torch.bfloat16)
+    C = torch.randn(M, N, device="cuda")
+    C0 = C.clone()
+    gemm(A, B, C, M, N, K, sa=(K, 1, 0), sb=(N, 1, 0), sc=(N, 1, 0), residual=C)
+    torch.cuda.synchronize()
+    assert torch.allclose(C, A.float() @ B.float() + C0, atol=1e-4, rtol=1e-4)
+    Cb = torch.randn(M, N, device="cuda").to(torch.bfloat16)
+    Cb0 = Cb.float().clone()
+    gemm(A, B, Cb, M, N, K, sa=(K, 1, 0), sb=(N, 1, 0), sc=(N, 1, 0), beta=0.5)
+    torch.cuda.synchronize()
+    assert torch.allclose(Cb.float(), A.float() @ B.float() + 0.5 * Cb0, atol=6e-2, rtol=1e-2)
+
+
 @pytest.mark.parametrize("M,N", [(64, 64), (3, 128), (64, 6)])
 def test_split_k_weight_gradient_matches_torch(M, N):
     """K = 2^17 pixels into a small fp32 output takes the split-K path (partials + fixed-order sum)."""
