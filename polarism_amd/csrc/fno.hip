@@ -22,6 +22,9 @@ namespace pgpe {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+// Staging type of the large-tile loads: a native vector, so that the prefetched tile stays in
+// registers (a copy of the uint4 struct goes through memcpy and keeps its array in scratch).
+typedef uint32_t vu32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int GT_M = 128, GT_N = 128, GT_K = 32, GT_PAD = 8;
 
@@ -41,6 +44,7 @@ struct GemmParams {
     int gelu;                // 1: apply tanh-free GELU (erf form)
     int out_bf16;
     int vec_a, vec_b;        // 16-byte vector loads are aligned for A / B
+    int batch;
 };
 
 __device__ __forceinline__ float gelu_erf(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752440f)); }
@@ -88,6 +92,49 @@ __device__ __forceinline__ void load_tile(__hip_bfloat16 (*S)[GT_K + GT_PAD], co
     }
 }
 
+// Epilogue of a wavefront's BI x BJ blocks of 32 x 32 (block (i, j) at rows m_base + 32 i, columns
+// n_base + 32 j), one block at a time: the block's old C and residual values are all loaded
+// before its first store (16 loads in flight per lane instead of one round trip per value).
+// Every output element belongs to one lane, so this stays exact when the residual is C itself.
+// Per element: alpha acc (+ beta C) (+ bias) (+ residual), then GELU, in that order.
+template <int BI, int BJ>
+__device__ __forceinline__ void gemm_epilogue(const GemmParams& p, const f32x16 (&acc)[BI][BJ], int bz, int m_base,
+                                              int n_base, int lane) {
+#pragma unroll
+    for (int i = 0; i < BI; ++i)
+#pragma unroll
+        for (int j = 0; j < BJ; ++j) {
+            const int gn = n_base + 32 * j + (lane & 31);
+            const int gm0 = m_base + 32 * i + 4 * (lane >> 5);
+            float old[16], res[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int gm = gm0 + 8 * (r >> 2) + (r & 3);
+                old[r] = 0.0f;
+                res[r] = 0.0f;
+                if (gm >= p.M || gn >= p.N) continue;
+                const int64_t off = bz * p.scb + gm * p.scm + gn * p.scn;
+                if (p.beta != 0.0f)
+                    old[r] = p.out_bf16 ? __bfloat162float(((const __hip_bfloat16*)p.C)[off]) : ((const float*)p.C)[off];
+                if (p.residual) res[r] = p.residual[bz * p.srb + gm * p.scm + gn * p.scn];
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int gm = gm0 + 8 * (r >> 2) + (r & 3);
+                if (gm >= p.M || gn >= p.N) continue;
+                const int64_t off = bz * p.scb + gm * p.scm + gn * p.scn;
+                float v = p.alpha * acc[i][j][r];
+                if (p.beta != 0.0f) v += p.beta * old[r];
+                if (p.bias_mode == 1) v += p.bias[gm];
+                else if (p.bias_mode == 2) v += p.bias[gn];
+                if (p.residual) v += res[r];
+                if (p.gelu) v = gelu_erf(v);
+                if (p.out_bf16) ((__hip_bfloat16*)p.C)[off] = __float2bfloat16(v);
+                else ((float*)p.C)[off] = v;
+            }
+        }
+}
+
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmParams p) {
     __shared__ __attribute__((aligned(16))) __hip_bfloat16 As[GT_M][GT_K + GT_PAD];
     __shared__ __attribute__((aligned(16))) __hip_bfloat16 Bs[GT_N][GT_K + GT_PAD];
@@ -127,42 +174,142 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmParams p) {
         }
         __syncthreads();
     }
-    // epilogue, one 32 x 32 block at a time: the block's old C and residual values are all loaded
-    // before its first store (16 loads in flight per lane instead of one round trip per value).
-    // Every output element belongs to one lane, so this stays exact when the residual is C itself.
+    gemm_epilogue<2, 2>(p, acc, bz, m0 + wm, n0 + wn, lane);
+}
+
+// ---------------------------------------------------------------- large tiles
+// Large products (M a multiple of 256, N of 128, K of 64; A k-contiguous; B k- or n-contiguous):
+// a 256-thread workgroup owns a 256 x 128 tile of C, each wavefront 128 x 64 of it as 4 x 2
+// MFMA blocks of 32 x 32 (8 accumulators of 16 per lane).  K advances 64 at a time through two
+// LDS buffers: the next K-tile's operands are loaded into registers before the current tile's
+// 32 MFMAs per wavefront and written to the other buffer after them, one barrier per K-tile.
+// LDS rows hold 64 k plus 8 of padding (As[m][k], Bs[n][k]: the MFMA operand of a lane is one
+// 16-byte read).  A row-major (n-contiguous) B is transposed on the way in: each thread loads a
+// 4 (k) x 8 (n) block as four 16-byte rows and recombines them with byte permutes into eight
+// 4-element k columns, written as 8-byte LDS stores.  Workgroups take tiles in an XCD-aware
+// order (consecutive tiles on one XCD) grouped by four row panels, so the workgroups an XCD runs
+// together share their A and B panels in its L2.
+constexpr int BG_M = 256, BG_N = 128, BG_K = 64, BG_LD = BG_K + 8;
+constexpr size_t kBigLdsBytes = (size_t)2 * (BG_M + BG_N) * BG_LD * sizeof(__hip_bfloat16);
+
+// 16-bit halves of two dwords: (lo(a), lo(b)) and (hi(a), hi(b)).
+__device__ __forceinline__ uint32_t pack_lo(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x05040100u); }
+__device__ __forceinline__ uint32_t pack_hi(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x07060302u); }
+
+template <bool B_KCONTIG>
+__global__ void __launch_bounds__(256) gemm_bf16_big_kernel(const GemmParams p) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char big_smem[];
+    typedef __hip_bfloat16 Row[BG_LD];
+    Row* As = reinterpret_cast<Row*>(big_smem);   // [2][BG_M]
+    Row* Bs = As + 2 * BG_M;                      // [2][BG_N]
+    const int tiles_m = p.M / BG_M, tiles_n = p.N / BG_N;
+    const int per_batch = tiles_m * tiles_n;
+    const int t = xcd_tile(blockIdx.x, per_batch * p.batch);
+    if (t < 0) return;
+    const int bz = t / per_batch;
+    const int tt = t - bz * per_batch;
+    // grouped raster: four row panels per group, columns inside
+    const int group = tt / (4 * tiles_n);
+    const int first_m = group * 4;
+    const int group_m = min(4, tiles_m - first_m);
+    const int in_group = tt - group * 4 * tiles_n;
+    const int m0 = (first_m + in_group % group_m) * BG_M;
+    const int n0 = (in_group / group_m) * BG_N;
+    const __hip_bfloat16* __restrict__ A = p.A + bz * p.sab;
+    const __hip_bfloat16* __restrict__ B = p.B + bz * p.sbb;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = (wave >> 1) * 128, wn = (wave & 1) * 64;
+
+    vu32x4 ra[8], rb[4];
+    auto load = [&](int k0) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 8; ++i) {
+            const int v = tid + 256 * i;  // 2048 vectors: 256 rows x 8 vectors of 8 k
+            ra[i] = *reinterpret_cast<const vu32x4*>(A + (int64_t)(m0 + (v >> 3)) * p.sam + k0 + (v & 7) * 8);
+        }
+        if constexpr (B_KCONTIG) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int gn = n0 + wn + 32 * j + (lane & 31);
-            const int gm0 = m0 + wm + 32 * i + 4 * (lane >> 5);
-            float old[16], res[16];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int gm = gm0 + 8 * (r >> 2) + (r & 3);
-                old[r] = 0.0f;
-                res[r] = 0.0f;
-                if (gm >= p.M || gn >= p.N) continue;
-                const int64_t off = bz * p.scb + gm * p.scm + gn * p.scn;
-                if (p.beta != 0.0f)
-                    old[r] = p.out_bf16 ? __bfloat162float(((const __hip_bfloat16*)p.C)[off]) : ((const float*)p.C)[off];
-                if (p.residual) res[r] = p.residual[bz * p.srb + gm * p.scm + gn * p.scn];
+            for (int i = 0; i < 4; ++i) {
+                const int v = tid + 256 * i;  // 1024 vectors: 128 rows (n) x 8 vectors of 8 k
+                rb[i] = *reinterpret_cast<const vu32x4*>(B + (int64_t)(n0 + (v >> 3)) * p.sbn + k0 + (v & 7) * 8);
             }
+        } else {
+            // block of 4 k x 8 n: 16 blocks along n, 16 along k
+            const int nb = (tid & 15) * 8, kb = (tid >> 4) * 4;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int gm = gm0 + 8 * (r >> 2) + (r & 3);
-                if (gm >= p.M || gn >= p.N) continue;
-                const int64_t off = bz * p.scb + gm * p.scm + gn * p.scn;
-                float v = p.alpha * acc[i][j][r];
-                if (p.beta != 0.0f) v += p.beta * old[r];
-                if (p.bias_mode == 1) v += p.bias[gm];
-                else if (p.bias_mode == 2) v += p.bias[gn];
-                if (p.residual) v += res[r];
-                if (p.gelu) v = gelu_erf(v);
-                if (p.out_bf16) ((__hip_bfloat16*)p.C)[off] = __float2bfloat16(v);
-                else ((float*)p.C)[off] = v;
+            for (int i = 0; i < 4; ++i)
+                rb[i] = *reinterpret_cast<const vu32x4*>(B + (int64_t)(k0 + kb + i) * p.sbk + n0 + nb);
+        }
+    };
+    auto store = [&](int buf) {
+        Row* as = As + buf * BG_M;
+        Row* bs = Bs + buf * BG_N;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int v = tid + 256 * i;
+            *reinterpret_cast<vu32x4*>(&as[v >> 3][(v & 7) * 8]) = ra[i];
+        }
+        if constexpr (B_KCONTIG) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int v = tid + 256 * i;
+                *reinterpret_cast<vu32x4*>(&bs[v >> 3][(v & 7) * 8]) = rb[i];
+            }
+        } else {
+            const int nb = (tid & 15) * 8, kb = (tid >> 4) * 4;
+            const uint32_t w[4][4] = {{rb[0].x, rb[0].y, rb[0].z, rb[0].w},
+                                      {rb[1].x, rb[1].y, rb[1].z, rb[1].w},
+                                      {rb[2].x, rb[2].y, rb[2].z, rb[2].w},
+                                      {rb[3].x, rb[3].y, rb[3].z, rb[3].w}};
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {  // dword d of a row holds columns nb + 2d, nb + 2d + 1
+                uint2 even, odd;          // k = kb .. kb + 3 of column nb + 2d (even), nb + 2d + 1 (odd)
+                even.x = pack_lo(w[0][d], w[1][d]);
+                even.y = pack_lo(w[2][d], w[3][d]);
+                odd.x = pack_hi(w[0][d], w[1][d]);
+                odd.y = pack_hi(w[2][d], w[3][d]);
+                *reinterpret_cast<uint2*>(&bs[nb + 2 * d][kb]) = even;
+                *reinterpret_cast<uint2*>(&bs[nb + 2 * d + 1][kb]) = odd;
             }
         }
+    };
+
+    f32x16 acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    const int ktiles = p.K / BG_K;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int kt = 0; kt < ktiles; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < ktiles) load((kt + 1) * BG_K);
+        const Row* as = As + buf * BG_M;
+        const Row* bs = Bs + buf * BG_N;
+#pragma unroll
+        for (int ks = 0; ks < BG_K; ks += 16) {
+            const int kk = ks + 8 * (lane >> 5);
+            bf16x8 a[4], b[2];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&as[wm + 32 * i + (lane & 31)][kk]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&bs[wn + 32 * j + (lane & 31)][kk]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < ktiles) store(buf ^ 1);
+        __syncthreads();
+    }
+    gemm_epilogue<4, 2>(p, acc, bz, m0 + wm, n0 + wn, lane);
 }
 
 // ---------------------------------------------------------------- elementwise
@@ -294,6 +441,7 @@ int pgpe_gemm_bf16(const pgpe_gemm_args* a, void* stream) {
     p.bias_mode = a->bias_mode;
     p.gelu = a->gelu;
     p.out_bf16 = a->out_bf16;
+    p.batch = a->batch;
     // 16-byte vectors along the unit-stride axis need a 16-byte aligned base and strides
     // that are multiples of 8 elements on the other axes.
     auto aligned = [](const void* ptr, int64_t s1, int64_t s2) {
@@ -301,6 +449,19 @@ int pgpe_gemm_bf16(const pgpe_gemm_args* a, void* stream) {
     };
     p.vec_a = a->sak == 1 ? aligned(a->A, a->sam, a->sab) : aligned(a->A, a->sak, a->sab);
     p.vec_b = a->sbk == 1 ? aligned(a->B, a->sbn, a->sbb) : aligned(a->B, a->sbk, a->sbb);
+    // large tiles: A k-contiguous, B k- or n-contiguous, all on 16-byte vectors, whole tiles
+    const bool big = a->sak == 1 && p.vec_a && p.vec_b && a->M % BG_M == 0 && a->N % BG_N == 0 && a->K % BG_K == 0 &&
+                     (int64_t)(a->M / BG_M) * (a->N / BG_N) * a->batch <= 0x7fffffff;
+    if (big) {
+        auto kern = a->sbk == 1 ? gemm_bf16_big_kernel<true> : gemm_bf16_big_kernel<false>;
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)kBigLdsBytes);
+        if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(gemm_bf16_big_kernel)");
+        const int tiles = (a->M / BG_M) * (a->N / BG_N);
+        hipLaunchKernelGGL(kern, dim3(xcd_grid(tiles * a->batch)), dim3(256), kBigLdsBytes, (hipStream_t)stream, p);
+        PGPE_LAUNCHED("gemm_bf16_big_kernel");
+        return PGPE_OK;
+    }
     const dim3 grid((a->N + GT_N - 1) / GT_N, (a->M + GT_M - 1) / GT_M, a->batch);
     hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream, p);
     PGPE_LAUNCHED("gemm_bf16_kernel");

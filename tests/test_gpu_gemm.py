@@ -32,6 +32,52 @@ def test_gemm_layouts_match_torch(M, N, K, batch, ta, tb):
     assert err < 1e-4, err
 
 
+@pytest.mark.parametrize("M,N,K,batch", [(256, 128, 64, 1), (512, 384, 192, 2), (768, 256, 128, 1)])
+@pytest.mark.parametrize("tb", [False, True])
+def test_large_tile_gemm_matches_torch(M, N, K, batch, tb):
+    """Whole 256 x 128 tiles with K a multiple of 64 and a k-contiguous A take the large-tile
+    kernel (double-buffered LDS; a row-major B transposed in registers, tb = False)."""
+    from polarism_amd.device import init
+    from polarism_amd.surrogate.kernels import gemm
+    init()
+    torch.manual_seed(M + N + K)
+    A = torch.randn(batch, M, K, device="cuda").to(torch.bfloat16)
+    B = torch.randn(batch, K, N, device="cuda").to(torch.bfloat16)
+    Bs = B.transpose(1, 2).contiguous() if tb else B.contiguous()
+    sb = (1, K, K * N) if tb else (N, 1, K * N)
+    C = torch.zeros(batch, M, N, device="cuda")
+    gemm(A.contiguous(), Bs, C, M, N, K, batch=batch, sa=(K, 1, M * K), sb=sb, sc=(N, 1, M * N))
+    torch.cuda.synchronize()
+    ref = torch.bmm(A.float(), B.float())
+    err = (C - ref).abs().max().item() / ref.abs().max().item()
+    assert err < 1e-4, err
+
+
+@pytest.mark.parametrize("bias_mode", [1, 2])
+def test_large_tile_gemm_epilogue(bias_mode):
+    from polarism_amd.device import init
+    from polarism_amd.surrogate.kernels import gemm
+    init()
+    torch.manual_seed(9 + bias_mode)
+    M, N, K = 256, 256, 128
+    A = torch.randn(M, K, device="cuda").to(torch.bfloat16)
+    B = torch.randn(K, N, device="cuda").to(torch.bfloat16)
+    bias = torch.randn(M if bias_mode == 1 else N, device="cuda")
+    res = torch.randn(M, N, device="cuda")
+    C = torch.randn(M, N, device="cuda")
+    C0 = C.clone()
+    gemm(A, B, C, M, N, K, sa=(K, 1, 0), sb=(N, 1, 0), sc=(N, 1, 0), alpha=0.5, beta=2.0, bias=bias,
+         bias_mode=bias_mode, residual=res, gelu=True)
+    torch.cuda.synchronize()
+    b = bias[:, None] if bias_mode == 1 else bias[None, :]
+    ref = torch.nn.functional.gelu(0.5 * (A.float() @ B.float()) + 2.0 * C0 + b + res)
+    assert torch.allclose(C, ref, atol=1e-4, rtol=1e-4)
+    Cb = torch.empty(M, N, device="cuda", dtype=torch.bfloat16)
+    gemm(A, B, Cb, M, N, K, sa=(K, 1, 0), sb=(N, 1, 0), sc=(N, 1, 0))
+    torch.cuda.synchronize()
+    assert torch.allclose(Cb.float(), A.float() @ B.float(), atol=3e-2, rtol=1e-2)
+
+
 def test_gemm_epilogue_bias_residual_gelu_and_beta():
     from polarism_amd.device import init
     from polarism_amd.surrogate.kernels import gemm
