@@ -65,6 +65,16 @@ __host__ __device__ __forceinline__ c2<T> make_c(T re, T im) {
     return v;
 }
 
+// Load of one complex value as a native two-element vector: a plain copy of a double2 goes
+// through a 16-byte memcpy, which keeps a struct that receives it (and arrays of such structs)
+// in scratch memory.
+template <typename T>
+__device__ __forceinline__ c2<T> ld_c(const c2<T>* p) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 t = *reinterpret_cast<const v2*>(p);
+    return make_c<T>(t.x, t.y);
+}
+
 __device__ __forceinline__ void sin_cos(float x, float* s, float* c) { sincosf(x, s, c); }
 __device__ __forceinline__ void sin_cos(double x, double* s, double* c) { sincos(x, s, c); }
 

@@ -123,8 +123,8 @@ __global__ void __launch_bounds__(NT) fft_rows_spinor_kernel(const SplitParams<T
         [&](int e) {
             const At a = locate(e);
             Pt q;
-            q.v[0] = p.in[psi_at(a, 0)];
-            q.v[1] = p.in[psi_at(a, 1)];
+            q.v[0] = ld_c<T>(&p.in[psi_at(a, 0)]);
+            q.v[1] = ld_c<T>(&p.in[psi_at(a, 1)]);
             if constexpr (MODE == ROW_SS_FIRST) {
                 load_n(a, q.nn);
                 q.P1 = p.pump[p.table[a.m].pump_off + a.cell] * p.coef[a.m];
@@ -152,8 +152,8 @@ __global__ void __launch_bounds__(NT) fft_rows_spinor_kernel(const SplitParams<T
             [&](int e) {
                 const At a = locate(e);
                 Pt q;
-                q.v[0] = buf[a.slot];
-                q.v[1] = buf[a.slot + ld];
+                q.v[0] = ld_c<T>(&buf[a.slot]);
+                q.v[1] = ld_c<T>(&buf[a.slot + ld]);
                 load_n(a, q.nn);
                 if constexpr (MODE == ROW_KO_MID) {
                     const T env = p.pump[p.table[a.m].pump_off + a.cell];
@@ -206,8 +206,8 @@ __global__ void __launch_bounds__(NT) fft_rows_spinor_kernel(const SplitParams<T
         [&](int e) {
             const At a = locate(e);
             Pt q;
-            q.v[0] = buf[a.slot];
-            q.v[1] = buf[a.slot + ld];
+            q.v[0] = ld_c<T>(&buf[a.slot]);
+            q.v[1] = ld_c<T>(&buf[a.slot + ld]);
             if constexpr (MODE == ROW_SS_LAST) {
                 load_n(a, q.nn);
                 q.P1 = p.pump[p.table[a.m].pump_off + a.cell] * p.coef[p.members + a.m];
@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(NT) fft_cols_spinor_kernel(const SplitParams<T
             [&](int i) {
                 const int y = by_count.div(i);
                 const int col = i - y * count;
-                return Pt{p.in[base + (int64_t)y * p.nx + x0 + col], (c * count + col) * ld + pad_at(y)};
+                return Pt{ld_c<T>(&p.in[base + (int64_t)y * p.nx + x0 + col]), (c * count + col) * ld + pad_at(y)};
             },
             [&](int, Pt q) { buf[q.at] = q.v; });
     }

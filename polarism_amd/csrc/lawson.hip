@@ -102,16 +102,25 @@ __device__ __forceinline__ bool trial_skipped(const LawsonParams<T>& p) { return
 template <typename T>
 __device__ __forceinline__ void apply_linear(const pgpe_member& mb, int comps, double kxv, double kyv, double tau,
                                              const c2<T>* v, c2<T>* out) {
+    // Both branches give values and the stores follow the merge: a store to out[0] in one
+    // branch and to out[1] in the other would be merged into one store through a selected
+    // pointer, which keeps the caller's output array in scratch memory.
+    c2<T> r0, r1 = make_c<T>((T)0, (T)0);
     if (comps == 1) {
         const T rate = (T)(-mb.hbar / (2.0 * mb.m_eff));
         const T k2 = (T)(kxv * kxv + kyv * kyv);
         const T angle = (rate * (T)tau) * k2;
         T s, c;
         sin_cos(angle, &s, &c);
-        out[0] = cmul<T>(make_c<T>(c, s), v[0]);
-        return;
+        r0 = cmul<T>(make_c<T>(c, s), v[0]);
+    } else {
+        c2<T> o[2];
+        spinor_propagate<T>(mb, kxv, kyv, tau, v, o);
+        r0 = o[0];
+        r1 = o[1];
     }
-    spinor_propagate<T>(mb, kxv, kyv, tau, v, out);
+    out[0] = r0;
+    out[1] = r1;
 }
 
 template <typename T, int RES>

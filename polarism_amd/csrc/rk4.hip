@@ -219,15 +219,15 @@ __device__ __forceinline__ c2<T> csum(const c2<T>& a, const c2<T>& b) {
 //   k = (-i/hbar)[-hbar^2/2m lap psi + (V + gC|psi|^2 + gR nA + gI nI) psi]
 //       + (R nA - gammaC)/2 psi + eta nA hbar/2m lap psi,
 // from the stage input ps, the kinetic sums K (kin5 / kin9) with their centre weight
-// (c0 / c20), the stage reservoir n and an optional complex potential V.  With
+// (c0 / c20), the stage reservoir n and a complex potential V (used when has_v).  With
 // E = (gC rho + gR nA + gI nI + Re V) / hbar - centre and G = R nA / 2 - gammaC / 2 + Im V / hbar,
 //   k = (G psi_r + (E psi_i + K_i), G psi_i - (E psi_r + K_r)).
 // The per-stage and the whole-step kernels all call this, so they perform the same operations in
 // the same order.
 template <typename T, int RES, bool ETA = true>
 __device__ __forceinline__ void scalar_rhs(const pgpe_member& mb, const ScalarConsts<T>& k, const c2<T>& ps,
-                                           const c2<T>& K, T centre, const T n[2], T P, const c2<T>* V, c2<T>& d,
-                                           T dn[2]) {
+                                           const c2<T>& K, T centre, const T n[2], T P, bool has_v, c2<T> V,
+                                           c2<T>& d, T dn[2]) {
 #pragma clang fp contract(on)
     constexpr int NF = ResFields<RES>::n;
     const T rho = ps.x * ps.x + ps.y * ps.y;
@@ -240,9 +240,9 @@ __device__ __forceinline__ void scalar_rhs(const pgpe_member& mb, const ScalarCo
     T E = NF > 1 ? k.gCh * rho + (k.gRh * nA + (k.gIh * nI - centre))
                  : (NF > 0 ? k.gCh * rho + (k.gRh * nA - centre) : k.gCh * rho - centre);
     T G = NF > 0 ? k.hR * nA - k.hG : -k.hG;
-    if (V != nullptr) {
-        E = E + V->x * k.inv_hbar;
-        G = G + V->y * k.inv_hbar;
+    if (has_v) {
+        E = E + V.x * k.inv_hbar;
+        G = G + V.y * k.inv_hbar;
     }
     d.x = G * ps.x + (E * ps.y + K.y);
     d.y = G * ps.y - (E * ps.x + K.x);
@@ -364,11 +364,11 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
                                        csum<T>(csum<T>(csum<T>(s_psi[c - W - 1], s_psi[c - W + 1]), s_psi[c + W - 1]),
                                                s_psi[c + W + 1]))
                              : kin5<T>(kc, sx, sy);
-        c2<T> vpot;
-        if (p.potential) vpot = p.potential[mb.pot_off + idx];
+        const bool has_v = p.potential != nullptr;
+        const c2<T> vpot = has_v ? p.potential[mb.pot_off + idx] : make_c<T>((T)0, (T)0);
         c2<T> d;
         T dn[2];
-        scalar_rhs<T, RES>(mb, kc, ps, K, centre, n, P, p.potential ? &vpot : nullptr, d, dn);
+        scalar_rhs<T, RES>(mb, kc, ps, K, centre, n, P, has_v, vpot, d, dn);
 
         if constexpr (KIND != 2) {
             st_stream_c<T>(&p.kout[off + idx], d);
@@ -496,10 +496,14 @@ __global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T
             te[c].x = -beta * ((dxx.x - dyy.x) + cross.x);
             te[c].y = -beta * ((dxx.y - dyy.y) + cross.y);
         }
-        // reservoir stage values and derivatives
+        // reservoir stage values and derivatives; the component loops run over both slots with
+        // constant indices (guarded by res_comps) so that these arrays stay in registers
+        const bool two = res_comps == 2;
         T n[2][2] = {{0, 0}, {0, 0}}, dn[2][2] = {{0, 0}, {0, 0}};
         const T P = env[cell] * coef;
-        for (int rc = 0; rc < res_comps; ++rc) {
+#pragma unroll
+        for (int rc = 0; rc < 2; ++rc) {
+            if (rc >= res_comps) continue;
             const int64_t g = roff + rc * plane + cell;
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
@@ -512,10 +516,10 @@ __global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T
             reservoir_rhs<T, RES>(mb, rr, Pc, n[rc], dn[rc]);
         }
         c2<T> d[2];
+#pragma unroll
         for (int c = 0; c < 2; ++c) {
-            const int rc = res_comps == 2 ? c : 0;
-            const T nA = NF > 0 ? n[rc][0] : (T)0;
-            const T nI = NF > 1 ? n[rc][1] : (T)0;
+            const T nA = NF > 0 ? (two ? n[c][0] : n[0][0]) : (T)0;
+            const T nI = NF > 1 ? (two ? n[c][1] : n[0][1]) : (T)0;
             const T sgn = c == 0 ? (T)1 : (T)-1;
             const T eff = ((((gC * rho[c]) + a2 * rho[1 - c]) + zh * sgn / (T)2) + gR * nA) + gI * nI;
             const T gain = (R * nA - gammaC) * (T)0.5;
@@ -530,13 +534,19 @@ __global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T
             }
         }
         if constexpr (KIND != 2) {
+#pragma unroll
             for (int c = 0; c < 2; ++c) p.kout[poff + c * plane + cell] = d[c];
-            for (int rc = 0; rc < res_comps; ++rc)
+#pragma unroll
+            for (int rc = 0; rc < 2; ++rc) {
+                if (rc >= res_comps) continue;
 #pragma unroll
                 for (int f = 0; f < NF; ++f) p.kres_out[f][roff + rc * plane + cell] = dn[rc][f];
+            }
         } else {
             T n_new[2][2] = {{0, 0}, {0, 0}};
-            for (int rc = 0; rc < res_comps; ++rc) {
+#pragma unroll
+            for (int rc = 0; rc < 2; ++rc) {
+                if (rc >= res_comps) continue;
                 const int64_t g = roff + rc * plane + cell;
 #pragma unroll
                 for (int f = 0; f < NF; ++f) {
@@ -547,6 +557,7 @@ __global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T
                     p.res_out[f][g] = v;
                 }
             }
+#pragma unroll
             for (int c = 0; c < 2; ++c) {
                 const int64_t g = poff + c * plane + cell;
                 const c2<T> a1 = p.k1[g], a2v = p.k2[g], a3 = p.kprev[g], z = p.psi0[g];
@@ -554,7 +565,7 @@ __global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T
                 out.x = z.x + p.dt6 * (((a1.x + (T)2 * a2v.x) + (T)2 * a3.x) + d[c].x);
                 out.y = z.y + p.dt6 * (((a1.y + (T)2 * a2v.y) + (T)2 * a3.y) + d[c].y);
                 if (p.noise) {
-                    const T act = NF > 0 ? n_new[res_comps == 2 ? c : 0][0] : (T)0;
+                    const T act = NF > 0 ? (two ? n_new[c][0] : n_new[0][0]) : (T)0;
                     wigner_site<T>(out.x, out.y, mb.seed, (uint32_t)cell, step, 1u + (uint32_t)c,
                                    (T)mb.noise_strength, (T)mb.noise_gamma, (T)mb.noise_R, act,
                                    (T)mb.noise_dt_over_4v);
@@ -681,7 +692,7 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
             const T P = env[j] * coef[s];
             c2<T> d;
             T dn[2];
-            scalar_rhs<T, RES>(mb, kc, ps, K, centre, ns[j], P, nullptr, d, dn);
+            scalar_rhs<T, RES>(mb, kc, ps, K, centre, ns[j], P, false, make_c<T>((T)0, (T)0), d, dn);
             if (s == 0) {
                 acc[j] = d;
 #pragma unroll
@@ -916,7 +927,7 @@ __device__ __forceinline__ void strip_stages(const Rk4StepParams<T>& p, c2<T> (&
             const T P = env[j] * coef[s];
             c2<T> d;
             T dn[2];
-            scalar_rhs<T, RES, ETA>(mb, kc, ps, K, centre, ns[j], P, nullptr, d, dn);
+            scalar_rhs<T, RES, ETA>(mb, kc, ps, K, centre, ns[j], P, false, make_c<T>((T)0, (T)0), d, dn);
             if (s == 0) {
                 acc[j] = d;
 #pragma unroll
