@@ -460,17 +460,34 @@ def _laplacian9(f, h, boundary):
     return (acc - 20.0 * f) * inv_6h2
 
 
-def te_tm_energy(p, field, dx, dy):
-    """Periodic TE-TM term -beta[(dxx - dyy) -/+ 2i dxy] psi_-/+ (model.py:175-208)."""
+def te_tm_energy(p, field, dx, dy, boundary="periodic"):
+    """TE-TM term -beta[(dxx - dyy) -/+ 2i dxy] psi_-/+ (model.py:175-208).
+
+    The second differences act on the other component: periodic wrap (model.py:187-194), or on
+    the closed interval the field extended by one mirrored point per side, f[-1] = f[1],
+    f[n] = f[n - 2], corners mirrored in both axes (model.py:196-205, numpy's "reflect" pad).
+    """
     if p.te_tm == 0.0:
         return 0
     other = field[:, ::-1]
-    dxx = (np.roll(other, -1, axis=-1) - 2 * other + np.roll(other, 1, axis=-1)) / dx ** 2
-    dyy = (np.roll(other, -1, axis=-2) - 2 * other + np.roll(other, 1, axis=-2)) / dy ** 2
-    dxy = (np.roll(np.roll(other, -1, axis=-2), -1, axis=-1)
-           - np.roll(np.roll(other, -1, axis=-2), 1, axis=-1)
-           - np.roll(np.roll(other, 1, axis=-2), -1, axis=-1)
-           + np.roll(np.roll(other, 1, axis=-2), 1, axis=-1)) / (4 * dx * dy)
+    if boundary == "periodic":
+        dxx = (np.roll(other, -1, axis=-1) - 2 * other + np.roll(other, 1, axis=-1)) / dx ** 2
+        dyy = (np.roll(other, -1, axis=-2) - 2 * other + np.roll(other, 1, axis=-2)) / dy ** 2
+        dxy = (np.roll(np.roll(other, -1, axis=-2), -1, axis=-1)
+               - np.roll(np.roll(other, -1, axis=-2), 1, axis=-1)
+               - np.roll(np.roll(other, 1, axis=-2), -1, axis=-1)
+               + np.roll(np.roll(other, 1, axis=-2), 1, axis=-1)) / (4 * dx * dy)
+    elif boundary == "closed-interval":
+        ny, nx = other.shape[-2:]
+        up, dn = _neighbour(ny, boundary)   # rows y - 1, y + 1 (mirrored at the walls)
+        lt, rt = _neighbour(nx, boundary)
+        rows = lambda idx: np.take(other, idx, axis=-2)
+        cols = lambda a, idx: np.take(a, idx, axis=-1)
+        dxx = (cols(other, rt) - 2 * other + cols(other, lt)) / dx ** 2
+        dyy = (rows(dn) - 2 * other + rows(up)) / dy ** 2
+        dxy = (cols(rows(dn), rt) - cols(rows(dn), lt) - cols(rows(up), rt) + cols(rows(up), lt)) / (4 * dx * dy)
+    else:
+        raise ValueError(boundary)
     cross = np.stack((-2j * dxy[:, 0], 2j * dxy[:, 1]), axis=1)
     return -p.te_tm * (dxx - dyy + cross)
 
@@ -502,7 +519,7 @@ def fdm_rhs(kind, p, grid, psi, res, pump, potential=0.0, pump_weights=None,
     if spinor:
         sign = np.asarray([1, -1], dtype=density.dtype).reshape(1, 2, 1, 1)
         eff = eff + p.alpha_2 * density[:, ::-1] + p.zeeman * sign / 2
-        extra = te_tm_energy(p, psi, grid.dx, grid.dy)
+        extra = te_tm_energy(p, psi, grid.dx, grid.dy, getattr(grid, "boundary", "periodic"))
     gain = (p.R * nA - p.gamma_C) / 2.0
     hb2m = p.hbar / (2.0 * p.m_eff)
     dpsi = ((-1j / p.hbar) * (kinetic + eff * psi + extra) + gain * psi

@@ -407,7 +407,10 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
 //   X_c = -hbar^2/2m lap psi_c + (gC rho_c + a2 rho_o +- Z/2 + gR nA + gI nI) psi_c
 //         - beta [(dxx - dyy) o -/+ 2i dxy o]
 //   k_c = (-i/hbar) X_c + (R nA - gammaC)/2 psi_c + eta nA hbar/2m lap psi_c
-// dxy is the 4-corner central difference, so the LDS tile keeps its corners.
+// dxy is the 4-corner central difference, so the LDS tile keeps its corners.  Periodic grids
+// wrap the halo; closed intervals mirror it about the wall nodes, which gives the Laplacian the
+// edge rows 2 (psi[1] - psi[0]) / h^2 of model.py:227-248 and the TE-TM differences their
+// reflect-padded values.
 template <typename T, int RES, int TH, int KIND>
 __global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T> p, int res_comps) {
     constexpr int NF = ResFields<RES>::n;
@@ -433,12 +436,10 @@ __global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T
         const int j = i - c * PL;
         const int ly = j / W;
         const int lx = j - ly * W;
-        int gy = y0 + ly - 1;
-        int gx = x0 + lx - 1;
-        gy = gy < 0 ? gy + ny : gy;
-        gx = gx < 0 ? gx + nx : gx;
-        if (gy >= ny) gy %= ny;
-        if (gx >= nx) gx %= nx;
+        // periodic wrap, or the closed interval's mirror about the wall nodes (the one-point
+        // reflect extension of rk4_cuda_solver.py:1563-1568 and model.py:196-205)
+        const int gy = map_index(y0 + ly - 1, ny, mb.boundary);
+        const int gx = map_index(x0 + lx - 1, nx, mb.boundary);
         const int64_t g = poff + c * plane + (int64_t)gy * nx + gx;
         c2<T> v = p.psi0[g];
         if constexpr (KIND != 0) {

@@ -87,8 +87,8 @@ class MemberTable:
             row.pot_off = s.pot_off
             if s.laplacian == "isotropic-9pt" and abs(s.dx - s.dy) > 1e-8 * max(s.dx, s.dy):
                 raise ValueError(f"laplacian 'isotropic-9pt' requires dx == dy, got dx={s.dx}, dy={s.dy}")
-            if s.components == 2 and (s.laplacian != "five-point" or s.boundary != "periodic"):
-                raise ValueError("spinor members run the periodic five-point stencil")
+            if s.components == 2 and s.laplacian != "five-point":
+                raise ValueError("spinor members run the five-point stencil")
             row.noise_gamma = get("gamma_C")
             row.noise_R = get("R")
             row.noise_strength = float(s.noise_strength)

@@ -55,9 +55,11 @@ class RK4FDMHipSolver(HipSolverBase):
                 raise ValueError(f"solver.laplacian='isotropic-9pt' requires dx == dy, got dx={grid.dx}, "
                                  f"dy={grid.dy} (relative difference {rel:.2e}).")
         self.boundary = getattr(grid, "boundary", "periodic")
-        if self.components == 2 and (self.laplacian != "five-point" or self.boundary != "periodic"):
-            raise SolverCompatibilityError("rk4-fdm-hip runs spinor fields on periodic grids with the "
-                                           "five-point stencil")
+        # the spinor stencil (rk4_cuda_solver.py:1600-1615) is the five-point Laplacian on periodic
+        # or closed-interval grids; the isotropic 9-point stencil exists for scalar fields only
+        if self.components == 2 and self.laplacian != "five-point":
+            raise SolverCompatibilityError("rk4-fdm-hip runs spinor fields with the five-point stencil "
+                                           "(solver.laplacian='isotropic-9pt' is for scalar fields)")
 
     def member_specs(self, model, batch):
         specs = super().member_specs(model, batch)

@@ -112,7 +112,7 @@ def test_tile_height_does_not_change_results(tile_h):
 
 
 def _oracle_spinor_rk4(cfg, n_steps, res_components=2):
-    g = O.Grid(cfg.grid.nx, cfg.grid.ny, cfg.grid.lx, cfg.grid.ly)
+    g = O.Grid(cfg.grid.nx, cfg.grid.ny, cfg.grid.lx, cfg.grid.ly, boundary=cfg.grid.grid_type)
     prec = cfg.solver.precision
     real = np.float32 if prec == "single" else np.float64
     kind = cfg.reservoir.reservoir_type
@@ -123,7 +123,7 @@ def _oracle_spinor_rk4(cfg, n_steps, res_components=2):
     psi = np.stack((a, b))[None]
     nf = {"none": 0, "single": 1, "double": 2, "quadratic-double": 2}[kind]
     res = tuple(np.zeros((1, res_components, *g.shape), dtype=real) for _ in range(nf))
-    env = O.gaussian_profile(g, cfg.laser.sigma_space).astype(real)
+    env = O.gaussian_profile(g, cfg.laser.sigma_space, cfg.laser.x0, cfg.laser.y0).astype(real)
     cp = cfg.laser.circular_polarization
     weights = np.asarray([(1 + cp) / 2, (1 - cp) / 2], dtype=real).reshape(1, 2, 1, 1)
     dt = cfg.solver.dt
@@ -146,6 +146,29 @@ def test_spinor_rk4_matches_oracle(kind, res_components, precision, tol):
     ctl = _run(cfg)
     psi_ref, res_ref = _oracle_spinor_rk4(cfg, 100, res_components)
     assert O.rel_l2(ctl.state.field.get(), psi_ref) <= tol
+    for d, r in zip(ctl.reservoir.get_state(), res_ref):
+        assert O.rel_l2(d.get(), r) <= tol
+
+
+@pytest.mark.parametrize("res_components", [2, 1])
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("single", 1e-5)])
+def test_spinor_closed_interval_matches_oracle(res_components, precision, tol):
+    """Spinor on the closed interval: the halo mirrored about the wall nodes feeds the Laplacian
+    (model.py:227-248) and the reflect-padded TE-TM differences (model.py:196-205); the pump sits
+    off-centre and TE-TM is strong so the walls matter."""
+    cfg = quickstart_config(grid__nx=80, grid__ny=56, grid__lx=40.0, grid__ly=28.0,
+                            grid__grid_type="closed-interval", solver__dt=2e-3, solver__total_time=0.2,
+                            physics__spin="spinor", physics__te_tm=0.2, physics__zeeman=0.02,
+                            physics__alpha_2=-1e-4, laser__circular_polarization=0.3, laser__P0=0.3,
+                            laser__sigma_space=6.0, laser__x0=14.0, laser__y0=-9.0,
+                            reservoir__reservoir_type="single", solver__precision=precision,
+                            noise__initial_amplitude=0.05)
+    cfg.reservoir.components = res_components
+    ctl = _run(cfg)
+    psi_ref, res_ref = _oracle_spinor_rk4(cfg, 100, res_components)
+    e = O.rel_l2(ctl.state.field.get(), psi_ref)
+    print(f"spinor closed-interval {precision} res_components={res_components}: rel-L2 psi={e:.3e}")
+    assert e <= tol
     for d, r in zip(ctl.reservoir.get_state(), res_ref):
         assert O.rel_l2(d.get(), r) <= tol
 

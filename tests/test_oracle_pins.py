@@ -59,6 +59,40 @@ def test_te_tm_stencil_reduces_to_dispersion_coupling():
     assert abs(cont - H[0, 1][i, j]) < 5e-2 * abs(H[0, 1][i, j])
 
 
+def test_te_tm_closed_interval_on_neumann_cosine_modes():
+    """On the closed interval the stencil sees the field mirrored about the wall nodes.
+
+    f = cos(a x_i) cos(b y_j) with a = pi m / (nx - 1), b = pi l / (ny - 1) is its own mirror
+    extension (f[-1] = f[1], f[n] = f[n - 2]), so at every point, walls and corners included,
+    dxx = (2 cos a - 2) / dx^2 f, dyy = (2 cos b - 2) / dy^2 f and
+    dxy = sin a sin b sin(a i) sin(b j) / (dx dy).  A periodic wrap, an edge-repeating mirror,
+    swapped axes or a flipped 2i dxy sign each miss these values at the walls or everywhere.
+    """
+    g = O.Grid(13, 9, 24.0, 12.0, boundary="closed-interval")
+    p = O.Physics(te_tm=0.05)
+    i = np.arange(g.nx)[None, :]
+    j = np.arange(g.ny)[:, None]
+    a, b = np.pi * 3 / (g.nx - 1), np.pi * 2 / (g.ny - 1)
+    mode = np.cos(a * i) * np.cos(b * j)
+    dxx = (2 * np.cos(a) - 2) / g.dx ** 2 * mode
+    dyy = (2 * np.cos(b) - 2) / g.dy ** 2 * mode
+    dxy = np.sin(a) * np.sin(b) * np.sin(a * i) * np.sin(b * j) / (g.dx * g.dy)
+    for src in (0, 1):
+        field = np.zeros((1, 2) + g.shape, dtype=complex)
+        field[0, src] = (0.6 - 0.8j) * mode
+        out = O.te_tm_energy(p, field, g.dx, g.dy, boundary="closed-interval")
+        dst = 1 - src
+        sign = -1 if dst == 0 else 1          # psi_+ row gets -2i dxy, psi_- row +2i dxy
+        expect = -p.te_tm * ((dxx - dyy) + sign * 2j * dxy) * (0.6 - 0.8j)
+        np.testing.assert_allclose(out[0, dst], expect, atol=1e-12)
+        np.testing.assert_allclose(out[0, src], 0, atol=1e-15)
+    # the periodic stencil on the same samples differs at the walls
+    field = np.zeros((1, 2) + g.shape, dtype=complex)
+    field[0, 1] = mode
+    per = O.te_tm_energy(p, field, g.dx, g.dy)
+    assert np.abs(per[0, 0] - O.te_tm_energy(p, field, g.dx, g.dy, boundary="closed-interval")[0, 0]).max() > 1e-3
+
+
 # ---------------------------------------------------------------- spinor dispersion
 @pytest.mark.parametrize("zeeman,te_tm", [(0.02, 0.05), (0.0, 0.05), (0.02, 0.0), (0.0, 0.0)])
 def test_spinor_propagator_equals_matrix_exponential(zeeman, te_tm):
