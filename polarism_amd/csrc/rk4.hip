@@ -770,8 +770,8 @@ __device__ __forceinline__ const V& at_offset(const V* base, uint32_t e) {
 
 // Value of the next (lane + 1) or previous (lane - 1) lane of the wavefront (DPP wave_shl:1 /
 // wave_shr:1); the last / first lane receives 0, a halo column whose value is never used.
-__device__ __forceinline__ int lane_next_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false); }
-__device__ __forceinline__ int lane_prev_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false); }
+__device__ __forceinline__ int lane_next_i(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, true); }
+__device__ __forceinline__ int lane_prev_i(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true); }
 __device__ __forceinline__ float lane_next(float v) { return __builtin_bit_cast(float, lane_next_i(__builtin_bit_cast(int, v))); }
 __device__ __forceinline__ float lane_prev(float v) { return __builtin_bit_cast(float, lane_prev_i(__builtin_bit_cast(int, v))); }
 __device__ __forceinline__ double lane_next(double v) {
