@@ -55,3 +55,16 @@ def test_corrupt_cache_is_ignored(tmp_path):
     t = Autotuner(path)
     assert t.lookup("k") is None
     assert t.choose("k", [LaunchChoice("stage", 16)], lambda o: 1.0) == LaunchChoice("stage", 16)
+
+
+def test_sample_members_takes_the_whole_table_within_the_budget():
+    from types import SimpleNamespace as S
+    from polarism_amd.autotune import sample_members
+    specs = [S(nx=256, ny=256) for _ in range(256)]            # 16.8 M points
+    assert sample_members(specs, 1 << 25) == specs
+    mixed = [S(nx=n, ny=n) for n in (128, 256, 384, 512) * 64]  # 31.5 M points
+    sub = sample_members(mixed, 1 << 24)
+    assert sum(s.nx * s.ny for s in sub) <= 1 << 24
+    assert len(sub) > len(mixed) // 3                            # close to the budget, not a token sample
+    assert {s.nx for s in sub} == {128, 256, 384, 512}           # the size mix is kept
+    assert len(sample_members([S(nx=8192, ny=8192)], 1 << 20)) == 1
