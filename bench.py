@@ -47,9 +47,11 @@ Extra legs (reported beside the headline, not part of value):
               GB/s with 2 x 8 B per point per transform).
   ffthbm:     batched forward 2-D FFT of 16 complex64 2048^2 planes.
   lawson:     spinor Lawson IFRK DP5, adaptive 1 ps ticks, 1024^2, double
-              reservoir, complex64 (device and host trial controller).
-  fno:        the surrogate's BF16 MFMA GEMM (8192^3) and its forward /
-              training step at batch 16 on 256^2 maps.
+              reservoir, complex64 (device and host trial controller), and
+              the complex64 field against a complex128 run after 6 ps.
+  fno:        the surrogate's BF16 MFMA GEMM (8192^3, B row-major and B with
+              k contiguous) and its forward / training step at batch 16 on
+              256^2 maps.
 
 cpu_baseline: the NumPy oracle's rk4-fdm step (complex64, one 256 x 256
 member) on one host core and on a pool of up to 16 processes (one member
@@ -577,6 +579,28 @@ def lawson_leg(ticks=20):
                 "ms_per_trial": round(wall * 1e3 / max(trials, 1), 3), "ps_per_s": round(ticks / wall, 3),
                 "dt_last_ps": sc.dt_last,
                 "fft_GBps_16_transforms_per_trial": round(transforms * 16.0 * pts * trials / wall / 1e9, 1)}
+    # the complex64 run against the same configuration in complex128 after a few ticks (each with
+    # its own adaptive trial sequence at rtol 1e-5, so the difference holds both the precision
+    # and the step choices)
+    span = 6
+    fields, counts = {}, {}
+    for prec in ("single", "double"):
+        cfg.solver.precision = prec
+        cfg.solver.total_time = float(span)
+        ctl = SimulationController(cfg)
+        ctl.solver.advance(ctl.model, ctl.state, span)
+        ctl.solver.stream.synchronize()
+        fields[prec] = ctl.state.field.get().astype(np.complex128)
+        sc = ctl.solver.step_controller
+        counts[prec] = {"accepted": sc.accepted, "rejected": sc.rejected}
+    a, b = fields["single"], fields["double"]
+    rho_a, rho_b = np.abs(a) ** 2, np.abs(b) ** 2
+    out["fp32_vs_fp64"] = {
+        "ps": span, "trials": counts,
+        "psi_rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+        "density_rel_l2": float(np.linalg.norm(rho_a - rho_b) / np.linalg.norm(rho_b)),
+        "norm_rel_diff": float(abs(rho_a.sum() - rho_b.sum()) / rho_b.sum())}
+    cfg.solver.precision = "single"
     return out
 
 
@@ -591,17 +615,24 @@ def fno_leg(n=8192, reps=10):
     a = torch.randn(n, n, device=dev).to(torch.bfloat16)
     b = torch.randn(n, n, device=dev).to(torch.bfloat16)
     c = torch.empty(n, n, device=dev)
-    run = lambda: gemm(a, b, c, n, n, n, sa=(n, 1, 0), sb=(n, 1, 0), sc=(n, 1, 0))
-    run()
-    torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record()
-    for _ in range(reps):
+
+    def gemm_ms(sb):
+        run = lambda: gemm(a, b, c, n, n, n, sa=(n, 1, 0), sb=sb, sc=(n, 1, 0))
         run()
-    stop.record()
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop) / reps
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(reps):
+            run()
+        stop.record()
+        torch.cuda.synchronize()
+        return start.elapsed_time(stop) / reps
+
+    # B row-major (n contiguous), then B with k contiguous (the transposed operand)
+    ms = gemm_ms((n, 1, 0))
     tflops = 2.0 * n ** 3 / (ms * 1e-3) / 1e12
+    ms_t = gemm_ms((1, n, 0))
+    tflops_t = 2.0 * n ** 3 / (ms_t * 1e-3) / 1e12
     cfg = FNOConfig()
     mod = FNOModule(cfg, seed=0)
     x = torch.randn(cfg.fin, 16, cfg.H, cfg.W, device=dev)
@@ -628,6 +659,8 @@ def fno_leg(n=8192, reps=10):
         t[name] = start.elapsed_time(stop) / reps
     return {"gemm_bf16": {"n": n, "avg_us": round(ms * 1e3, 1), "TFLOPs": round(tflops, 1),
                           "frac_of_2500_TFLOPs_dense_bf16": round(tflops / 2500.0, 4)},
+            "gemm_bf16_k_contiguous_b": {"n": n, "avg_us": round(ms_t * 1e3, 1), "TFLOPs": round(tflops_t, 1),
+                                         "frac_of_2500_TFLOPs_dense_bf16": round(tflops_t / 2500.0, 4)},
             "fno_batch16_256": {"params": sum(p.numel() for p in mod.parameters()),
                                 "forward_ms": round(t["forward"], 2), "train_step_ms": round(t["train_step"], 2)}}
 
