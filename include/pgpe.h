@@ -203,7 +203,11 @@ int pgpe_rk4_stage(const pgpe_rk4_args* args, void* stream);
  * same result as the four pgpe_rk4_stage launches (bit for bit on periodic grids, to
  * rounding on closed intervals, whose mirrored halo points are recomputed).  Replaces the
  * reference's one-launch tiled step (rk4_cuda_solver.py:1835-2194).  Scalar fields
- * (components 1); tile_y is the table's tile_h: 16, 32, or 64 (single precision).
+ * (components 1): tile_y is the table's tile_h, 16, 32, or 64 (single precision).  Spinor
+ * fields (components 2, res_components 1 or 2): single precision, tile_y 16, periodic
+ * grids only (closed intervals take pgpe_rk4_stage: the TE-TM cross term changes sign under
+ * a reflection, so the whole step cannot recompute a mirrored halo; not checked here, the
+ * table is device memory).
  * Uses psi0, res0, pump, psi_out, res_out, noise, step, step_base and coef, which here
  * points at 4 x members coefficients (stage s of member m at coef[s * members + m]).
  * psi_out must not alias psi0 and res_out must not alias res0 (tiles read their

@@ -402,6 +402,77 @@ __global__ void __launch_bounds__(256) rk4_stage_kernel(const Rk4Params<T> p) {
 }
 
 // ------------------------------------------------------------------ spinor
+// Spinor derivatives at one point (model.py:129-208, the stencil of rk4_cuda_solver.py:1600-1637)
+// from the stage input of both components in LDS (planes t0 = psi+, t1 = psi-, point ci, row
+// stride W, the point's eight neighbours in place), the stage reservoir n (per component, or one
+// shared in slot 0 when !two) and the pump P; d per component, dn per reservoir slot.  The
+// spinor stage and whole-step kernels both call this, so they perform the same operations in the
+// same order.
+template <typename T, int RES>
+__device__ __forceinline__ void spinor_rhs(const pgpe_member& mb, const c2<T>* t0, const c2<T>* t1, int ci, int W,
+                                           bool two, const T (&n)[2][2], T P, c2<T> (&d)[2], T (&dn)[2][2]) {
+    constexpr int NF = ResFields<RES>::n;
+    const T inv_dx2 = (T)mb.inv_dx2, inv_dy2 = (T)mb.inv_dy2, inv4 = (T)mb.inv_4dxdy;
+    const T kin = (T)mb.kin, inv_hbar = (T)mb.inv_hbar;
+    const T gC = (T)mb.g_C, gR = (T)mb.g_R, gI = (T)mb.g_I, R = (T)mb.R, gammaC = (T)mb.gamma_C;
+    const T a2 = (T)mb.alpha_2, zh = (T)mb.zeeman, beta = (T)mb.te_tm;
+    const T eta_c = (T)mb.eta, hb2m = (T)mb.hbar_over_2m;
+    c2<T> ps[2], lap[2], te[2];
+    T rho[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const c2<T>* t = c == 0 ? t0 : t1;
+        const c2<T> z = t[ci], nn = t[ci + W], so = t[ci - W], e = t[ci + 1], w = t[ci - 1];
+        ps[c] = z;
+        rho[c] = z.x * z.x + z.y * z.y;
+        lap[c].x = ((nn.x - (T)2 * z.x) + so.x) * inv_dy2 + ((e.x - (T)2 * z.x) + w.x) * inv_dx2;
+        lap[c].y = ((nn.y - (T)2 * z.y) + so.y) * inv_dy2 + ((e.y - (T)2 * z.y) + w.y) * inv_dx2;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        te[c] = make_c<T>(0, 0);
+        if (beta == (T)0) continue;
+        const c2<T>* t = c == 0 ? t1 : t0;   // the other component
+        const c2<T> z = t[ci], nn = t[ci + W], so = t[ci - W], e = t[ci + 1], w = t[ci - 1];
+        const c2<T> ne = t[ci + W + 1], nw = t[ci + W - 1], se = t[ci - W + 1], sw = t[ci - W - 1];
+        c2<T> dxx, dyy, dxy;
+        dxx.x = ((e.x - (T)2 * z.x) + w.x) * inv_dx2;
+        dxx.y = ((e.y - (T)2 * z.y) + w.y) * inv_dx2;
+        dyy.x = ((nn.x - (T)2 * z.x) + so.x) * inv_dy2;
+        dyy.y = ((nn.y - (T)2 * z.y) + so.y) * inv_dy2;
+        dxy.x = (((ne.x - nw.x) - se.x) + sw.x) * inv4;
+        dxy.y = (((ne.y - nw.y) - se.y) + sw.y) * inv4;
+        // cross = -2i dxy (c = 0) or +2i dxy (c = 1)
+        const c2<T> cross = c == 0 ? make_c<T>((T)2 * dxy.y, -(T)2 * dxy.x) : make_c<T>(-(T)2 * dxy.y, (T)2 * dxy.x);
+        te[c].x = -beta * ((dxx.x - dyy.x) + cross.x);
+        te[c].y = -beta * ((dxx.y - dyy.y) + cross.y);
+    }
+#pragma unroll
+    for (int rc = 0; rc < 2; ++rc) {
+        if (rc > 0 && !two) continue;
+        const T rr = two ? rho[rc] : rho[0] + rho[1];
+        const T Pc = two ? P * (T)(rc == 0 ? mb.pump_w_plus : mb.pump_w_minus) : P;
+        reservoir_rhs<T, RES>(mb, rr, Pc, n[rc], dn[rc]);
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const T nA = NF > 0 ? (two ? n[c][0] : n[0][0]) : (T)0;
+        const T nI = NF > 1 ? (two ? n[c][1] : n[0][1]) : (T)0;
+        const T sgn = c == 0 ? (T)1 : (T)-1;
+        const T eff = ((((gC * rho[c]) + a2 * rho[1 - c]) + zh * sgn / (T)2) + gR * nA) + gI * nI;
+        const T gain = (R * nA - gammaC) * (T)0.5;
+        const T Xr = (kin * lap[c].x + eff * ps[c].x) + te[c].x;
+        const T Xi = (kin * lap[c].y + eff * ps[c].y) + te[c].y;
+        d[c].x = inv_hbar * Xi + gain * ps[c].x;
+        d[c].y = -(inv_hbar * Xr) + gain * ps[c].y;
+        if (eta_c != (T)0) {
+            const T s = (eta_c * nA) * hb2m;
+            d[c].x = d[c].x + s * lap[c].x;
+            d[c].y = d[c].y + s * lap[c].y;
+        }
+    }
+}
+
 // Circular-basis spinor stage (model.py:129-208): per component c, with o the
 // other component,
 //   X_c = -hbar^2/2m lap psi_c + (gC rho_c + a2 rho_o +- Z/2 + gR nA + gI nI) psi_c
@@ -454,52 +525,19 @@ __global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T
     const int lane = threadIdx.x & 63;
     const int gx = x0 + lane;
     if (gx >= nx) return;
-    const T inv_dx2 = (T)mb.inv_dx2, inv_dy2 = (T)mb.inv_dy2, inv4 = (T)mb.inv_4dxdy;
-    const T kin = (T)mb.kin, inv_hbar = (T)mb.inv_hbar;
-    const T gC = (T)mb.g_C, gR = (T)mb.g_R, gI = (T)mb.g_I, R = (T)mb.R, gammaC = (T)mb.gamma_C;
-    const T a2 = (T)mb.alpha_2, zh = (T)mb.zeeman, beta = (T)mb.te_tm;
-    const T eta_c = (T)mb.eta, hb2m = (T)mb.hbar_over_2m;
     const T coef = p.coef[m];
     const T* __restrict__ env = p.pump + mb.pump_off;
     uint64_t step = p.step;
     if (KIND == 2 && p.noise && p.step_base) step += *p.step_base;
+    const bool two = res_comps == 2;
 
     for (int r = threadIdx.x >> 6; r < TH; r += 4) {
         const int gy = y0 + r;
         if (gy >= ny) break;
         const int64_t cell = (int64_t)gy * nx + gx;
         const int ci = (r + 1) * W + lane + 1;
-        c2<T> ps[2], lap[2], te[2];
-        T rho[2];
-        for (int c = 0; c < 2; ++c) {
-            const c2<T>* t = s_psi + c * PL;
-            const c2<T> z = t[ci], n = t[ci + W], so = t[ci - W], e = t[ci + 1], w = t[ci - 1];
-            ps[c] = z;
-            rho[c] = z.x * z.x + z.y * z.y;
-            lap[c].x = ((n.x - (T)2 * z.x) + so.x) * inv_dy2 + ((e.x - (T)2 * z.x) + w.x) * inv_dx2;
-            lap[c].y = ((n.y - (T)2 * z.y) + so.y) * inv_dy2 + ((e.y - (T)2 * z.y) + w.y) * inv_dx2;
-        }
-        for (int c = 0; c < 2; ++c) {
-            te[c] = make_c<T>(0, 0);
-            if (beta == (T)0) continue;
-            const c2<T>* t = s_psi + (1 - c) * PL;   // the other component
-            const c2<T> z = t[ci], n = t[ci + W], so = t[ci - W], e = t[ci + 1], w = t[ci - 1];
-            const c2<T> ne = t[ci + W + 1], nw = t[ci + W - 1], se = t[ci - W + 1], sw = t[ci - W - 1];
-            c2<T> dxx, dyy, dxy;
-            dxx.x = ((e.x - (T)2 * z.x) + w.x) * inv_dx2;
-            dxx.y = ((e.y - (T)2 * z.y) + w.y) * inv_dx2;
-            dyy.x = ((n.x - (T)2 * z.x) + so.x) * inv_dy2;
-            dyy.y = ((n.y - (T)2 * z.y) + so.y) * inv_dy2;
-            dxy.x = (((ne.x - nw.x) - se.x) + sw.x) * inv4;
-            dxy.y = (((ne.y - nw.y) - se.y) + sw.y) * inv4;
-            // cross = -2i dxy (c = 0) or +2i dxy (c = 1)
-            const c2<T> cross = c == 0 ? make_c<T>((T)2 * dxy.y, -(T)2 * dxy.x) : make_c<T>(-(T)2 * dxy.y, (T)2 * dxy.x);
-            te[c].x = -beta * ((dxx.x - dyy.x) + cross.x);
-            te[c].y = -beta * ((dxx.y - dyy.y) + cross.y);
-        }
-        // reservoir stage values and derivatives; the component loops run over both slots with
-        // constant indices (guarded by res_comps) so that these arrays stay in registers
-        const bool two = res_comps == 2;
+        // reservoir stage values; the component loops run over both slots with constant indices
+        // (guarded by res_comps) so that these arrays stay in registers
         T n[2][2] = {{0, 0}, {0, 0}}, dn[2][2] = {{0, 0}, {0, 0}};
         const T P = env[cell] * coef;
 #pragma unroll
@@ -512,28 +550,9 @@ __global__ void __launch_bounds__(256) rk4_spinor_stage_kernel(const Rk4Params<T
                 if constexpr (KIND != 0) v = v + p.a * p.kres_prev[f][g];
                 n[rc][f] = v;
             }
-            const T rr = res_comps == 2 ? rho[rc] : rho[0] + rho[1];
-            const T Pc = res_comps == 2 ? P * (T)(rc == 0 ? mb.pump_w_plus : mb.pump_w_minus) : P;
-            reservoir_rhs<T, RES>(mb, rr, Pc, n[rc], dn[rc]);
         }
         c2<T> d[2];
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const T nA = NF > 0 ? (two ? n[c][0] : n[0][0]) : (T)0;
-            const T nI = NF > 1 ? (two ? n[c][1] : n[0][1]) : (T)0;
-            const T sgn = c == 0 ? (T)1 : (T)-1;
-            const T eff = ((((gC * rho[c]) + a2 * rho[1 - c]) + zh * sgn / (T)2) + gR * nA) + gI * nI;
-            const T gain = (R * nA - gammaC) * (T)0.5;
-            const T Xr = (kin * lap[c].x + eff * ps[c].x) + te[c].x;
-            const T Xi = (kin * lap[c].y + eff * ps[c].y) + te[c].y;
-            d[c].x = inv_hbar * Xi + gain * ps[c].x;
-            d[c].y = -(inv_hbar * Xr) + gain * ps[c].y;
-            if (eta_c != (T)0) {
-                const T s = (eta_c * nA) * hb2m;
-                d[c].x = d[c].x + s * lap[c].x;
-                d[c].y = d[c].y + s * lap[c].y;
-            }
-        }
+        spinor_rhs<T, RES>(mb, s_psi, s_psi + PL, ci, W, two, n, P, d, dn);
         if constexpr (KIND != 2) {
 #pragma unroll
             for (int c = 0; c < 2; ++c) p.kout[poff + c * plane + cell] = d[c];
@@ -609,6 +628,7 @@ struct Rk4StepParams {
     uint64_t step;
     const uint64_t* step_base;
     const int32_t* active_tiles;  // device count of tile_map entries to run, or null (num_tiles)
+    int res_comps;                // spinor members: reservoir planes per member (1 shared, 2 per component)
 };
 
 constexpr int kStepHalo = 4;
@@ -738,6 +758,168 @@ __global__ void __launch_bounds__(256) rk4_step_kernel(const Rk4StepParams<T> p)
                                    (T)mb.noise_dt_over_4v);
                 }
                 p.psi_out[off + idx] = out;
+            }
+        }
+        if (s < 3) __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ whole step, spinor
+// Whole RK4 step of a spinor member tile on a periodic grid: the temporal blocking of
+// rk4_step_kernel (64 x TH output points, a 4-point halo recomputed stage by stage, the stage
+// inputs of both components double-buffered in LDS, the per-point state in registers) with
+// spinor_rhs at every point and the spinor stage kernels' combination order, so that the two
+// paths agree bit for bit.  Closed intervals take the stage kernels: their deeper halo would be
+// recomputed beyond the wall, and the TE-TM cross term changes sign under a reflection, so the
+// mirrored field does not evolve as the mirror image (the reference's spinor step kernel is
+// periodic as well, rk4_cuda_solver.py:1835-1853).
+template <typename T, int RES, int TH>
+__global__ void __launch_bounds__(512) rk4_spinor_step_kernel(const Rk4StepParams<T> p) {
+    constexpr int NT = 512;
+    constexpr int NF = ResFields<RES>::n;
+    constexpr int EW = kTileW + 2 * kStepHalo;
+    constexpr int EH = TH + 2 * kStepHalo;
+    constexpr int NP = EW * EH;
+    constexpr int PPT = (NP + NT - 1) / NT;
+    // each component's buffer has a one-row-plus-one pad on both sides (every neighbour
+    // f +- EW +- 1 is an LDS address); buffer b of component c starts (2 b + c) PLANE in
+    constexpr int PAD = EW + 1;
+    constexpr int PLANE = NP + 2 * PAD;
+    extern __shared__ unsigned char smem_raw[];
+    c2<T>* base = reinterpret_cast<c2<T>*>(smem_raw) + PAD;
+
+    const int tile = xcd_tile(blockIdx.x, p.active_tiles ? *p.active_tiles : p.num_tiles);
+    if (tile < 0) return;
+    const int m = p.tile_map[2 * tile];
+    const pgpe_member mb = p.table[m];
+    const int nx = mb.nx, ny = mb.ny;
+    const int local = p.tile_map[2 * tile + 1];
+    const int x0 = (local % mb.tiles_x) * kTileW;
+    const int y0 = (local / mb.tiles_x) * TH;
+    const int64_t plane = (int64_t)nx * ny;
+    const int64_t poff = mb.off * 2;
+    const int res_comps = p.res_comps;
+    const bool two = res_comps == 2;
+    const int64_t roff = mb.off * res_comps;
+    T coef[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) coef[s] = p.coef[s * p.members + m];
+    uint64_t step = p.step;
+    if (p.noise && p.step_base) step += *p.step_base;
+
+    c2<T> z[PPT][2], acc[PPT][2];
+    T env[PPT], n0[PPT][2][2], ns[PPT][2][2], accn[PPT][2][2];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const int f = tid + NT * j;
+#pragma unroll
+        for (int rc = 0; rc < 2; ++rc)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) n0[j][rc][q] = ns[j][rc][q] = accn[j][rc][q] = (T)0;
+        if (f < NP) {
+            const int ly = f / EW;
+            const int lx = f - ly * EW;
+            const int gy = map_index(y0 + ly - kStepHalo, ny, mb.boundary);
+            const int gx = map_index(x0 + lx - kStepHalo, nx, mb.boundary);
+            const int64_t cell = (int64_t)gy * nx + gx;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                z[j][c] = p.psi0[poff + c * plane + cell];
+                base[c * PLANE + f] = z[j][c];
+            }
+            env[j] = p.pump[mb.pump_off + cell];
+#pragma unroll
+            for (int rc = 0; rc < 2; ++rc) {
+                if (rc >= res_comps) continue;
+#pragma unroll
+                for (int q = 0; q < NF; ++q) {
+                    n0[j][rc][q] = p.res0[q][roff + rc * plane + cell];
+                    ns[j][rc][q] = n0[j][rc][q];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const c2<T>* cur = base + (s & 1) * 2 * PLANE;
+        c2<T>* nxt = base + ((s + 1) & 1) * 2 * PLANE;
+        const T a_next = s < 2 ? p.a_half : p.a_full;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const int f = tid + NT * j;
+            if (f >= NP) continue;
+            const T P = env[j] * coef[s];
+            c2<T> d[2];
+            T dn[2][2] = {{0, 0}, {0, 0}};
+            spinor_rhs<T, RES>(mb, cur, cur + PLANE, f, EW, two, ns[j], P, d, dn);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                if (s == 0) {
+                    acc[j][c] = d[c];
+                } else if (s < 3) {
+                    acc[j][c].x = acc[j][c].x + (T)2 * d[c].x;
+                    acc[j][c].y = acc[j][c].y + (T)2 * d[c].y;
+                }
+            }
+#pragma unroll
+            for (int rc = 0; rc < 2; ++rc) {
+                if (rc >= res_comps) continue;
+#pragma unroll
+                for (int q = 0; q < NF; ++q) {
+                    if (s == 0) accn[j][rc][q] = dn[rc][q];
+                    else if (s < 3) accn[j][rc][q] = accn[j][rc][q] + (T)2 * dn[rc][q];
+                }
+            }
+            if (s < 3) {
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    c2<T> v;
+                    v.x = z[j][c].x + a_next * d[c].x;
+                    v.y = z[j][c].y + a_next * d[c].y;
+                    nxt[c * PLANE + f] = v;
+                }
+#pragma unroll
+                for (int rc = 0; rc < 2; ++rc) {
+                    if (rc >= res_comps) continue;
+#pragma unroll
+                    for (int q = 0; q < NF; ++q) ns[j][rc][q] = n0[j][rc][q] + a_next * dn[rc][q];
+                }
+            } else {
+                const int ly = f / EW;
+                const int lx = f - ly * EW;
+                const int oy = y0 + ly - kStepHalo, ox = x0 + lx - kStepHalo;
+                if (lx < kStepHalo || lx >= kStepHalo + kTileW || ly < kStepHalo || ly >= kStepHalo + TH ||
+                    ox >= nx || oy >= ny)
+                    continue;
+                const int64_t cell = (int64_t)oy * nx + ox;
+                T n_new[2][2] = {{0, 0}, {0, 0}};
+#pragma unroll
+                for (int rc = 0; rc < 2; ++rc) {
+                    if (rc >= res_comps) continue;
+#pragma unroll
+                    for (int q = 0; q < NF; ++q) {
+                        T v = n0[j][rc][q] + p.dt6 * (accn[j][rc][q] + dn[rc][q]);
+                        v = v > (T)0 ? v : (T)0;  // set_state clamp (single_reservoir.py:70-73)
+                        n_new[rc][q] = v;
+                        p.res_out[q][roff + rc * plane + cell] = v;
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    c2<T> out;
+                    out.x = z[j][c].x + p.dt6 * (acc[j][c].x + d[c].x);
+                    out.y = z[j][c].y + p.dt6 * (acc[j][c].y + d[c].y);
+                    if (p.noise) {
+                        const T act = NF > 0 ? (two ? n_new[c][0] : n_new[0][0]) : (T)0;
+                        wigner_site<T>(out.x, out.y, mb.seed, (uint32_t)cell, step, 1u + (uint32_t)c,
+                                       (T)mb.noise_strength, (T)mb.noise_gamma, (T)mb.noise_R, act,
+                                       (T)mb.noise_dt_over_4v);
+                    }
+                    p.psi_out[poff + c * plane + cell] = out;
+                }
             }
         }
         if (s < 3) __syncthreads();
@@ -1157,8 +1339,34 @@ static int launch_step_th(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hip
     return PGPE_OK;
 }
 
+// Spinor whole step, single precision: 64 x 16 tiles on 512 threads (the spinor stage kernels'
+// tile map of height 16).  Double precision would hold more than the 256 registers a 512-thread
+// workgroup allows, so double-precision spinors take the stage kernels.
+template <typename T, int RES>
+static int launch_spinor_step(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
+    constexpr int TH = 16;
+    if constexpr (sizeof(T) != 4) {
+        return fail(PGPE_ERR_ARG, "pgpe_rk4_step: spinor fields are single precision only (the stage kernels take both)");
+    } else {
+    if (a->strips || a->tile_y != TH) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: spinor fields take tile_y 16 (no strips)");
+    constexpr int NP = (kTileW + 2 * kStepHalo) * (TH + 2 * kStepHalo);
+    constexpr int PAD = kTileW + 2 * kStepHalo + 1;
+    constexpr size_t shm = 4 * (NP + 2 * PAD) * sizeof(c2<T>);
+    static_assert(shm <= 160 * 1024, "spinor step tile exceeds the LDS");
+    auto kern = rk4_spinor_step_kernel<T, RES, TH>;
+    if (shm > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute(rk4_spinor_step_kernel)");
+    }
+    hipLaunchKernelGGL(kern, dim3(xcd_grid(a->num_tiles)), dim3(512), shm, s, p);
+    PGPE_LAUNCHED("rk4_spinor_step_kernel");
+    return PGPE_OK;
+    }
+}
+
 template <typename T, int RES>
 static int launch_step_res(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
+    if (a->components == 2) return launch_spinor_step<T, RES>(a, p, s);
     if (a->strips) return launch_strip_res<T, RES>(a, p, s);
     switch (a->tile_y) {
         case 16: return launch_step_th<T, RES, 16>(a, p, s);
@@ -1192,6 +1400,7 @@ static int launch_step_typed(const pgpe_rk4_args* a, hipStream_t s) {
     p.step = a->step;
     p.step_base = a->step_base;
     p.active_tiles = a->active_tiles;
+    p.res_comps = a->components == 2 ? a->res_components : 1;
     switch (a->reservoir) {
         case PGPE_RES_NONE: return launch_step_res<T, PGPE_RES_NONE>(a, p, s);
         case PGPE_RES_SINGLE: return launch_step_res<T, PGPE_RES_SINGLE>(a, p, s);
@@ -1323,7 +1532,9 @@ extern "C" int pgpe_rk4_stage(const pgpe_rk4_args* a, void* stream) {
 extern "C" int pgpe_rk4_step(const pgpe_rk4_args* a, void* stream) {
     if (!a) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: null args");
     if (a->num_tiles <= 0 || a->members <= 0) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: empty member table");
-    if (a->components != 1) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: scalar fields only (components 1)");
+    if (a->components != 1 && a->components != 2) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: components must be 1 or 2");
+    if (a->components == 2 && a->res_components != 1 && a->res_components != 2)
+        return fail(PGPE_ERR_ARG, "pgpe_rk4_step: res_components must be 1 or 2");
     if (!a->table || !a->tile_map || !a->psi0 || !a->psi_out || !a->pump || !a->coef)
         return fail(PGPE_ERR_ARG, "pgpe_rk4_step: null table, field, pump or coefficient");
     if (a->psi_out == a->psi0) return fail(PGPE_ERR_ARG, "pgpe_rk4_step: psi_out must not alias psi0");

@@ -29,6 +29,15 @@ def complex_dtype(precision):
     return np.complex64 if precision == "single" else np.complex128
 
 
+def spinor_whole_step_applies(table, precision: str) -> bool:
+    """Whether the spinor whole-step kernel runs this member table: single precision (double
+    needs more registers than its workgroup has), tiles of height 16 and periodic grids only (on
+    a closed interval the TE-TM cross term changes sign under the wall's reflection, so the halo
+    cannot be recomputed there; the stage kernels mirror one point per stage instead)."""
+    return (precision == "single" and table.tile_h == 16
+            and all(getattr(s, "boundary", "periodic") == "periodic" for s in table.specs))
+
+
 class Rk4Engine:
     def __init__(self, table: MemberTable, pump: DeviceArray, precision: str, reservoir: str,
                  stream: Stream | None = None, whole_step: bool = False, strip_h: int | None = None):
@@ -48,9 +57,12 @@ class Rk4Engine:
         c, r = self.cdtype, self.rdtype
         self.whole_step = bool(whole_step)
         if self.whole_step:
-            if self.components != 1:
-                raise ValueError("whole_step runs scalar fields only")
-            if table.tile_h not in (16, 32, 64) or (table.tile_h == 64 and precision != "single"):
+            if self.components == 2:
+                # the spinor whole-step kernel: single precision, periodic grids, 64 x 16 tiles
+                if not spinor_whole_step_applies(table, precision):
+                    raise ValueError("whole_step runs spinor fields in single precision on periodic grids "
+                                     "with tile_h 16")
+            elif table.tile_h not in (16, 32, 64) or (table.tile_h == 64 and precision != "single"):
                 raise ValueError("whole_step needs tile_h 16 or 32 (or 64 in single precision)")
         elif table.tile_h not in (8, 16, 32):
             raise ValueError("the stage kernels need tile_h 8, 16 or 32")

@@ -15,7 +15,7 @@ from __future__ import annotations
 import numpy as np
 
 from ..device import DeviceArray
-from ..engine import Rk4Engine
+from ..engine import Rk4Engine, spinor_whole_step_applies
 from ..solver_api import ACCEPTED, SolverCapabilities, SolverCompatibilityError, register_solver
 from .base import HipSolverBase
 
@@ -99,9 +99,13 @@ class RK4FDMHipSolver(HipSolverBase):
             per = 0 if self._pot_shared else self.grid.nx * self.grid.ny
             if any(s.pot_off != m * per for m, s in enumerate(tab.specs)):
                 tab.set_pot_offsets([m * per for m in range(batch)])
-        # one whole-step launch per step where it applies (scalar, no potential); the four
-        # stage launches otherwise -- both give the same result
-        whole = self.components == 1 and pot is None and tab.tile_h in (16, 32, 64)
+        # one whole-step launch per step where it applies (scalar with no potential, or spinor
+        # in single precision on a periodic grid); the four stage launches otherwise -- both give
+        # the same result
+        if self.components == 1:
+            whole = pot is None and tab.tile_h in (16, 32, 64)
+        else:
+            whole = spinor_whole_step_applies(tab, self.precision)
         if self.engine is None or self.engine.table is not tab or self.engine.whole_step != whole:
             if self.engine is not None and self.engine.table is tab:
                 self._carry_state(whole)

@@ -157,3 +157,71 @@ def test_strip_options_validated():
             Rk4Engine(e_stage.table, e_stage.pump, "double", "single", whole_step=True, strip_h=bad)
     with pytest.raises(ValueError, match="whole-step"):
         Rk4Engine(e_stage.table, e_stage.pump, "double", "single", strip_h=16)
+
+
+def _spinor_engines(sizes, reservoir, res_components, noise_strength=0.0):
+    """Spinor members (psi+ and psi-, TE-TM, Zeeman, alpha_2, kinetic relaxation) on periodic grids
+    in single precision: the stage launches and the spinor whole-step kernel."""
+    from polarism_amd.config import PhysicsConstants
+    from polarism_amd.device import DeviceArray, init
+    from polarism_amd.engine import Rk4Engine
+    from polarism_amd.layout import MemberSpec, MemberTable
+    init()
+    p = PhysicsConstants()
+    p.g_I = 0.0005
+    p.kinetic_relaxation_eta = 0.02
+    p.te_tm = 0.05
+    p.zeeman = 0.02
+    p.alpha_2 = -1e-4
+    rng = np.random.default_rng(11)
+    specs, envs, off = [], [], 0
+    for i, (nx, ny) in enumerate(sizes):
+        specs.append(MemberSpec(nx=nx, ny=ny, dx=0.5, dy=0.6, physics=p, reservoir=reservoir, components=2,
+                                res_components=res_components, pump_off=off, seed=41 + i,
+                                noise_strength=noise_strength, dt=0.05, circular_polarization=0.3))
+        envs.append(rng.uniform(0.0, 1.0, nx * ny))
+        off += nx * ny
+    pump = DeviceArray.from_numpy(np.concatenate(envs).astype(np.float32))
+    coef = DeviceArray.from_numpy(np.tile(rng.uniform(0.02, 0.08, len(sizes)), (4, 1)).astype(np.float32))
+    e_stage = Rk4Engine(MemberTable(specs, 16), pump, "single", reservoir)
+    e_step = Rk4Engine(MemberTable(specs, 16), pump, "single", reservoir, whole_step=True)
+    total = e_stage.table.total_points
+    psi = (0.3 * (rng.standard_normal(2 * total) + 1j * rng.standard_normal(2 * total))).astype(np.complex64)
+    res = [rng.uniform(0.0, 3.0, res_components * total).astype(np.float32) for _ in range(e_stage.nres)]
+    for e in (e_stage, e_step):
+        e.set_dt(0.05)
+        e.psi[0].upload(psi)
+        for f in range(e.nres):
+            e.res[f].upload(res[f])
+    return e_stage, e_step, coef
+
+
+@pytest.mark.parametrize("reservoir", ["none", "single", "double", "quadratic-double"])
+@pytest.mark.parametrize("res_components", [2, 1])
+def test_spinor_whole_step_equals_stage_launches_bitwise(reservoir, res_components):
+    e_stage, e_step, coef = _spinor_engines(SIZES, reservoir, res_components)
+    for e in (e_stage, e_step):
+        e.run_steps(5, coef, True, False, 0, graphs=False)
+    _compare(e_stage, e_step)
+
+
+@pytest.mark.parametrize("res_components", [2, 1])
+def test_spinor_whole_step_graph_chunks_noise_and_retired_members(res_components):
+    """A graph-captured chunk, then single launches, with both Wigner streams fused in, on a tile
+    map that covers a subset of the members (the others are left untouched in each field buffer,
+    so the step count is even: the result is back in the buffer that holds their values)."""
+    e_stage, e_step, coef = _spinor_engines(SIZES, "single", res_components, noise_strength=1.0)
+    keep = [0, 1, 3]
+    for e in (e_stage, e_step):
+        e.set_active_tiles(e.table.tiles_of(np.asarray(keep)))
+        e.run_steps(6, coef, True, True, 3, graphs=True)
+        e.run_steps(2, coef, True, True, 9, graphs=False)
+    _compare(e_stage, e_step)
+
+
+def test_spinor_whole_step_scope():
+    """Double precision and closed intervals keep the stage kernels; the engine says so."""
+    from polarism_amd.engine import Rk4Engine
+    e_stage, _, _ = _spinor_engines(SIZES[:1], "single", 2)
+    with pytest.raises(ValueError, match="spinor"):
+        Rk4Engine(e_stage.table, e_stage.pump, "double", "single", whole_step=True)
