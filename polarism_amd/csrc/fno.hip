@@ -196,6 +196,13 @@ constexpr size_t kBigLdsBytes = (size_t)2 * (BG_M + BG_N) * BG_LD * sizeof(__hip
 __device__ __forceinline__ uint32_t pack_lo(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x05040100u); }
 __device__ __forceinline__ uint32_t pack_hi(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x07060302u); }
 
+// Row-major B is transposed in registers: thread tid holds the 4 k x 8 n block at (kb, nb) and
+// writes it as eight 8-byte pieces to rows nb .. nb + 7 of the k-contiguous LDS tile (row stride
+// 144 bytes).  Lanes of one wavefront on 16 different n blocks would put their rows 8 x 144 bytes
+// apart, all on two bank offsets (an 8-way conflict); on the 16 k blocks of 4 n blocks they
+// cover every bank twice, the minimum for 512 bytes.
+__device__ __forceinline__ int b_block_n(int tid) { return ((tid >> 6) * 4 + ((tid >> 4) & 3)) * 8; }
+
 template <bool B_KCONTIG>
 __global__ void __launch_bounds__(256) gemm_bf16_big_kernel(const GemmParams p) {
     extern __shared__ __attribute__((aligned(16))) unsigned char big_smem[];
@@ -235,8 +242,9 @@ __global__ void __launch_bounds__(256) gemm_bf16_big_kernel(const GemmParams p) 
                 rb[i] = *reinterpret_cast<const vu32x4*>(B + (int64_t)(n0 + (v >> 3)) * p.sbn + k0 + (v & 7) * 8);
             }
         } else {
-            // block of 4 k x 8 n: 16 blocks along n, 16 along k
-            const int nb = (tid & 15) * 8, kb = (tid >> 4) * 4;
+            // block of 4 k x 8 n: 16 blocks along n, 16 along k; a wavefront's lanes take all 16
+            // k blocks of 4 n blocks (b_block_n)
+            const int nb = b_block_n(tid), kb = (tid & 15) * 4;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 rb[i] = *reinterpret_cast<const vu32x4*>(B + (int64_t)(k0 + kb + i) * p.sbk + n0 + nb);
@@ -257,7 +265,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_big_kernel(const GemmParams p) 
                 *reinterpret_cast<vu32x4*>(&bs[v >> 3][(v & 7) * 8]) = rb[i];
             }
         } else {
-            const int nb = (tid & 15) * 8, kb = (tid >> 4) * 4;
+            const int nb = b_block_n(tid), kb = (tid & 15) * 4;
             const uint32_t w[4][4] = {{rb[0].x, rb[0].y, rb[0].z, rb[0].w},
                                       {rb[1].x, rb[1].y, rb[1].z, rb[1].w},
                                       {rb[2].x, rb[2].y, rb[2].z, rb[2].w},
