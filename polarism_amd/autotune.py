@@ -130,14 +130,16 @@ def sample_members(specs, budget: int):
 
 
 def tune_rk4(specs, precision: str, reservoir: str, *, steps: int = 20, tuner: Autotuner | None = None,
-             sample_points: int = 1 << 25) -> LaunchChoice:
+             sample_points: int = 1 << 25, options: list[LaunchChoice] | None = None) -> LaunchChoice:
     """Fastest RK4 launch form and tile height for the members ``specs`` (MemberSpec list).
 
     Times ``steps`` (even) graph-captured steps per candidate, twice, keeping the faster, on the
     whole member table when it holds at most ``sample_points`` points, else on members drawn
     evenly from ``specs`` up to that many points.  The table has to be near full size: on a
     fraction of it the persistent strip kernels run a few strips per workgroup and short strips
-    time like tall ones, though on the full table the tall ones are faster.
+    time like tall ones, though on the full table the tall ones are faster.  ``options`` limits
+    the forms (default: every candidate of the precision and component count); a cached choice
+    outside them is timed again.
     """
     from .device import DeviceArray, Event, device_name, init
     from .engine import Rk4Engine, real_dtype
@@ -147,8 +149,9 @@ def tune_rk4(specs, precision: str, reservoir: str, *, steps: int = 20, tuner: A
     sizes = [(s.nx, s.ny) for s in specs]
     key = tune_key("rk4-fdm-hip", device_name(), precision, reservoir, components, sizes)
     tuner = tuner or Autotuner()
+    opts = list(options) if options is not None else candidates(precision, components)
     hit = tuner.lookup(key)
-    if hit is not None:
+    if hit is not None and hit in opts:
         return hit
     sub = sample_members(specs, sample_points)
     off = 0
@@ -179,4 +182,4 @@ def tune_rk4(specs, precision: str, reservoir: str, *, steps: int = 20, tuner: A
             best = min(best, start.elapsed_ms(stop) * 1e-3 / steps)
         return best
 
-    return tuner.choose(key, candidates(precision, components), timer)
+    return tuner.choose(key, opts, timer)

@@ -14,8 +14,9 @@ from __future__ import annotations
 
 import numpy as np
 
+from ..autotune import candidates, tune_rk4
 from ..device import DeviceArray
-from ..engine import Rk4Engine, spinor_whole_step_applies
+from ..engine import STRIP_MAX_POINTS, Rk4Engine, spinor_whole_step_applies
 from ..solver_api import ACCEPTED, SolverCapabilities, SolverCompatibilityError, register_solver
 from .base import HipSolverBase
 
@@ -44,6 +45,10 @@ class RK4FDMHipSolver(HipSolverBase):
         self._pot_key = None
         self._pot_obj = None
         self._pot_dev = None
+        # with the default table height, scalar fields without a potential take the whole-step
+        # form the autotuner measures fastest on this member table (LDS tiles or register row
+        # strips; every form gives the same result); False keeps the LDS tiles
+        self.autotune = True
         # rk4_cuda_solver.py:2280-2294: the two stencils, and the 9-point one needs dx == dy
         self.laplacian = getattr(config.solver, "laplacian", "five-point")
         if self.laplacian not in ("five-point", "isotropic-9pt"):
@@ -102,27 +107,40 @@ class RK4FDMHipSolver(HipSolverBase):
         # one whole-step launch per step where it applies (scalar with no potential, or spinor
         # in single precision on a periodic grid); the four stage launches otherwise -- both give
         # the same result
+        strip_h = None
         if self.components == 1:
             whole = pot is None and tab.tile_h in (16, 32, 64)
+            if whole and tab.tile_h == 16 and self.autotune:
+                strip_h = self._tuned_strip_height(tab)
         else:
             whole = spinor_whole_step_applies(tab, self.precision)
-        if self.engine is None or self.engine.table is not tab or self.engine.whole_step != whole:
-            if self.engine is not None and self.engine.table is tab:
-                self._carry_state(whole)
+        e = self.engine
+        if e is None or e.table is not tab or e.whole_step != whole or e.strip_h != strip_h:
+            if e is not None and e.table is tab:
+                self._carry_state(whole, strip_h)
             else:
                 self.engine = Rk4Engine(tab, self._pump_dev, self.precision, self.reservoir_kind, self.stream,
-                                        whole_step=whole)
+                                        whole_step=whole, strip_h=strip_h)
             self._const_coef = None
         self.engine.potential = pot
         if getattr(self.engine, "_dt", None) != self.config.solver.dt:
             self.engine.set_dt(self.config.solver.dt)
         return self.engine
 
-    def _carry_state(self, whole):
+    def _tuned_strip_height(self, tab):
+        """Strip height of the fastest whole-step form on this table of height 16 (None: LDS tiles),
+        from the autotuner's cache or its timing of LDS tiles and every strip height."""
+        if any(s.nx * s.ny > STRIP_MAX_POINTS for s in tab.specs):
+            return None
+        opts = [c for c in candidates(self.precision, 1) if c.whole_step and c.table_h == 16]
+        choice = tune_rk4(tab.specs, self.precision, self.reservoir_kind, options=opts)
+        return choice.strip_h
+
+    def _carry_state(self, whole, strip_h=None):
         """Switch launch form on the same table, keeping the current field and reservoir."""
         old = self.engine
         self.engine = Rk4Engine(old.table, self._pump_dev, self.precision, self.reservoir_kind, self.stream,
-                                whole_step=whole)
+                                whole_step=whole, strip_h=strip_h)
         self.engine.psi[0].copy_from(old.psi[0], self.stream)
         for a, b in zip(self.engine.res, old.res):
             a.copy_from(b, self.stream)

@@ -100,6 +100,24 @@ def test_batch_member_equals_single_run_bitwise():
         np.testing.assert_array_equal(batch.state.field.get()[m], single.state.field.get()[0])
 
 
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_tuned_form_equals_lds_tiles(precision):
+    """The controller's whole-step form chosen by the autotuner (strips or LDS tiles) gives the
+    LDS tiles' result bit for bit on a periodic grid, Wigner noise included."""
+    from polarism_amd import SimulationController
+    cfg = quickstart_config(solver__total_time=0.03, grid__nx=96, grid__ny=72, solver__precision=precision,
+                            noise__stochastic="wigner")
+    runs = []
+    for tuned in (True, False):
+        ctl = SimulationController(cfg)
+        ctl.solver.autotune = tuned
+        ctl.run()
+        runs.append((ctl.state.field.get(), ctl.reservoir.get_state()[0].get(), ctl.solver.engine.strip_h))
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+    assert runs[1][2] is None
+
+
 @pytest.mark.parametrize("tile_h", [8, 32])
 def test_tile_height_does_not_change_results(tile_h):
     from polarism_amd import SimulationController
