@@ -171,7 +171,7 @@ def sweep_step_kernel(reps=20):
     ms = _time_launches(e, lambda: e.launch_step(e.psi[0], e.psi[1], ptrs, False, 0, res_in=r0, res_out=r1), reps)
     if choice.form == "strip":
         # strip height TH = NW R - 8 (wavefronts per workgroup x rows per wavefront, less the halo)
-        nw, r = {16: (4, 6), 24: (8, 4), 32: (8, 5), 40: (8, 6)}[choice.tile_h]
+        nw, r = {8: (4, 4), 16: (4, 6), 24: (8, 4), 32: (8, 5), 40: (8, 6)}[choice.tile_h]
         name = f"rk4_strip_pf_kernel<1, {nw}, {r}, false, false, true>"
         shape = f"row strips of 56 x {choice.tile_h} output points"
     else:

@@ -187,7 +187,7 @@ typedef struct pgpe_rk4_args {
                                         retire and the map is rewritten in place */
     int32_t strips;                  /* pgpe_rk4_step only: 0 = 64 x tile_y tiles of the member
                                         table; 1 = row strips of 56 output columns x tile_y rows
-                                        (16, 24, or 32 and 40 in single precision), local index
+                                        (8, 16, 24, or 32 and 40 in single precision), local index
                                         (row << 16) | column, from the strip map; every
                                         member must hold at most 2^28 points (the kernel
                                         addresses a member with 32-bit byte offsets) */

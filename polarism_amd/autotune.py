@@ -49,7 +49,7 @@ def candidates(precision: str, components: int) -> list[LaunchChoice]:
     out = [LaunchChoice("stage", h) for h in (8, 16, 32)]
     if components == 1:
         out += [LaunchChoice("step", h) for h in (16, 32)]
-        out += [LaunchChoice("strip", h) for h in (16, 24)]
+        out += [LaunchChoice("strip", h) for h in (8, 16, 24)]
         if precision == "single":
             out += [LaunchChoice("step", 64), LaunchChoice("strip", 32), LaunchChoice("strip", 40)]
     return out

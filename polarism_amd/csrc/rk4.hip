@@ -1309,6 +1309,7 @@ template <typename T, int RES>
 static int launch_strip_res(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, hipStream_t s) {
     // strip height TH = NW R - 8: wavefronts per workgroup x rows per wavefront, less the halo
     switch (a->tile_y) {
+        case 8: return launch_strip_r<T, RES, 4, 4>(a, p, s);
         case 16: return launch_strip_r<T, RES, 4, 6>(a, p, s);
         case 24: return launch_strip_r<T, RES, 8, 4>(a, p, s);
         case 32:
@@ -1319,7 +1320,7 @@ static int launch_strip_res(const pgpe_rk4_args* a, const Rk4StepParams<T>& p, h
                 return launch_strip_r<T, RES, 8, 6>(a, p, s);
             }
             return fail(PGPE_ERR_ARG, "pgpe_rk4_step: strip heights 32 and 40 are single precision only");
-        default: return fail(PGPE_ERR_ARG, "pgpe_rk4_step: strip height (tile_y) must be 16, 24, 32 or 40");
+        default: return fail(PGPE_ERR_ARG, "pgpe_rk4_step: strip height (tile_y) must be 8, 16, 24, 32 or 40");
     }
 }
 

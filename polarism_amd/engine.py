@@ -71,8 +71,8 @@ class Rk4Engine:
         if self.strip_h is not None:
             if not self.whole_step:
                 raise ValueError("row strips run the whole-step kernel (whole_step=True)")
-            if self.strip_h not in (16, 24, 32, 40) or (self.strip_h > 24 and precision != "single"):
-                raise ValueError("strip_h must be 16 or 24 (or 32 and 40 in single precision)")
+            if self.strip_h not in (8, 16, 24, 32, 40) or (self.strip_h > 24 and precision != "single"):
+                raise ValueError("strip_h must be 8, 16 or 24 (or 32 and 40 in single precision)")
             # the strip kernel addresses a member with 32-bit byte offsets
             if any(s.nx * s.ny > STRIP_MAX_POINTS for s in table.specs):
                 raise ValueError(f"row strips take members of at most {STRIP_MAX_POINTS} points")

@@ -19,6 +19,7 @@ def test_candidates_respect_kernel_limits():
     assert LaunchChoice("strip", 40) in candidates("single", 1)
     assert LaunchChoice("strip", 40) not in candidates("double", 1)
     assert LaunchChoice("strip", 24) in candidates("double", 1)
+    assert LaunchChoice("strip", 8) in candidates("double", 1) and LaunchChoice("strip", 8) in candidates("single", 1)
 
 
 def test_choice_geometry():

@@ -119,7 +119,8 @@ def test_strips_equal_stage_launches_bitwise(reservoir, precision):
     _compare(e_stage, e_strip)
 
 
-@pytest.mark.parametrize("strip_h,precision", [(24, "single"), (32, "single"), (40, "single"), (24, "double")])
+@pytest.mark.parametrize("strip_h,precision", [(8, "single"), (8, "double"), (24, "single"), (32, "single"), (40, "single"),
+                                              (24, "double")])
 def test_strip_heights_and_graph_chunks(strip_h, precision):
     """Graph-captured chunks plus a trailing single step, Wigner noise fused in."""
     e_stage, e_strip, coef = _engines(SIZES, "single", precision, 16, noise_strength=1.0, strip_h=strip_h)
