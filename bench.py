@@ -41,6 +41,8 @@ Extra legs (reported beside the headline, not part of value):
               (grid-point-stage updates/s).
   quickstart: the reference quickstart (rk4-fdm, 128^2, complex128, 2000
               steps of 1e-3 ps), steps/s.
+  spinor:     spinor RK4 on one 1024^2 complex64 member: the whole-step
+              kernel against the four stage launches (us per step).
   fft:        split-step FFT (CPU operator order, 4 transforms per step) and
               the K-outer order (2 per step), 512 x 512, single reservoir,
               Wigner noise seed 1234, dt 5e-3 ps, complex64 (steps/s and FFT
@@ -293,6 +295,39 @@ def uneven_leg(steps=2000, seed=4):
             "wall_ms": round(ms, 1), "point_steps_per_s": round(pt_steps / (ms * 1e-3), 1),
             "point_stage_updates_per_s": round(4 * pt_steps / (ms * 1e-3), 1),
             "effective_GBps_224B_model": round(224.0 * pt_steps / (ms * 1e-3) / 1e9, 1)}
+
+
+def spinor_leg(n=1024, steps=20):
+    """Spinor RK4 on one n^2 member, complex64, TE-TM 0.05, Zeeman 0.02, alpha_2 -1e-4, single
+    reservoir per component, periodic grid: the whole-step kernel (one launch per step) against
+    the four stage launches, each as graph-launched steps after one warm-up replay of the same
+    graph (us per step)."""
+    from polarism_amd.config import PhysicsConstants
+    from polarism_amd.device import DeviceArray, Event
+    from polarism_amd.engine import Rk4Engine
+    from polarism_amd.layout import MemberSpec, MemberTable
+    p = PhysicsConstants()
+    p.te_tm, p.zeeman, p.alpha_2 = 0.05, 0.02, -1e-4
+    rng = np.random.default_rng(5)
+    spec = MemberSpec(nx=n, ny=n, dx=0.3, dy=0.3, physics=p, reservoir="single", components=2, res_components=2,
+                      dt=0.05, circular_polarization=0.3)
+    pump = DeviceArray.from_numpy(rng.uniform(0.0, 1.0, n * n).astype(np.float32))
+    coef = DeviceArray.from_numpy(np.full(4, 0.05, dtype=np.float32))
+    psi = (0.3 * (rng.standard_normal(2 * n * n) + 1j * rng.standard_normal(2 * n * n))).astype(np.complex64)
+    out = {"n": n, "components": 2, "res_components": 2}
+    for name, whole in (("whole_step", True), ("stage_launches", False)):
+        e = Rk4Engine(MemberTable([spec], 16), pump, "single", "single", whole_step=whole)
+        e.set_dt(0.05)
+        e.psi[0].upload(psi)
+        e.run_steps(steps, coef, True, False, 0)
+        start, stop = Event(), Event()
+        start.record(e.stream)
+        e.run_steps(steps, coef, True, False, 0)
+        stop.record(e.stream)
+        e.stream.synchronize()
+        out[name + "_us_per_step"] = round(start.elapsed_ms(stop) * 1e3 / steps, 2)
+    out["speedup"] = round(out["stage_launches_us_per_step"] / out["whole_step_us_per_step"], 3)
+    return out
 
 
 def quickstart_leg(steps=2000):
@@ -752,7 +787,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--members", type=int, default=SWEEP["members"])
-    ap.add_argument("--legs", default="sweep,copy,roofline,uneven,quickstart,fft,ffthbm,lawson,fno,cpu")
+    ap.add_argument("--legs", default="sweep,copy,roofline,uneven,quickstart,spinor,fft,ffthbm,lawson,fno,cpu")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--dump", default=None, help="write per-member stop times and density errors to this .npz")
     ap.add_argument("--max-time", type=float, default=SWEEP["max_time"],
@@ -827,6 +862,9 @@ def main():
     if "quickstart" in legs:
         result["quickstart_rk4_fp64"] = quickstart_leg()
         progress("quickstart leg done")
+    if "spinor" in legs:
+        result["spinor_rk4_1024"] = spinor_leg()
+        progress("spinor leg done")
     if "fft" in legs:
         result["split_step_fft"] = fft_leg()
         progress("fft leg done")
